@@ -1,0 +1,134 @@
+// agal_common.hpp -- compile-time velocity sets and small device helpers for the gfx950 kernels.
+//
+// Direction order, weights and opposite directions are the reference's: V_CONN_ID / V_CONN_MAP
+// (inc/util.h:94-101), LBMw / LBMpb (src/solver_lbm/solver_lbm_init.cu:79-88).  They are
+// constexpr here so every unrolled direction loop sees its velocity as an immediate.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace agal {
+
+constexpr int N_SKIPID = -999;
+constexpr int REF_UNREFINED = 0, REF_UNREFINED_VIO = 9, REF_REFINED = 1, REF_REFINED_WCHILD = 2,
+              REF_REFINED_PERM = 3, REF_REFINED_WCHILD_PERM = 10, REF_MARK_REFINE = 4,
+              REF_MARK_COARSEN = 5, REF_NEW = 6, REF_REMOVE = 7, REF_INACTIVE = 8;
+constexpr int CELL_INTERIOR = 0, CELL_INTERFACE = 1, CELL_GHOST = 2, CELL_SOLID = -1;
+constexpr int BLK_REGULAR = 0, BLK_INTERFACE = 1, BLK_SOLID = -3, BLK_SOLIDA = -2;
+
+// D3Q27 order; D3Q19 is its first 19 entries (inc/util.h:95).
+__host__ __device__ constexpr int c3x(int p)
+{
+    constexpr int t[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
+    return t[p];
+}
+__host__ __device__ constexpr int c3y(int p)
+{
+    constexpr int t[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1};
+    return t[p];
+}
+__host__ __device__ constexpr int c3z(int p)
+{
+    constexpr int t[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1};
+    return t[p];
+}
+// D2Q9 (inc/util.h:94).
+__host__ __device__ constexpr int c2x(int p)
+{
+    constexpr int t[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    return t[p];
+}
+__host__ __device__ constexpr int c2y(int p)
+{
+    constexpr int t[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    return t[p];
+}
+
+template <int Q>
+struct VelSet;
+
+template <>
+struct VelSet<9> {
+    static constexpr int Q = 9, DIM = 2, QMAX = 9, CELLS = 16, CHILDREN = 4;
+    __host__ __device__ static constexpr int cx(int p) { return c2x(p); }
+    __host__ __device__ static constexpr int cy(int p) { return c2y(p); }
+    __host__ __device__ static constexpr int cz(int) { return 0; }
+    __host__ __device__ static constexpr double w(int p) { return p == 0 ? 4.0 / 9.0 : (p < 5 ? 1.0 / 9.0 : 1.0 / 36.0); }
+    __host__ __device__ static constexpr int pb(int p)
+    {
+        constexpr int t[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+        return t[p];
+    }
+    // Directions that initiate the swap in the reference stream (solver_lbm_kernels.cu:597).
+    __host__ __device__ static constexpr bool half(int p) { return p == 1 || (p + 1) % 3 == 0; }
+};
+
+template <>
+struct VelSet<19> {
+    static constexpr int Q = 19, DIM = 3, QMAX = 27, CELLS = 64, CHILDREN = 8;
+    __host__ __device__ static constexpr int cx(int p) { return c3x(p); }
+    __host__ __device__ static constexpr int cy(int p) { return c3y(p); }
+    __host__ __device__ static constexpr int cz(int p) { return c3z(p); }
+    __host__ __device__ static constexpr double w(int p) { return p == 0 ? 1.0 / 3.0 : (p < 7 ? 1.0 / 18.0 : 1.0 / 36.0); }
+    __host__ __device__ static constexpr int pb(int p) { return p == 0 ? 0 : (p % 2 == 1 ? p + 1 : p - 1); }
+    __host__ __device__ static constexpr bool half(int p) { return p == 26 || ((p - 1) % 2 == 0 && p < 25); }
+};
+
+template <>
+struct VelSet<27> {
+    static constexpr int Q = 27, DIM = 3, QMAX = 27, CELLS = 64, CHILDREN = 8;
+    __host__ __device__ static constexpr int cx(int p) { return c3x(p); }
+    __host__ __device__ static constexpr int cy(int p) { return c3y(p); }
+    __host__ __device__ static constexpr int cz(int p) { return c3z(p); }
+    __host__ __device__ static constexpr double w(int p)
+    {
+        return p == 0 ? 8.0 / 27.0 : (p < 7 ? 2.0 / 27.0 : (p < 19 ? 1.0 / 54.0 : 1.0 / 216.0));
+    }
+    __host__ __device__ static constexpr int pb(int p) { return p == 0 ? 0 : (p % 2 == 1 ? p + 1 : p - 1); }
+    __host__ __device__ static constexpr bool half(int p) { return p == 26 || ((p - 1) % 2 == 0 && p < 25); }
+};
+
+// Velocity index of a neighbour offset (dx,dy,dz) in {-1,0,1}^3, D3Q27 order (V_CONN_MAP).
+__host__ __device__ constexpr int dir3(int dx, int dy, int dz)
+{
+    constexpr int m[27] = {20, 12, 26, 10, 6, 15, 24, 17, 21, 8, 4, 13, 2, 0, 1, 14, 3, 7, 22, 18, 23, 16, 5, 9, 25, 11, 19};
+    return m[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
+}
+// D2Q9 order.
+__host__ __device__ constexpr int dir2(int dx, int dy)
+{
+    constexpr int m[9] = {7, 4, 8, 3, 0, 1, 6, 2, 5};
+    return m[(dx + 1) + 3 * (dy + 1)];
+}
+
+// Boundary-condition presets of the reference hook Cu_ImposeBC (inc/custom.h:66-123).
+enum BcPreset { BC_NONE = 0, BC_LDC2D = 1, BC_LDC3D = 2, BC_FPSC2D = 3, BC_FPSC3D = 4 };
+
+template <typename T>
+__device__ __forceinline__ T bc_hook(int preset, T U, int nbr_id, T f, T u, T v, T w, T wp, T cxp, T cyp, T czp)
+{
+    bool wall = false, outflow = false;
+    if (preset == BC_LDC2D) wall = (nbr_id == -4);
+    if (preset == BC_LDC3D) wall = (nbr_id == -6);
+    if (preset == BC_FPSC2D) {
+        wall = (nbr_id == -1 || nbr_id == -3 || nbr_id == -4);
+        outflow = (nbr_id == -2);
+    }
+    if (preset == BC_FPSC3D) {
+        wall = (nbr_id == -1 || nbr_id == -3 || nbr_id == -4 || nbr_id == -5 || nbr_id == -6);
+        outflow = (nbr_id == -2);
+    }
+    if (wall) {
+        T cdotu = (T)(3.0) * (cxp * U);
+        return f - (T)(2.0) * wp * cdotu;
+    }
+    if (outflow) {
+        T cdotu = cxp * u + cyp * v + czp * w;
+        cdotu = (T)(1.0) + (T)(4.5) * cdotu * cdotu - (T)(1.5) * (u * u + v * v + w * w);
+        return -f + (T)(2.0) * wp * cdotu;
+    }
+    return f;
+}
+
+}  // namespace agal

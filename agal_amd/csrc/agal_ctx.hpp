@@ -1,0 +1,105 @@
+// agal_ctx.hpp -- device state of one mesh + LBM solver on one GPU.
+//
+// HBM layout (all structure-of-arrays, 256-byte aligned allocations):
+//   F[0], F[1]   two population buffers, Q rows x n_maxcells (row q = direction q, physical order);
+//                level L reads F[cur[L]] and writes F[cur[L]^1] in its fused step.
+//   M            rho,u,v,w rows x n_maxcells, written only by explicit refresh calls.
+//   cells_mask   n_maxcells ints.
+//   nbr, nbr_child  Q_max rows x n_maxcblocks ints, the reference's velocity-ordered tables
+//                (inc/mesh.h:397-399); nbr_slot is the same table in 3x3x3 slot order, one row of
+//                Q_max ints per block, for the fused kernel's lane-parallel load.
+//   ref, level, onb, blk_mask, part  n_maxcblocks ints; X  DIM rows x n_maxcblocks floats.
+//   id_set       MAX_LEVELS rows x n_maxcblocks (sorted ascending per level), gap_set n_maxcblocks.
+//   iface        per level: blocks of id_set[L] whose block mask is INTERFACE.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/agal_capi.h"
+
+namespace agal {
+
+struct LevelTiming {
+    bool enabled = false;
+    double fused_ms = 0;
+    long long fused_launches = 0;
+    double fused_cells = 0;
+    std::vector<hipEvent_t> ev_pairs;  // start/stop events of fused launches awaiting readout
+};
+
+struct Ctx {
+    agal_params P{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+
+    // Geometry of the pool.
+    int Q = 19, DIM = 3, QMAX = 27, CELLS = 64, NCH = 8;
+    int precision = AGAL_FP32;
+    size_t tsize = 4;
+    long long n_maxcblocks = 0, n_maxcells = 0;
+    int Nxi[3] = {0, 0, 0};
+    int MAX_LEVELS = 1, MAX_LEVELS_INTERIOR = 1, MAX_LEVELS_WALL = 1, N_LEVEL_START = 0;
+    int n_coarsecblocks = 0;
+    double dx = 0, dx_cblock = 0;         // stored in the precision's type on the host side
+    std::vector<double> dxf_vec, tau_vec; // exact values of the precision type, widened
+    bool ic_done = false;
+
+    // Host mirrors of the pool bookkeeping (inc/mesh.h:447-462).
+    std::vector<int> n_ids, id_max, n_iface;
+    int n_gaps = 0;
+    std::vector<int> cur;  // current population buffer per level
+    int global_Liter = 0;
+    unsigned stamp = 0;    // generation counter of the rebuild's neighbourhood map
+
+    // Device arrays.
+    void *F[2] = {nullptr, nullptr};
+    void *M = nullptr;
+    int *cells_mask = nullptr;
+    float *X = nullptr;
+    int *nbr = nullptr, *nbr_child = nullptr, *nbr_slot = nullptr;
+    int *ref = nullptr, *level = nullptr, *onb = nullptr, *blk_mask = nullptr, *part = nullptr;
+    int *simple = nullptr;
+    int *id_set = nullptr, *gap_set = nullptr, *iface = nullptr;
+    // Rebuild scratch.
+    unsigned *map_stamp = nullptr;
+    int *first_child = nullptr, *scan_a = nullptr, *scan_b = nullptr, *partials = nullptr;
+    int *dscal = nullptr;     // small device scalars (counters)
+    int *hscal = nullptr;     // pinned host mirror
+    // Exceptions of the stream guard: (initiator cell, partner cell, direction) + value.
+    long long exc_cap = 0;
+    int *exc_idx = nullptr;   // 3 ints per entry
+    void *exc_val = nullptr;  // one value per entry
+    int *exc_count = nullptr; // [0] count, [1] overflow flag
+
+    LevelTiming timing;
+
+    int *id_set_L(int L) const { return id_set + (size_t)L * n_maxcblocks; }
+    int *iface_L(int L) const { return iface + (size_t)L * n_maxcblocks; }
+};
+
+// Records the HIP error text and returns AGAL_ERR_HIP.
+int hip_fail(Ctx *c, hipError_t e, const char *what);
+#define AGAL_HIP(c, call)                                   \
+    do {                                                    \
+        hipError_t _e = (call);                             \
+        if (_e != hipSuccess) return ::agal::hip_fail((c), _e, #call); \
+    } while (0)
+
+// Kernel launchers (lbm_kernels.hip).
+int launch_set_ic(Ctx *c, int L);
+int launch_refresh(Ctx *c, int L);
+int launch_collide_stream(Ctx *c, int L);
+int launch_interpolate(Ctx *c, int L, int var);
+int launch_average(Ctx *c, int L, int var);
+
+// Mesh kernels and the rebuild (amr_kernels.hip).
+int launch_ref_criteria(Ctx *c);
+int refine_coarsen(Ctx *c, int out_counts[4]);
+int rebuild_level_lists(Ctx *c, long long range_hint);
+int launch_fill(Ctx *c, int *p, long long n, int v);
+
+}  // namespace agal

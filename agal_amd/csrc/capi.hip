@@ -1,0 +1,881 @@
+// capi.hip -- the extern "C" entry points of include/agal_capi.h.
+//
+// Host orchestration only: building the root grid (Mesh::M_Init, src/mesh/mesh_init.cu:10-567),
+// the recursive sub-cycling order of Solver_LBM::S_Advance (solver_lbm_advance.cu:30-174), the
+// solution-based rebuild driver (mesh_advance.cu:184-243) and data exchange.  All arithmetic on
+// populations runs in the kernels of lbm_kernels.hip / amr_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "agal_common.hpp"
+#include "agal_ctx.hpp"
+
+using namespace agal;
+
+namespace agal {
+
+int hip_fail(Ctx *c, hipError_t e, const char *what)
+{
+    if (c) c->last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return AGAL_ERR_HIP;
+}
+
+// Element-wise gather (dir 0) or scatter (dir 1) between a block's current population buffer
+// and a contiguous staging array, Q rows of n_blocks*cells.
+template <typename T>
+__global__ void k_stage_pops(T *F0, T *F1, T *stage, long long ncm, long long n_blocks, int cells, int Q,
+                             const int *level, int4 cur_lo, int4 cur_hi, int dir)
+{
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    long long nc = n_blocks * cells;
+    if (i >= nc) return;
+    int b = (int)(i / cells);
+    int L = level[b];
+    int cl[8] = {cur_lo.x, cur_lo.y, cur_lo.z, cur_lo.w, cur_hi.x, cur_hi.y, cur_hi.z, cur_hi.w};
+    int k = (L >= 0 && L < 8) ? cl[L] : 0;
+    T *F = k ? F1 : F0;
+    for (int q = 0; q < Q; q++) {
+        if (dir == 0)
+            stage[q * nc + i] = F[q * ncm + i];
+        else
+            F[q * ncm + i] = stage[q * nc + i];
+    }
+}
+
+__global__ void k_init_gaps(int *gap, long long n, long long nmax)
+{
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) gap[i] = (int)(nmax - 1 - i);
+}
+
+__global__ void k_mark_uniform(const int *ids, int n, int *ref)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n && ref[ids[i]] == REF_UNREFINED) ref[ids[i]] = REF_MARK_REFINE;
+}
+
+__global__ void k_mark_list(const int *ids, int n, int *ref, int mark)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) ref[ids[i]] = mark;
+}
+
+__global__ void k_freeze(int *ref, int n, int var)
+{
+    int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    int r = ref[k];
+    if (var == 0) {
+        if (r == REF_REFINED) ref[k] = REF_REFINED_PERM;
+        if (r == REF_REFINED_WCHILD) ref[k] = REF_REFINED_WCHILD_PERM;
+    } else {
+        if (r == REF_REFINED_PERM) ref[k] = REF_REFINED;
+        if (r == REF_REFINED_WCHILD_PERM) ref[k] = REF_REFINED_WCHILD;
+    }
+}
+
+}  // namespace agal
+
+namespace {
+
+// Level spacings and relaxation times in the precision's type, computed the way the reference
+// does (mesh_init.cu:22-29,218-220; solver_lbm_init.cu:47-52).
+template <typename T>
+void compute_level_params(Ctx *c)
+{
+    float Lx = (float)c->P.L_c;
+    T dx = (T)(Lx / c->P.Nx);
+    T dxc = 4 * dx;
+    c->dx = (double)dx;
+    c->dx_cblock = (double)dxc;
+    c->dxf_vec.assign(c->MAX_LEVELS, 0.0);
+    c->tau_vec.assign(c->MAX_LEVELS, 0.0);
+    for (int L = 0; L < c->MAX_LEVELS; L++) {
+        T dxf = (T)((double)dx / std::pow(2.0, (double)(T)L));
+        c->dxf_vec[L] = (double)dxf;
+        c->tau_vec[L] = (double)(T)(c->P.v0 * 3.0 + 0.5 * (double)dxf);
+    }
+}
+
+void free_ctx(Ctx *c)
+{
+    if (!c) return;
+    void *ptrs[] = {c->F[0], c->F[1], c->M, c->cells_mask, c->X, c->nbr, c->nbr_child, c->nbr_slot, c->ref, c->level,
+                    c->onb, c->blk_mask, c->part, c->simple, c->id_set, c->gap_set, c->iface, c->map_stamp,
+                    c->first_child, c->scan_a, c->scan_b, c->partials, c->dscal, c->exc_idx, c->exc_val,
+                    c->exc_count};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->hscal) (void)hipHostFree(c->hscal);
+    for (hipEvent_t e : c->timing.ev_pairs) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+template <typename P>
+int dalloc(Ctx *c, P **p, size_t bytes)
+{
+    void *v = nullptr;
+    hipError_t e = hipMalloc(&v, std::max<size_t>(bytes, 256));
+    if (e != hipSuccess) {
+        c->last_error = std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e);
+        return AGAL_ERR_ALLOC;
+    }
+    *p = static_cast<P *>(v);
+    return AGAL_OK;
+}
+
+// Root grid of Mesh::M_Init (mesh_init.cu:325-516): block IDs in x-fastest order over a padded
+// index grid whose rim holds the face IDs (-1..-6, x overriding y overriding z on edges) or the
+// periodic images.
+int build_root(Ctx *c)
+{
+    const int DIM = c->DIM, QM = c->QMAX;
+    const long long nmax = c->n_maxcblocks;
+    const int bx = c->Nxi[0] / 4, by = c->Nxi[1] / 4;
+    const int gh = DIM == 2 ? 1 : c->Nxi[2] / 4;
+    const int ghm = DIM == 2 ? 0 : 1;
+    const int px = bx + 2, py = by + 2, pz = gh + 2;
+    std::vector<int> grid((size_t)px * py * pz, 0);
+    auto G = [&](int i, int j, int k) -> int & { return grid[(size_t)i + px * ((size_t)j + py * k)]; };
+    int id = 0;
+    for (int k = ghm; k < gh * ghm + 1; k++)
+        for (int j = 1; j < by + 1; j++)
+            for (int i = 1; i < bx + 1; i++) G(i, j, k) = id++;
+    if (DIM == 3)
+        for (int j = 0; j < py; j++)
+            for (int i = 0; i < px; i++) {
+                if (c->P.PERIODIC_Z == 1) {
+                    G(i, j, 0) = G(i, j, pz - 2);
+                    G(i, j, pz - 1) = G(i, j, 1);
+                } else {
+                    G(i, j, 0) = -5;
+                    G(i, j, pz - 1) = -6;
+                }
+            }
+    for (int k = 0; k < gh + 2 * ghm; k++)
+        for (int i = 0; i < px; i++) {
+            if (c->P.PERIODIC_Y == 1) {
+                G(i, 0, k) = G(i, py - 2, k);
+                G(i, py - 1, k) = G(i, 1, k);
+            } else {
+                G(i, 0, k) = -3;
+                G(i, py - 1, k) = -4;
+            }
+        }
+    for (int k = 0; k < gh + 2 * ghm; k++)
+        for (int j = 0; j < py; j++) {
+            if (c->P.PERIODIC_X == 1) {
+                G(0, j, k) = G(px - 2, j, k);
+                G(px - 1, j, k) = G(1, j, k);
+            } else {
+                G(0, j, k) = -1;
+                G(px - 1, j, k) = -2;
+            }
+        }
+    const int nc = c->n_coarsecblocks;
+    std::vector<int> nbr((size_t)QM * nc), nbr_child((size_t)QM * nc), onb(nc), ref(nc, REF_UNREFINED);
+    std::vector<float> X((size_t)DIM * nc);
+    for (int k = ghm; k < gh * ghm + 1; k++)
+        for (int j = 1; j < by + 1; j++)
+            for (int i = 1; i < bx + 1; i++) {
+                int b = G(i, j, k);
+                auto coord = [&](int n) {
+                    if (c->precision == AGAL_FP64) return (float)((double)n * c->dx_cblock);
+                    return (float)((float)n * (float)c->dx_cblock);
+                };
+                X[b + 0 * (size_t)nc] = coord(i - 1);
+                X[b + 1 * (size_t)nc] = coord(j - 1);
+                if (DIM == 3) X[b + 2 * (size_t)nc] = coord(k - 1);
+                int ob = 0;
+                for (int p = 0; p < QM; p++) {
+                    int cx = DIM == 3 ? c3x(p) : c2x(p), cy = DIM == 3 ? c3y(p) : c2y(p), cz = DIM == 3 ? c3z(p) : 0;
+                    int nb = G(i + cx, j + cy, ghm * (k + cz));
+                    nbr[b + (size_t)p * nc] = nb;
+                    if (nb < 0) ob = 1;
+                    nbr_child[b + (size_t)p * nc] = nb < 0 ? nb : N_SKIPID;
+                }
+                onb[b] = ob;
+            }
+    for (int p = 0; p < QM; p++) {
+        AGAL_HIP(c, hipMemcpy(c->nbr + p * nmax, nbr.data() + (size_t)p * nc, sizeof(int) * nc, hipMemcpyHostToDevice));
+        AGAL_HIP(c, hipMemcpy(c->nbr_child + p * nmax, nbr_child.data() + (size_t)p * nc, sizeof(int) * nc, hipMemcpyHostToDevice));
+    }
+    for (int d = 0; d < DIM; d++)
+        AGAL_HIP(c, hipMemcpy(c->X + d * nmax, X.data() + (size_t)d * nc, sizeof(float) * nc, hipMemcpyHostToDevice));
+    AGAL_HIP(c, hipMemcpy(c->onb, onb.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+    AGAL_HIP(c, hipMemcpy(c->ref, ref.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+    return AGAL_OK;
+}
+
+int advance_rec(Ctx *c, int L)
+{
+    int st;
+    const int ML = c->MAX_LEVELS, L0 = c->N_LEVEL_START;
+    if (ML > 1 && ML != L0 + 1) {
+        if (L == L0) {
+            c->global_Liter = 0;
+            if ((st = launch_interpolate(c, L, AGAL_INTERP_INTERFACE))) return st;
+            if ((st = launch_collide_stream(c, L))) return st;
+            if ((st = advance_rec(c, L + 1))) return st;
+            if ((st = launch_average(c, L, AGAL_AVERAGE_INTERFACE))) return st;
+        } else {
+            if (L < ML - 1)
+                if ((st = launch_interpolate(c, L, AGAL_INTERP_INTERFACE))) return st;
+            if ((st = launch_collide_stream(c, L))) return st;
+            if (L == ML - 1) c->global_Liter++;
+            if (L < ML - 1) {
+                if ((st = advance_rec(c, L + 1))) return st;
+                if ((st = launch_average(c, L, AGAL_AVERAGE_INTERFACE))) return st;
+                if ((st = launch_interpolate(c, L, AGAL_INTERP_INTERFACE))) return st;
+            }
+            if ((st = launch_collide_stream(c, L))) return st;
+            if (L == ML - 1) c->global_Liter++;
+            if (L < ML - 1) {
+                if ((st = advance_rec(c, L + 1))) return st;
+                if ((st = launch_average(c, L, AGAL_AVERAGE_INTERFACE))) return st;
+            }
+        }
+    } else {
+        if ((st = launch_collide_stream(c, L))) return st;
+    }
+    return AGAL_OK;
+}
+
+int refine_with_solution_impl(Ctx *c, int out_counts[4])
+{
+    int st;
+    if (!(c->MAX_LEVELS_INTERIOR > 1)) {
+        if (out_counts) std::fill(out_counts, out_counts + 4, 0);
+        return AGAL_OK;
+    }
+    // The reference's S_Average / S_Interpolate refresh the macroscopic fields first
+    // (solver_lbm_kernels_comm.cu:143,354); the criterion reads those stored fields.
+    for (int L = c->MAX_LEVELS - 2; L >= c->N_LEVEL_START; L--) {
+        if ((st = launch_refresh(c, L + 1))) return st;
+        if ((st = launch_average(c, L, AGAL_AVERAGE_GRID))) return st;
+    }
+    for (int L = c->N_LEVEL_START; L < c->MAX_LEVELS - 1; L++) {
+        if ((st = launch_refresh(c, L))) return st;
+        if ((st = launch_interpolate(c, L, AGAL_INTERP_INTERFACE))) return st;
+    }
+    if ((st = launch_ref_criteria(c))) return st;
+    return refine_coarsen(c, out_counts);
+}
+
+int check_exceptions(Ctx *c)
+{
+    int v[2] = {0, 0};
+    AGAL_HIP(c, hipMemcpyAsync(v, c->exc_count, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    if (v[1]) {
+        c->last_error = "stream-guard exception list overflowed";
+        return AGAL_ERR_OVERFLOW;
+    }
+    return AGAL_OK;
+}
+
+int collect_timing(Ctx *c)
+{
+    auto &tm = c->timing;
+    if (tm.ev_pairs.empty()) return AGAL_OK;
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k + 1 < tm.ev_pairs.size(); k += 2) {
+        float ms = 0;
+        AGAL_HIP(c, hipEventElapsedTime(&ms, tm.ev_pairs[k], tm.ev_pairs[k + 1]));
+        tm.fused_ms += ms;
+        (void)hipEventDestroy(tm.ev_pairs[k]);
+        (void)hipEventDestroy(tm.ev_pairs[k + 1]);
+    }
+    tm.ev_pairs.clear();
+    return AGAL_OK;
+}
+
+}  // namespace
+
+#define CTX_OR_ARG(ctx) \
+    if (!(ctx)) return AGAL_ERR_ARG;
+
+extern "C" {
+
+void agal_params_default(agal_params *p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->velocity_set = AGAL_D3Q19;
+    p->precision = AGAL_FP32;
+    p->collision = AGAL_BGK;
+    p->interp_order = AGAL_INTERP_LINEAR;
+    p->Nx = 64;
+    p->L_c = p->L_fy = p->L_fz = 1.0;
+    p->MAX_LEVELS_WALL = 5;
+    p->MAX_LEVELS_INTERIOR = 4;
+    p->N_LEVEL_START = 0;
+    p->v0 = 1.5625e-5;
+    p->S_CRITERION = 0;
+    p->N_REFINE_START = -3.0;
+    p->N_REFINE_INC = 1.0;
+    p->N_REFINE_MAX = 1.0;
+    p->P_REFINE = 32;
+    p->bc_preset = AGAL_BC_LDC3D;
+    p->bc_velocity = 0.05;
+    p->ic_rho = 1.0;
+}
+
+const char *agal_status_string(int s)
+{
+    switch (s) {
+        case AGAL_OK: return "ok";
+        case AGAL_ERR_ARG: return "invalid argument";
+        case AGAL_ERR_HIP: return "HIP runtime error";
+        case AGAL_ERR_ALLOC: return "allocation failed";
+        case AGAL_ERR_CAPACITY: return "root grid exceeds the block pool";
+        case AGAL_ERR_POOL_EXHAUSTED: return "refinement cancelled: block pool exhausted";
+        case AGAL_ERR_STATE: return "call out of order";
+        case AGAL_ERR_OVERFLOW: return "internal list overflow";
+        case AGAL_ERR_NOT_BUILT: return "variant not built";
+    }
+    return "unknown status";
+}
+
+const char *agal_last_error(agal_ctx *ctx)
+{
+    if (!ctx) return "";
+    return reinterpret_cast<Ctx *>(ctx)->last_error.c_str();
+}
+
+int agal_create(const agal_params *p, long long block_capacity, int device, agal_ctx **out)
+{
+    if (!p || !out || block_capacity <= 0) return AGAL_ERR_ARG;
+    if (p->velocity_set != 9 && p->velocity_set != 19 && p->velocity_set != 27) return AGAL_ERR_ARG;
+    if (p->precision != AGAL_FP32 && p->precision != AGAL_FP64) return AGAL_ERR_ARG;
+    if (p->collision != AGAL_BGK || p->interp_order != AGAL_INTERP_LINEAR) return AGAL_ERR_NOT_BUILT;
+    if (p->Nx <= 0 || p->Nx % 4 != 0) return AGAL_ERR_ARG;
+    Ctx *c = new (std::nothrow) Ctx;
+    if (!c) return AGAL_ERR_ALLOC;
+    c->P = *p;
+    c->device = device;
+    c->Q = p->velocity_set;
+    c->DIM = c->Q == 9 ? 2 : 3;
+    c->QMAX = c->DIM == 2 ? 9 : 27;
+    c->CELLS = c->DIM == 2 ? 16 : 64;
+    c->NCH = c->DIM == 2 ? 4 : 8;
+    c->precision = p->precision;
+    c->tsize = p->precision == AGAL_FP64 ? 8 : 4;
+    float Lx = (float)p->L_c;
+    float Ly = (float)(p->L_fy * Lx), Lz = (float)(p->L_fz * Lx);
+    c->Nxi[0] = p->Nx;
+    c->Nxi[1] = (int)(p->Nx * (Ly / Lx));
+    c->Nxi[2] = c->DIM == 2 ? 1 : (int)(p->Nx * (Lz / Lx));
+    c->MAX_LEVELS_WALL = p->MAX_LEVELS_WALL;
+    c->MAX_LEVELS_INTERIOR = p->MAX_LEVELS_INTERIOR;
+    c->N_LEVEL_START = p->N_LEVEL_START;
+    c->MAX_LEVELS = std::max(p->MAX_LEVELS_WALL, p->MAX_LEVELS_INTERIOR);
+    if (c->MAX_LEVELS == 1) c->N_LEVEL_START = 0;
+    if (c->N_LEVEL_START > c->MAX_LEVELS - 1) c->N_LEVEL_START = c->MAX_LEVELS - 1;
+    if (c->MAX_LEVELS < 1 || c->MAX_LEVELS > 8 || c->Nxi[1] % 4 || c->Nxi[2] % (c->DIM == 3 ? 4 : 1) || c->P.P_REFINE <= 0) {
+        delete c;
+        return AGAL_ERR_ARG;
+    }
+    c->n_maxcblocks = block_capacity;
+    c->n_maxcells = block_capacity * c->CELLS;
+    c->n_coarsecblocks = (c->Nxi[0] * c->Nxi[1] * c->Nxi[2]) / c->CELLS;
+    if ((long long)c->Nxi[0] * c->Nxi[1] * c->Nxi[2] >= c->n_maxcells || c->n_maxcells >= (1LL << 31)) {
+        delete c;
+        return AGAL_ERR_CAPACITY;
+    }
+    if (p->precision == AGAL_FP64)
+        compute_level_params<double>(c);
+    else
+        compute_level_params<float>(c);
+    c->n_ids.assign(c->MAX_LEVELS + 1, 0);
+    c->n_iface.assign(c->MAX_LEVELS + 1, 0);
+    c->id_max.assign(c->MAX_LEVELS + 1, 0);
+    c->cur.assign(c->MAX_LEVELS, 0);
+
+    int st = AGAL_OK;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        st = hip_fail(c, e, "hipSetDevice");
+        free_ctx(c);
+        return st;
+    }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        st = hip_fail(c, e, "hipStreamCreate");
+        free_ctx(c);
+        return st;
+    }
+    const long long nmax = c->n_maxcblocks, ncm = c->n_maxcells;
+    const size_t ts = c->tsize;
+    c->exc_cap = std::min<long long>(ncm, 1LL << 22);
+#define ALLOC(ptr, bytes)                                        \
+    if ((st = dalloc(c, &(ptr), (bytes))) != AGAL_OK) {          \
+        free_ctx(c);                                             \
+        return st;                                               \
+    }
+    ALLOC(c->F[0], (size_t)c->Q * ncm * ts);
+    ALLOC(c->F[1], (size_t)c->Q * ncm * ts);
+    ALLOC(c->M, (size_t)4 * ncm * ts);
+    ALLOC(c->cells_mask, sizeof(int) * ncm);
+    ALLOC(c->X, sizeof(float) * c->DIM * nmax);
+    ALLOC(c->nbr, sizeof(int) * c->QMAX * nmax);
+    ALLOC(c->nbr_child, sizeof(int) * c->QMAX * nmax);
+    ALLOC(c->nbr_slot, sizeof(int) * c->QMAX * nmax);
+    ALLOC(c->ref, sizeof(int) * nmax);
+    ALLOC(c->level, sizeof(int) * nmax);
+    ALLOC(c->onb, sizeof(int) * nmax);
+    ALLOC(c->blk_mask, sizeof(int) * nmax);
+    ALLOC(c->part, sizeof(int) * nmax);
+    ALLOC(c->simple, sizeof(int) * nmax);
+    ALLOC(c->id_set, sizeof(int) * c->MAX_LEVELS * nmax);
+    ALLOC(c->iface, sizeof(int) * c->MAX_LEVELS * nmax);
+    ALLOC(c->gap_set, sizeof(int) * nmax);
+    ALLOC(c->map_stamp, sizeof(unsigned) * nmax);
+    ALLOC(c->first_child, sizeof(int) * nmax);
+    ALLOC(c->scan_a, sizeof(int) * nmax);
+    ALLOC(c->scan_b, sizeof(int) * nmax);
+    ALLOC(c->partials, sizeof(int) * ((nmax + 2047) / 2048 + 64));
+    ALLOC(c->dscal, sizeof(int) * 64);
+    ALLOC(c->exc_idx, sizeof(int) * 3 * c->exc_cap);
+    ALLOC(c->exc_val, ts * c->exc_cap);
+    ALLOC(c->exc_count, sizeof(int) * 4);
+#undef ALLOC
+    e = hipHostMalloc((void **)&c->hscal, sizeof(int) * 64, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        st = hip_fail(c, e, "hipHostMalloc");
+        free_ctx(c);
+        return st;
+    }
+    std::memset(c->hscal, 0, sizeof(int) * 64);
+    // Initial values (mesh_init.cu:284-304).
+    auto fill = [&](int *p, long long n, int v) { return launch_fill(c, p, n, v); };
+    if ((st = fill(c->cells_mask, ncm, CELL_INTERIOR)) || (st = fill(c->blk_mask, nmax, BLK_REGULAR)) ||
+        (st = fill(c->nbr, c->QMAX * nmax, N_SKIPID)) || (st = fill(c->nbr_child, c->QMAX * nmax, N_SKIPID)) ||
+        (st = fill(c->ref, nmax, REF_INACTIVE)) || (st = fill(c->level, nmax, 0)) || (st = fill(c->onb, nmax, 0)) ||
+        (st = fill((int *)c->map_stamp, nmax, 0)) || (st = fill(c->exc_count, 4, 0)) || (st = fill(c->part, nmax, 1)) ||
+        (st = fill(c->simple, nmax, 0)) || (st = fill((int *)c->X, c->DIM * nmax, 0))) {
+        free_ctx(c);
+        return st;
+    }
+    hipLaunchKernelGGL(k_init_gaps, dim3((unsigned)((nmax - c->n_coarsecblocks + 255) / 256 + 1)), dim3(256), 0,
+                       c->stream, c->gap_set, nmax - c->n_coarsecblocks, nmax);
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        st = hip_fail(c, e, "init");
+        free_ctx(c);
+        return st;
+    }
+    if ((st = build_root(c))) {
+        free_ctx(c);
+        return st;
+    }
+    c->n_gaps = (int)(nmax - c->n_coarsecblocks);
+    c->id_max[0] = c->id_max[c->MAX_LEVELS] = c->n_coarsecblocks;
+    if ((st = rebuild_level_lists(c, c->n_coarsecblocks))) {
+        free_ctx(c);
+        return st;
+    }
+    *out = reinterpret_cast<agal_ctx *>(c);
+    return AGAL_OK;
+}
+
+int agal_destroy(agal_ctx *ctx)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_ctx(c);
+    return AGAL_OK;
+}
+
+int agal_get_sizes(agal_ctx *ctx, agal_sizes *s)
+{
+    CTX_OR_ARG(ctx);
+    if (!s) return AGAL_ERR_ARG;
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    std::memset(s, 0, sizeof(*s));
+    s->n_dim = c->DIM;
+    s->n_q = c->Q;
+    s->n_q_max = c->QMAX;
+    s->cells_per_block = c->CELLS;
+    s->n_children = c->NCH;
+    s->max_levels = c->MAX_LEVELS;
+    s->n_maxcblocks = c->n_maxcblocks;
+    s->n_maxcells = c->n_maxcells;
+    s->n_coarsecblocks = c->n_coarsecblocks;
+    for (int L = 0; L < c->MAX_LEVELS; L++) s->n_ids[L] = c->n_ids[L];
+    for (int L = 0; L <= c->MAX_LEVELS; L++) s->id_max[L] = c->id_max[L];
+    s->n_gaps = c->n_gaps;
+    return AGAL_OK;
+}
+
+int agal_sync(agal_ctx *ctx)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    return check_exceptions(c);
+}
+
+int agal_set_ic(agal_ctx *ctx, int L)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (L < 0 || L >= c->MAX_LEVELS) return AGAL_ERR_ARG;
+    c->ic_done = true;
+    return launch_set_ic(c, L);
+}
+
+int agal_advance(agal_ctx *ctx, int L0)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (L0 < 0 || L0 >= c->MAX_LEVELS) return AGAL_ERR_ARG;
+    if (!c->ic_done) return AGAL_ERR_STATE;
+    return advance_rec(c, L0);
+}
+
+int agal_collide_stream(agal_ctx *ctx, int L)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (L < 0 || L >= c->MAX_LEVELS) return AGAL_ERR_ARG;
+    return launch_collide_stream(c, L);
+}
+
+int agal_interpolate(agal_ctx *ctx, int L, int var)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (L < 0 || L >= c->MAX_LEVELS || (var != AGAL_INTERP_INTERFACE && var != AGAL_INTERP_ADDED)) return AGAL_ERR_ARG;
+    if (L >= c->MAX_LEVELS - 1) return AGAL_OK;
+    int st = launch_refresh(c, L);
+    if (st) return st;
+    return launch_interpolate(c, L, var);
+}
+
+int agal_average(agal_ctx *ctx, int L, int var)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (L < 0 || L >= c->MAX_LEVELS || var < 0 || var > 2) return AGAL_ERR_ARG;
+    if (L >= c->MAX_LEVELS - 1) return AGAL_OK;
+    int st = launch_refresh(c, L + 1);
+    if (st) return st;
+    return launch_average(c, L, var);
+}
+
+int agal_refresh(agal_ctx *ctx, int L)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (L < 0 || L >= c->MAX_LEVELS) return AGAL_ERR_ARG;
+    return launch_refresh(c, L);
+}
+
+int agal_ref_criteria(agal_ctx *ctx)
+{
+    CTX_OR_ARG(ctx);
+    return launch_ref_criteria(reinterpret_cast<Ctx *>(ctx));
+}
+
+int agal_refine_coarsen(agal_ctx *ctx, int out_counts[4])
+{
+    CTX_OR_ARG(ctx);
+    return refine_coarsen(reinterpret_cast<Ctx *>(ctx), out_counts);
+}
+
+int agal_refine_with_solution(agal_ctx *ctx, int out_counts[4])
+{
+    CTX_OR_ARG(ctx);
+    return refine_with_solution_impl(reinterpret_cast<Ctx *>(ctx), out_counts);
+}
+
+int agal_mark_uniform(agal_ctx *ctx, int L)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (L < 0 || L >= c->MAX_LEVELS) return AGAL_ERR_ARG;
+    int n = c->n_ids[L];
+    if (n <= 0) return AGAL_OK;
+    hipLaunchKernelGGL(k_mark_uniform, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->id_set_L(L), n, c->ref);
+    AGAL_HIP(c, hipGetLastError());
+    return AGAL_OK;
+}
+
+int agal_mark_blocks(agal_ctx *ctx, const int *ids, int n, int mark)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (n < 0 || (n > 0 && !ids) || n > c->n_maxcblocks) return AGAL_ERR_ARG;
+    for (int k = 0; k < n; k++)
+        if (ids[k] < 0 || ids[k] >= c->n_maxcblocks) return AGAL_ERR_ARG;
+    if (n == 0) return AGAL_OK;
+    AGAL_HIP(c, hipMemcpyAsync(c->scan_a, ids, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_mark_list, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->scan_a, n, c->ref, mark);
+    AGAL_HIP(c, hipGetLastError());
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    return AGAL_OK;
+}
+
+int agal_freeze_refined(agal_ctx *ctx, int var)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    int n = c->id_max[c->MAX_LEVELS];
+    if (n <= 0) return AGAL_OK;
+    hipLaunchKernelGGL(k_freeze, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->ref, n, var);
+    AGAL_HIP(c, hipGetLastError());
+    return AGAL_OK;
+}
+
+int agal_advance_loop(agal_ctx *ctx, int i0, int n, double *out_step_ms, double *out_rebuild_ms)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c->ic_done) return AGAL_ERR_STATE;
+    if (n < 0) return AGAL_ERR_ARG;
+    hipEvent_t e0, e1;
+    AGAL_HIP(c, hipEventCreate(&e0));
+    AGAL_HIP(c, hipEventCreate(&e1));
+    double ts = 0, tr = 0;
+    int status = AGAL_OK, st;
+    const int iter_mult = 1 << c->N_LEVEL_START;
+    const int ML = c->MAX_LEVELS;
+    AGAL_HIP(c, hipEventRecord(e0, c->stream));
+    for (int i = i0; i < i0 + n; i++) {
+        for (int j = 0; j < iter_mult; j++)
+            if ((st = advance_rec(c, c->N_LEVEL_START))) return st;
+        if (ML > 1 && ML != c->N_LEVEL_START + 1 && (i + 1) % c->P.P_REFINE == 0 && i > 0) {
+            float ms = 0;
+            AGAL_HIP(c, hipEventRecord(e1, c->stream));
+            AGAL_HIP(c, hipEventSynchronize(e1));
+            AGAL_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+            ts += ms;
+            auto t0 = std::chrono::steady_clock::now();
+            st = refine_with_solution_impl(c, nullptr);
+            if (st == AGAL_ERR_POOL_EXHAUSTED)
+                status = st;
+            else if (st)
+                return st;
+            AGAL_HIP(c, hipStreamSynchronize(c->stream));
+            tr += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            AGAL_HIP(c, hipEventRecord(e0, c->stream));
+        }
+    }
+    {
+        float ms = 0;
+        AGAL_HIP(c, hipEventRecord(e1, c->stream));
+        AGAL_HIP(c, hipEventSynchronize(e1));
+        AGAL_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+        ts += ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if ((st = check_exceptions(c))) return st;
+    if (out_step_ms) *out_step_ms = ts;
+    if (out_rebuild_ms) *out_rebuild_ms = tr;
+    return status;
+}
+
+// ---- data exchange ---------------------------------------------------------------------------
+static int stage_pops(Ctx *c, void *host, long long n_blocks, long long stride, int dir)
+{
+    if (n_blocks < 0 || n_blocks > c->n_maxcblocks) return AGAL_ERR_ARG;
+    if (n_blocks == 0) return AGAL_OK;
+    const long long nc = n_blocks * c->CELLS;
+    if (stride < nc) return AGAL_ERR_ARG;
+    void *stage = nullptr;
+    AGAL_HIP(c, hipMalloc(&stage, (size_t)c->Q * nc * c->tsize));
+    int cl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int L = 0; L < c->MAX_LEVELS; L++) cl[L] = c->cur[L];
+    int4 lo = make_int4(cl[0], cl[1], cl[2], cl[3]), hi = make_int4(cl[4], cl[5], cl[6], cl[7]);
+    int rc = AGAL_OK;
+    auto body = [&]() -> int {
+        const char *h = static_cast<const char *>(host);
+        if (dir == 1)
+            for (int q = 0; q < c->Q; q++)
+                AGAL_HIP(c, hipMemcpyAsync((char *)stage + (size_t)q * nc * c->tsize, h + (size_t)q * stride * c->tsize,
+                                           nc * c->tsize, hipMemcpyHostToDevice, c->stream));
+        unsigned grid = (unsigned)((nc + 255) / 256);
+        if (c->precision == AGAL_FP64)
+            hipLaunchKernelGGL(k_stage_pops<double>, dim3(grid), dim3(256), 0, c->stream, (double *)c->F[0], (double *)c->F[1],
+                               (double *)stage, c->n_maxcells, n_blocks, c->CELLS, c->Q, c->level, lo, hi, dir);
+        else
+            hipLaunchKernelGGL(k_stage_pops<float>, dim3(grid), dim3(256), 0, c->stream, (float *)c->F[0], (float *)c->F[1],
+                               (float *)stage, c->n_maxcells, n_blocks, c->CELLS, c->Q, c->level, lo, hi, dir);
+        AGAL_HIP(c, hipGetLastError());
+        if (dir == 0)
+            for (int q = 0; q < c->Q; q++)
+                AGAL_HIP(c, hipMemcpyAsync((char *)host + (size_t)q * stride * c->tsize, (char *)stage + (size_t)q * nc * c->tsize,
+                                           nc * c->tsize, hipMemcpyDeviceToHost, c->stream));
+        AGAL_HIP(c, hipStreamSynchronize(c->stream));
+        return AGAL_OK;
+    };
+    rc = body();
+    (void)hipFree(stage);
+    return rc;
+}
+
+int agal_download_populations(agal_ctx *ctx, void *dst, long long n_blocks, long long dst_stride)
+{
+    CTX_OR_ARG(ctx);
+    if (!dst) return AGAL_ERR_ARG;
+    return stage_pops(reinterpret_cast<Ctx *>(ctx), dst, n_blocks, dst_stride, 0);
+}
+
+int agal_upload_populations(agal_ctx *ctx, const void *src, long long n_blocks, long long src_stride)
+{
+    CTX_OR_ARG(ctx);
+    if (!src) return AGAL_ERR_ARG;
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    c->ic_done = true;
+    return stage_pops(c, const_cast<void *>(src), n_blocks, src_stride, 1);
+}
+
+int agal_download_macros(agal_ctx *ctx, void *dst, long long n_blocks, long long dst_stride)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!dst || n_blocks < 0 || n_blocks > c->n_maxcblocks) return AGAL_ERR_ARG;
+    const long long nc = n_blocks * c->CELLS;
+    if (dst_stride < nc) return AGAL_ERR_ARG;
+    for (int k = 0; k < 4; k++)
+        AGAL_HIP(c, hipMemcpyAsync((char *)dst + (size_t)k * dst_stride * c->tsize,
+                                   (char *)c->M + (size_t)k * c->n_maxcells * c->tsize, nc * c->tsize,
+                                   hipMemcpyDeviceToHost, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    return AGAL_OK;
+}
+
+int agal_download_cell_masks(agal_ctx *ctx, int *dst, long long n_blocks)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!dst || n_blocks < 0 || n_blocks > c->n_maxcblocks) return AGAL_ERR_ARG;
+    AGAL_HIP(c, hipMemcpyAsync(dst, c->cells_mask, sizeof(int) * n_blocks * c->CELLS, hipMemcpyDeviceToHost, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    return AGAL_OK;
+}
+
+int agal_upload_cell_masks(agal_ctx *ctx, const int *src, long long n_blocks)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!src || n_blocks < 0 || n_blocks > c->n_maxcblocks) return AGAL_ERR_ARG;
+    AGAL_HIP(c, hipMemcpyAsync(c->cells_mask, src, sizeof(int) * n_blocks * c->CELLS, hipMemcpyHostToDevice, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    return rebuild_level_lists(c, 0);
+}
+
+static int *block_array(Ctx *c, int which, int &rows)
+{
+    rows = 1;
+    switch (which) {
+        case AGAL_BLK_NBR: rows = c->QMAX; return c->nbr;
+        case AGAL_BLK_NBR_CHILD: rows = c->QMAX; return c->nbr_child;
+        case AGAL_BLK_REF: return c->ref;
+        case AGAL_BLK_LEVEL: return c->level;
+        case AGAL_BLK_ONB: return c->onb;
+        case AGAL_BLK_MASK: return c->blk_mask;
+    }
+    return nullptr;
+}
+
+int agal_download_block_ints(agal_ctx *ctx, int which, int *dst, long long n_blocks, long long dst_stride)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    int rows;
+    int *src = block_array(c, which, rows);
+    if (!src || !dst || n_blocks < 0 || n_blocks > c->n_maxcblocks || (rows > 1 && dst_stride < n_blocks)) return AGAL_ERR_ARG;
+    for (int r = 0; r < rows; r++)
+        AGAL_HIP(c, hipMemcpyAsync(dst + r * dst_stride, src + r * c->n_maxcblocks, sizeof(int) * n_blocks,
+                                   hipMemcpyDeviceToHost, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    return AGAL_OK;
+}
+
+int agal_upload_block_ints(agal_ctx *ctx, int which, const int *src, long long n_blocks, long long src_stride)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    int rows;
+    int *dstp = block_array(c, which, rows);
+    if (!dstp || !src || n_blocks < 0 || n_blocks > c->n_maxcblocks || (rows > 1 && src_stride < n_blocks)) return AGAL_ERR_ARG;
+    for (int r = 0; r < rows; r++)
+        AGAL_HIP(c, hipMemcpyAsync(dstp + r * c->n_maxcblocks, src + r * src_stride, sizeof(int) * n_blocks,
+                                   hipMemcpyHostToDevice, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    return rebuild_level_lists(c, 0);
+}
+
+int agal_download_coords(agal_ctx *ctx, float *dst, long long n_blocks, long long dst_stride)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!dst || n_blocks < 0 || n_blocks > c->n_maxcblocks || dst_stride < n_blocks) return AGAL_ERR_ARG;
+    for (int d = 0; d < c->DIM; d++)
+        AGAL_HIP(c, hipMemcpyAsync(dst + d * dst_stride, c->X + d * c->n_maxcblocks, sizeof(float) * n_blocks,
+                                   hipMemcpyDeviceToHost, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    return AGAL_OK;
+}
+
+int agal_download_id_set(agal_ctx *ctx, int L, int *dst, int n)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (L < 0 || L >= c->MAX_LEVELS || n < 0 || n > c->n_ids[L] || (n > 0 && !dst)) return AGAL_ERR_ARG;
+    if (n == 0) return AGAL_OK;
+    AGAL_HIP(c, hipMemcpyAsync(dst, c->id_set_L(L), sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    return AGAL_OK;
+}
+
+int agal_download_gap_set(agal_ctx *ctx, int *dst, int n)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (n < 0 || n > c->n_gaps || (n > 0 && !dst)) return AGAL_ERR_ARG;
+    if (n == 0) return AGAL_OK;
+    AGAL_HIP(c, hipMemcpyAsync(dst, c->gap_set, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    return AGAL_OK;
+}
+
+int agal_timing_reset(agal_ctx *ctx, int enable)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    int st = collect_timing(c);
+    if (st) return st;
+    c->timing.enabled = enable != 0;
+    c->timing.fused_ms = 0;
+    c->timing.fused_launches = 0;
+    c->timing.fused_cells = 0;
+    return AGAL_OK;
+}
+
+int agal_timing_get(agal_ctx *ctx, double *fused_ms_total, long long *fused_launches, double *fused_cells_total)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    int st = collect_timing(c);
+    if (st) return st;
+    if (fused_ms_total) *fused_ms_total = c->timing.fused_ms;
+    if (fused_launches) *fused_launches = c->timing.fused_launches;
+    if (fused_cells_total) *fused_cells_total = c->timing.fused_cells;
+    return AGAL_OK;
+}
+
+}  // extern "C"

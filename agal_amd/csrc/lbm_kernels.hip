@@ -1,0 +1,535 @@
+// lbm_kernels.hip -- the per-level LBM kernels for gfx950.
+//
+// One 64-lane wavefront owns one 4x4x4 cell-block (3D) or four 4x4 blocks (2D); a 256-thread
+// workgroup therefore covers 4 (3D) or 16 (2D) blocks of a level's sorted ID list.
+//
+// The fused step replaces the reference's three passes Cu_Collide -> Cu_ImposeBC -> Cu_Stream
+// (src/solver_lbm/solver_lbm_kernels.cu:204-289, 691-887, 532-633) by one read of the Q
+// pre-collision populations of a cell and one write of its Q streamed populations into the other
+// buffer (push).  The reference's in-place swap exchanges a link only when the initiating cell is
+// not solid and the partner's post-collision value is > 0 (kernels.cu:624); the push reproduces
+// that rule exactly:
+//   - every cell computes its post-collision values f*_q (BGK for cells that collide; for cells
+//     that do not -- solid cells and blocks that are solid with children -- the swap semantics
+//     of the reference amount to f*_q = f_{pb(q)});
+//   - the domain-face hook modifies f*_q on links that leave the domain (Cu_ImposeBC);
+//   - a link (x -> x+c_h, h in the reference's half set) streams when x collides and the
+//     partner's f*_{pb(h)} > 0; otherwise both ends bounce back into their own opposite slot.
+//     The partner knows its value, the initiator does not: the initiator always pushes, and a
+//     partner whose value fails the guard records the link in an exception list that a one-
+//     workgroup fix-up kernel resolves after the step (normally empty).
+#include <hip/hip_runtime.h>
+
+#include "agal_common.hpp"
+#include "agal_ctx.hpp"
+
+namespace agal {
+
+constexpr int WG = 256;
+
+// ---------------------------------------------------------------------------------------------
+// Initial conditions: Cu_SetInitialConditions (solver_lbm_kernels.cu:13-77), uniform state hook.
+// ---------------------------------------------------------------------------------------------
+template <typename T, int Q>
+__global__ __launch_bounds__(WG) void k_set_ic(const int *__restrict__ ids, int n, T *__restrict__ F, long long ncm,
+                                               const int *__restrict__ cells_mask, T rho, T u, T v, T w)
+{
+    using VS = VelSet<Q>;
+    constexpr int CELLS = VS::CELLS;
+    const int t = threadIdx.x % CELLS;
+    const int bi = blockIdx.x * (WG / CELLS) + threadIdx.x / CELLS;
+    if (bi >= n) return;
+    const long long cell = (long long)ids[bi] * CELLS + t;
+    const T udotu = u * u + v * v + w * w;
+    const int m = cells_mask[cell];
+#pragma unroll
+    for (int p = 0; p < Q; p++) {
+        T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+        T f = (T)(-1.0);
+        if (m != CELL_SOLID) f = rho * (T)VS::w(p) * ((T)1.0 + (T)3.0 * cdotu + (T)4.5 * cdotu * cdotu - (T)1.5 * udotu);
+        F[(long long)p * ncm + cell] = f;
+    }
+}
+
+// Moments in the reference's order (p-order sums, then division): Cu_RefreshVariables (:110-159).
+template <typename T, int Q>
+__device__ __forceinline__ void moments(const T *f, T &rho, T &u, T &v, T &w)
+{
+    using VS = VelSet<Q>;
+    rho = 0;
+    u = 0;
+    v = 0;
+    w = 0;
+#pragma unroll
+    for (int p = 0; p < Q; p++) {
+        rho += f[p];
+        u += (T)VS::cx(p) * f[p];
+        v += (T)VS::cy(p) * f[p];
+        if (VS::DIM == 3) w += (T)VS::cz(p) * f[p];
+    }
+    u /= rho;
+    v /= rho;
+    if (VS::DIM == 3) w /= rho;
+}
+
+template <typename T, int Q>
+__global__ __launch_bounds__(WG) void k_refresh(const int *__restrict__ ids, int n, const T *__restrict__ F,
+                                                T *__restrict__ M, long long ncm)
+{
+    using VS = VelSet<Q>;
+    constexpr int CELLS = VS::CELLS;
+    const int t = threadIdx.x % CELLS;
+    const int bi = blockIdx.x * (WG / CELLS) + threadIdx.x / CELLS;
+    if (bi >= n) return;
+    const long long cell = (long long)ids[bi] * CELLS + t;
+    T f[Q];
+#pragma unroll
+    for (int p = 0; p < Q; p++) f[p] = F[(long long)p * ncm + cell];
+    T rho, u, v, w;
+    moments<T, Q>(f, rho, u, v, w);
+    M[0 * ncm + cell] = rho;
+    M[1 * ncm + cell] = u;
+    M[2 * ncm + cell] = v;
+    M[3 * ncm + cell] = (VS::DIM == 3) ? w : (T)0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fused collide + boundary hook + stream of one level sub-step.
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+struct StepArgs {
+    const int *ids;
+    int n;
+    const T *A;
+    T *B;
+    long long ncm;
+    const int *nbr_slot;   // per block, Q_max neighbour IDs in 3x3(x3) slot order
+    const int *part;       // block collides and streams (not solid-with-children)
+    const int *simple;     // block and its neighbourhood: all participate, no solid cells
+    const int *cells_mask;
+    const int *onb;
+    T omeg, omegp, U;
+    int bc_preset;
+    int *exc_idx;
+    T *exc_val;
+    int *exc_count;
+    long long exc_cap;
+};
+
+template <typename T, int Q>
+__global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T> a)
+{
+    using VS = VelSet<Q>;
+    constexpr int CELLS = VS::CELLS, DIM = VS::DIM, QMAX = VS::QMAX;
+    const int t = threadIdx.x % CELLS;
+    const int lane = threadIdx.x % 64;
+    const int gbase = lane - t;  // lane of cell 0 of this block within the wavefront
+    const int bi = blockIdx.x * (WG / CELLS) + threadIdx.x / CELLS;
+    const bool active = bi < a.n;
+    const int b = active ? a.ids[bi] : 0;
+    // Lane-parallel load of the block's neighbour table; distributed below with ds_bpermute.
+    const int my_nb = (active && t < QMAX) ? a.nbr_slot[(long long)b * QMAX + t] : N_SKIPID;
+    const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
+    const long long cell = (long long)b * CELLS + t;
+
+    // Neighbour block and cell of (I,J,K) + c_q for every direction (all lanes participate).
+    int tb[Q], tcell[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int Ip = I + VS::cx(q), Jp = J + VS::cy(q), Kp = K + VS::cz(q);
+        const int S = (Ip < 0 ? 0 : (Ip >= 4 ? 2 : 1)) + 3 * (Jp < 0 ? 0 : (Jp >= 4 ? 2 : 1)) +
+                      (DIM == 3 ? 9 * (Kp < 0 ? 0 : (Kp >= 4 ? 2 : 1)) : 0);
+        tb[q] = __shfl(my_nb, gbase + S, 64);
+        tcell[q] = (Ip & 3) + 4 * (Jp & 3) + (DIM == 3 ? 16 * (Kp & 3) : 0);
+    }
+    if (!active) return;
+
+    const bool simple = a.simple[b] != 0;
+    const bool part = a.part[b] != 0;
+    int mask = CELL_INTERIOR;
+    if (!simple) mask = a.cells_mask[cell];
+    const bool inert = !part || mask == CELL_SOLID;
+
+    T f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = a.A[(long long)q * a.ncm + cell];
+
+    T fs[Q];
+    T rho, u, v, w;
+    moments<T, Q>(f, rho, u, v, w);
+    if (!inert) {
+        const T udotu = u * u + v * v + w * w;
+#pragma unroll
+        for (int p = 0; p < Q; p++) {
+            T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+            fs[p] = f[p] * a.omegp + ((T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu)) * a.omeg;
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < Q; p++) fs[p] = f[VS::pb(p)];
+    }
+
+    // Domain-face hook on links that leave the domain (Cu_ImposeBC, onb blocks, non-solid cells).
+    if (a.onb[b] == 1 && mask != CELL_SOLID) {
+#pragma unroll
+        for (int q = 1; q < Q; q++) {
+            if (tb[q] < 0 && tb[q] != N_SKIPID)
+                fs[q] = bc_hook<T>(a.bc_preset, a.U, tb[q], fs[q], u, v, w, (T)VS::w(q), (T)VS::cx(q), (T)VS::cy(q), (T)VS::cz(q));
+        }
+    }
+
+    a.B[cell] = fs[0];
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+        const int nbk = tb[q];
+        const long long tgt = (long long)nbk * CELLS + tcell[q];
+        bool push;
+        bool exc = false;
+        if (VS::half(q)) {
+            // Initiator of the link x -> x + c_q.
+            push = !inert && nbk >= 0;
+            if (push && !simple) {
+                const bool p_inert = !a.part[nbk] || a.cells_mask[tgt] == CELL_SOLID;
+                if (p_inert) push = a.A[(long long)q * a.ncm + tgt] > (T)0;
+            }
+        } else {
+            // Partner of the link initiated by the cell x + c_q in direction pb(q).
+            push = nbk >= 0;
+            if (push && !simple) push = a.part[nbk] && a.cells_mask[tgt] != CELL_SOLID;
+            if (push && !(fs[q] > (T)0)) {
+                push = false;
+                exc = !inert;  // a colliding partner cannot tell the initiator in time
+            }
+        }
+        if (push) {
+            a.B[(long long)q * a.ncm + tgt] = fs[q];
+        } else if (exc) {
+            int k = atomicAdd(&a.exc_count[0], 1);
+            if (k < a.exc_cap) {
+                a.exc_idx[3 * (long long)k + 0] = (int)tgt;  // cell ids fit in int for pools < 2^31 cells
+                a.exc_idx[3 * (long long)k + 1] = (int)cell;
+                a.exc_idx[3 * (long long)k + 2] = q;
+                a.exc_val[k] = fs[q];
+            } else {
+                a.exc_count[1] = 1;
+            }
+        } else {
+            a.B[(long long)VS::pb(q) * a.ncm + cell] = fs[q];
+        }
+    }
+}
+
+// Resolves guard exceptions: the initiator's pushed value returns to it (bounce-back) and the
+// partner keeps its own post-collision value.  One workgroup; clears the counter when done.
+template <typename T, int Q>
+__global__ __launch_bounds__(WG) void k_fixup(T *__restrict__ B, long long ncm, const int *__restrict__ exc_idx,
+                                              const T *__restrict__ exc_val, int *exc_count, long long exc_cap)
+{
+    using VS = VelSet<Q>;
+    const long long n = min((long long)exc_count[0], exc_cap);
+    for (long long k = threadIdx.x; k < n; k += WG) {
+        const long long i = exc_idx[3 * k + 0], c = exc_idx[3 * k + 1];
+        const int q = exc_idx[3 * k + 2];
+        int qb = 0;
+#pragma unroll
+        for (int p = 0; p < Q; p++)
+            if (p == q) qb = VS::pb(p);
+        B[(long long)q * ncm + i] = B[(long long)qb * ncm + c];
+        B[(long long)qb * ncm + c] = exc_val[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) exc_count[0] = 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Coarse -> fine: Cu_Interpolate_Linear (solver_lbm_kernels_comm.cu:13-138).  One wavefront per
+// parent block (3D); the rescaled parent populations of one direction are shared through LDS.
+// ---------------------------------------------------------------------------------------------
+template <typename T, int Q, bool ADDED>
+__global__ __launch_bounds__(WG) void k_interp(const int *__restrict__ ids, int n, const T *__restrict__ Fp,
+                                               T *__restrict__ Fc, long long ncm, const int *__restrict__ nbr_child,
+                                               const int *__restrict__ blk_mask, const int *__restrict__ ref,
+                                               const int *__restrict__ cells_mask, T tau_ratio)
+{
+    using VS = VelSet<Q>;
+    constexpr int CELLS = VS::CELLS, DIM = VS::DIM, NCH = VS::CHILDREN;
+    __shared__ T s_F[WG];
+    const int t = threadIdx.x % CELLS;
+    const int base = threadIdx.x - t;
+    const int bi = blockIdx.x * (WG / CELLS) + threadIdx.x / CELLS;
+    int b = -1;
+    if (bi < n) b = ids[bi];
+    bool work = false;
+    int fc = 0;
+    if (b >= 0) {
+        fc = nbr_child[b];
+        work = fc >= 0 && (ADDED ? (ref[b] == REF_MARK_REFINE) : (blk_mask[b] == BLK_INTERFACE));
+    }
+    const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
+    const T Ix = (T)(-0.25) + (T)(0.5) * (T)I;
+    const T Iy = (T)(-0.25) + (T)(0.5) * (T)J;
+    const T Iz = (DIM == 3) ? (T)(-0.25) + (T)(0.5) * (T)K : (T)0;
+    T f[Q];
+    T rho = 0, u = 0, v = 0, w = 0;
+    bool ghost[NCH];
+    if (work) {
+        const long long cell = (long long)b * CELLS + t;
+#pragma unroll
+        for (int p = 0; p < Q; p++) f[p] = Fp[(long long)p * ncm + cell];
+        moments<T, Q>(f, rho, u, v, w);
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+            ghost[c] = ADDED || cells_mask[(long long)(fc + c) * CELLS + t] == CELL_GHOST;
+    }
+    const T udotu = u * u + v * v + w * w;
+#pragma unroll
+    for (int p = 0; p < Q; p++) {
+        if (work) {
+            T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+            T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
+            s_F[threadIdx.x] = fe + (f[p] - fe) * tau_ratio;
+        }
+        __syncthreads();
+        if (work) {
+            const T *s = s_F + base;
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                const int ci = c % 2, cj = (c / 2) % 2, ck = (c / 4) % 2;
+                if (!ghost[c]) continue;
+                T val;
+                if (DIM == 2) {
+                    const int o = 2 * ci + 2 * 4 * cj;
+                    val = s[0 + o] + (s[1 + o] - s[0 + o]) * Ix + (s[4 + o] - s[0 + o]) * Iy +
+                          (s[5 + o] - s[4 + o] - s[1 + o] + s[0 + o]) * Ix * Iy;
+                } else {
+                    const int o = 2 * ci + 2 * 4 * cj + 2 * 4 * 4 * ck;
+                    val = s[0 + o] + (s[1 + o] - s[0 + o]) * Ix + (s[4 + o] - s[0 + o]) * Iy + (s[16 + o] - s[0 + o]) * Iz +
+                          (s[5 + o] - s[1 + o] - s[4 + o] + s[0 + o]) * Ix * Iy +
+                          (s[17 + o] - s[1 + o] - s[16 + o] + s[0 + o]) * Ix * Iz +
+                          (s[20 + o] - s[4 + o] - s[16 + o] + s[0 + o]) * Iy * Iz +
+                          (s[21 + o] + s[1 + o] + s[4 + o] + s[16 + o] - s[5 + o] - s[17 + o] - s[20 + o] - s[0 + o]) * Ix * Iy * Iz;
+                }
+                Fc[(long long)p * ncm + (long long)(fc + c) * CELLS + t] = val;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fine -> coarse: Cu_Average (solver_lbm_kernels_comm.cu:174-347).
+// ---------------------------------------------------------------------------------------------
+template <typename T, int Q>
+__global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int n, T *__restrict__ Fp,
+                                                const T *__restrict__ Fc, long long ncm, const int *__restrict__ nbr_child,
+                                                const int *__restrict__ blk_mask, const int *__restrict__ cells_mask,
+                                                int ave_type, T tau_ratio)
+{
+    using VS = VelSet<Q>;
+    constexpr int CELLS = VS::CELLS, DIM = VS::DIM, NCH = VS::CHILDREN;
+    __shared__ T s_Fc[WG];
+    __shared__ int s_mask[WG];
+    const int t = threadIdx.x % CELLS;
+    const int base = threadIdx.x - t;
+    const int bi = blockIdx.x * (WG / CELLS) + threadIdx.x / CELLS;
+    int b = -1;
+    if (bi < n) b = ids[bi];
+    bool work = false;
+    int fc = 0;
+    if (b >= 0) {
+        fc = nbr_child[b];
+        work = (fc > -1) && (ave_type == 2 || blk_mask[b] == BLK_INTERFACE);
+    }
+    const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
+    int c0 = 2 * (I % 2) + 4 * (2 * (J % 2));
+    if (DIM == 3) c0 += 4 * 4 * (2 * (K % 2));
+#pragma unroll 1
+    for (int c = 0; c < NCH; c++) {
+        const int ci = c % 2, cj = (c / 2) % 2, ck = (c / 4) % 2;
+        bool inq = (I >= 2 * ci) && (I < 2 + 2 * ci) && (J >= 2 * cj) && (J < 2 + 2 * cj);
+        if (DIM == 3) inq = inq && (K >= 2 * ck) && (K < 2 + 2 * ck);
+        T f[Q];
+        T rho = 0, u = 0, v = 0, w = 0;
+        if (work) {
+            const long long ccell = (long long)(fc + c) * CELLS + t;
+            int m = cells_mask[ccell];
+            if (ave_type > 0 && m < 2) m = 1;
+            s_mask[threadIdx.x] = m;
+#pragma unroll
+            for (int p = 0; p < Q; p++) f[p] = Fc[(long long)p * ncm + ccell];
+            moments<T, Q>(f, rho, u, v, w);
+        }
+        const T udotu = (DIM == 3) ? u * u + v * v + w * w : u * u + v * v;
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < Q; p++) {
+            if (work) {
+                T cdotu = (DIM == 3) ? (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w
+                                     : (T)VS::cx(p) * u + (T)VS::cy(p) * v;
+                T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
+                s_Fc[threadIdx.x] = fe + (f[p] - fe) * tau_ratio;
+            }
+            __syncthreads();
+            if (work && inq && s_mask[base + c0] == 1) {
+                const T *s = s_Fc + base;
+                T val;
+                if (DIM == 2)
+                    val = (T)(0.25) * (s[c0 + 0] + s[c0 + 1] + s[c0 + 4] + s[c0 + 5]);
+                else
+                    val = (T)(0.125) * (s[c0 + 0] + s[c0 + 1] + s[c0 + 4] + s[c0 + 5] + s[c0 + 16] + s[c0 + 17] +
+                                        s[c0 + 20] + s[c0 + 21]);
+                Fp[(long long)p * ncm + (long long)b * CELLS + t] = val;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+static T *Fbuf(Ctx *c, int k) { return static_cast<T *>(c->F[k]); }
+
+static inline int nwg(long long nblocks, int cells) { return (int)((nblocks * cells + WG - 1) / WG); }
+
+template <typename T, int Q>
+static int set_ic_t(Ctx *c, int L)
+{
+    int n = c->n_ids[L];
+    if (n <= 0) return AGAL_OK;
+    hipLaunchKernelGGL((k_set_ic<T, Q>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, c->id_set_L(L), n,
+                       Fbuf<T>(c, c->cur[L]), c->n_maxcells, c->cells_mask, (T)c->P.ic_rho, (T)c->P.ic_u,
+                       (T)c->P.ic_v, (T)c->P.ic_w);
+    AGAL_HIP(c, hipGetLastError());
+    return AGAL_OK;
+}
+
+template <typename T, int Q>
+static int refresh_t(Ctx *c, int L)
+{
+    int n = c->n_ids[L];
+    if (n <= 0) return AGAL_OK;
+    hipLaunchKernelGGL((k_refresh<T, Q>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, c->id_set_L(L), n,
+                       Fbuf<T>(c, c->cur[L]), static_cast<T *>(c->M), c->n_maxcells);
+    AGAL_HIP(c, hipGetLastError());
+    return AGAL_OK;
+}
+
+template <typename T, int Q>
+static int collide_stream_t(Ctx *c, int L)
+{
+    int n = c->n_ids[L];
+    if (n <= 0) return AGAL_OK;
+    StepArgs<T> a;
+    a.ids = c->id_set_L(L);
+    a.n = n;
+    a.A = Fbuf<T>(c, c->cur[L]);
+    a.B = Fbuf<T>(c, c->cur[L] ^ 1);
+    a.ncm = c->n_maxcells;
+    a.nbr_slot = c->nbr_slot;
+    a.part = c->part;
+    a.simple = c->simple;
+    a.cells_mask = c->cells_mask;
+    a.onb = c->onb;
+    const T dx_L = (T)c->dxf_vec[L], tau_L = (T)c->tau_vec[L];
+    a.omeg = dx_L / tau_L;
+    a.omegp = (T)(1.0) - a.omeg;
+    a.U = (T)c->P.bc_velocity;
+    a.bc_preset = c->P.bc_preset;
+    a.exc_idx = c->exc_idx;
+    a.exc_val = static_cast<T *>(c->exc_val);
+    a.exc_count = c->exc_count;
+    a.exc_cap = c->exc_cap;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing.enabled) {
+        AGAL_HIP(c, hipEventCreate(&e0));
+        AGAL_HIP(c, hipEventCreate(&e1));
+        AGAL_HIP(c, hipEventRecord(e0, c->stream));
+    }
+    hipLaunchKernelGGL((k_collide_stream<T, Q>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, a);
+    AGAL_HIP(c, hipGetLastError());
+    if (c->timing.enabled) {
+        AGAL_HIP(c, hipEventRecord(e1, c->stream));
+        c->timing.ev_pairs.push_back(e0);
+        c->timing.ev_pairs.push_back(e1);
+        c->timing.fused_launches++;
+        c->timing.fused_cells += (double)n * VelSet<Q>::CELLS;
+    }
+    hipLaunchKernelGGL((k_fixup<T, Q>), dim3(1), dim3(WG), 0, c->stream, a.B, a.ncm, c->exc_idx, a.exc_val, c->exc_count,
+                       c->exc_cap);
+    AGAL_HIP(c, hipGetLastError());
+    c->cur[L] ^= 1;
+    return AGAL_OK;
+}
+
+template <typename T, int Q>
+static int interpolate_t(Ctx *c, int L, int var)
+{
+    if (L >= c->MAX_LEVELS - 1) return AGAL_OK;
+    const T ratio = (T)c->tau_vec[L + 1] / (T)c->tau_vec[L];
+    const int *ids;
+    int n;
+    if (var == AGAL_INTERP_INTERFACE) {
+        ids = c->iface_L(L);
+        n = c->n_iface[L];
+    } else {
+        ids = c->id_set_L(L);
+        n = c->n_ids[L];
+    }
+    if (n <= 0) return AGAL_OK;
+    if (var == AGAL_INTERP_ADDED)
+        hipLaunchKernelGGL((k_interp<T, Q, true>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, ids, n,
+                           Fbuf<T>(c, c->cur[L]), Fbuf<T>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
+                           c->ref, c->cells_mask, ratio);
+    else
+        hipLaunchKernelGGL((k_interp<T, Q, false>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, ids, n,
+                           Fbuf<T>(c, c->cur[L]), Fbuf<T>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
+                           c->ref, c->cells_mask, ratio);
+    AGAL_HIP(c, hipGetLastError());
+    return AGAL_OK;
+}
+
+template <typename T, int Q>
+static int average_t(Ctx *c, int L, int var)
+{
+    if (L >= c->MAX_LEVELS - 1) return AGAL_OK;
+    const T ratio = (T)c->tau_vec[L] / (T)c->tau_vec[L + 1];
+    const int *ids;
+    int n;
+    if (var == AGAL_AVERAGE_INTERFACE) {
+        ids = c->iface_L(L);
+        n = c->n_iface[L];
+    } else {
+        ids = c->id_set_L(L);
+        n = c->n_ids[L];
+    }
+    if (n <= 0) return AGAL_OK;
+    hipLaunchKernelGGL((k_average<T, Q>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, ids, n,
+                       Fbuf<T>(c, c->cur[L]), Fbuf<T>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
+                       c->cells_mask, var, ratio);
+    AGAL_HIP(c, hipGetLastError());
+    return AGAL_OK;
+}
+
+#define AGAL_DISPATCH(c, FN, ...)                                                          \
+    do {                                                                                   \
+        if ((c)->precision == AGAL_FP64) {                                                 \
+            if ((c)->Q == 9) return FN<double, 9>(__VA_ARGS__);                            \
+            if ((c)->Q == 19) return FN<double, 19>(__VA_ARGS__);                          \
+            if ((c)->Q == 27) return FN<double, 27>(__VA_ARGS__);                          \
+        } else {                                                                           \
+            if ((c)->Q == 9) return FN<float, 9>(__VA_ARGS__);                             \
+            if ((c)->Q == 19) return FN<float, 19>(__VA_ARGS__);                           \
+            if ((c)->Q == 27) return FN<float, 27>(__VA_ARGS__);                           \
+        }                                                                                  \
+        return AGAL_ERR_ARG;                                                               \
+    } while (0)
+
+int launch_set_ic(Ctx *c, int L) { AGAL_DISPATCH(c, set_ic_t, c, L); }
+int launch_refresh(Ctx *c, int L) { AGAL_DISPATCH(c, refresh_t, c, L); }
+int launch_collide_stream(Ctx *c, int L) { AGAL_DISPATCH(c, collide_stream_t, c, L); }
+int launch_interpolate(Ctx *c, int L, int var) { AGAL_DISPATCH(c, interpolate_t, c, L, var); }
+int launch_average(Ctx *c, int L, int var) { AGAL_DISPATCH(c, average_t, c, L, var); }
+
+}  // namespace agal

@@ -1,0 +1,199 @@
+"""Host-side mirror of the reference's Mesh / Solver_LBM setup API over the C ABI.
+
+`LbmMesh` owns one context created through a bound C ABI (the HIP library, `agal_*`).  Method
+names follow the reference (inc/mesh.h, inc/solver_lbm.h): M_RefineAndCoarsenBlocks,
+S_SetIC, S_Advance, ...; each raises `AgalError` on a non-zero status, except the pool-exhausted
+refinement cancel, which the reference also survives (src/mesh/mesh_amr.cu:2197-2206) and which
+is reported through the return value.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import AgalError, Params, Sizes, np_ptr
+
+
+class LbmMesh:
+    """One mesh + LBM solver on one device, driven through a bound C ABI."""
+
+    def __init__(self, params: Params, capacity: int, *, fns: dict | None = None, lib=None,
+                 prefix: str = "agal_", device: int = 0):
+        if fns is None:
+            lib = capi.load_library() if lib is None else lib
+            fns = capi.bind(lib, prefix, capi.PROTOTYPES.keys())
+        self._f = fns
+        self._lib = lib
+        self.params = params
+        self.dtype = np.float64 if params.precision == capi.FP64 else np.float32
+        h = C.c_void_p()
+        st = fns["create"](C.byref(params), int(capacity), int(device), C.byref(h))
+        if st != capi.AGAL_OK:
+            raise AgalError(st, prefix + "create")
+        self.h = h
+        self.prefix = prefix
+        self.refresh_sizes()
+
+    # -- lifecycle ---------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self._f["destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args, ok=(capi.AGAL_OK,)):
+        st = self._f[name](self.h, *args)
+        if st not in ok:
+            detail = ""
+            if self._lib is not None and hasattr(self._lib, "agal_last_error") and self.prefix == "agal_":
+                detail = (self._lib.agal_last_error(self.h) or b"").decode()
+            raise AgalError(st, self.prefix + name, detail)
+        return st
+
+    def refresh_sizes(self) -> Sizes:
+        s = Sizes()
+        self._call("get_sizes", C.byref(s))
+        self.sizes = s
+        return s
+
+    def sync(self):
+        self._call("sync")
+
+    # -- solver interface (inc/solver.h:62-71) -------------------------------------------------
+    def S_SetIC(self, L):
+        self._call("set_ic", int(L))
+
+    def S_Advance(self, L0=None):
+        self._call("advance", int(self.params.N_LEVEL_START if L0 is None else L0))
+
+    def collide_stream(self, L):
+        self._call("collide_stream", int(L))
+
+    def S_Interpolate(self, L, var):
+        self._call("interpolate", int(L), int(var))
+
+    def S_Average(self, L, var):
+        self._call("average", int(L), int(var))
+
+    def S_RefreshVariables(self, L):
+        self._call("refresh", int(L))
+
+    def S_ComputeRefCriteria(self):
+        self._call("ref_criteria")
+
+    # -- mesh interface ------------------------------------------------------------------------
+    def M_RefineAndCoarsenBlocks(self):
+        counts = (C.c_int * 4)()
+        st = self._call("refine_coarsen", counts, ok=(capi.AGAL_OK, capi.AGAL_ERR_POOL_EXHAUSTED))
+        self.refresh_sizes()
+        return st, list(counts)
+
+    def M_Advance_RefineWithSolution(self):
+        counts = (C.c_int * 4)()
+        st = self._call("refine_with_solution", counts, ok=(capi.AGAL_OK, capi.AGAL_ERR_POOL_EXHAUSTED))
+        self.refresh_sizes()
+        return st, list(counts)
+
+    def mark_uniform(self, L):
+        self._call("mark_uniform", int(L))
+
+    def mark_blocks(self, ids, mark):
+        a = np.ascontiguousarray(np.asarray(ids, dtype=np.int32))
+        self._call("mark_blocks", np_ptr(a, C.c_int), int(a.size), int(mark))
+
+    def M_FreezeRefinedCells(self, var=0):
+        self._call("freeze_refined", int(var))
+
+    def advance_loop(self, i0, n):
+        ts, tr = C.c_double(), C.c_double()
+        st = self._call("advance_loop", int(i0), int(n), C.byref(ts), C.byref(tr),
+                        ok=(capi.AGAL_OK, capi.AGAL_ERR_POOL_EXHAUSTED))
+        self.refresh_sizes()
+        return st, ts.value, tr.value
+
+    # -- data exchange -------------------------------------------------------------------------
+    def n_blocks(self) -> int:
+        """Blocks up to the largest ID ever used (id_max over all levels)."""
+        s = self.refresh_sizes()
+        return int(s.id_max[s.max_levels])
+
+    def populations(self, n_blocks=None) -> np.ndarray:
+        nb = self.n_blocks() if n_blocks is None else int(n_blocks)
+        nc = nb * self.sizes.cells_per_block
+        out = np.empty((self.sizes.n_q, nc), dtype=self.dtype)
+        self._call("download_populations", np_ptr(out), nb, nc)
+        return out
+
+    def set_populations(self, f: np.ndarray):
+        f = np.ascontiguousarray(f, dtype=self.dtype)
+        nc = f.shape[1]
+        nb = nc // self.sizes.cells_per_block
+        self._call("upload_populations", np_ptr(f), nb, nc)
+
+    def macros(self, n_blocks=None) -> np.ndarray:
+        nb = self.n_blocks() if n_blocks is None else int(n_blocks)
+        nc = nb * self.sizes.cells_per_block
+        out = np.empty((4, nc), dtype=self.dtype)
+        self._call("download_macros", np_ptr(out), nb, nc)
+        return out
+
+    def cell_masks(self, n_blocks=None) -> np.ndarray:
+        nb = self.n_blocks() if n_blocks is None else int(n_blocks)
+        out = np.empty(nb * self.sizes.cells_per_block, dtype=np.int32)
+        self._call("download_cell_masks", np_ptr(out, C.c_int), nb)
+        return out
+
+    def block_ints(self, which, n_blocks=None) -> np.ndarray:
+        nb = self.n_blocks() if n_blocks is None else int(n_blocks)
+        rows = self.sizes.n_q_max if which in (capi.BLK_NBR, capi.BLK_NBR_CHILD) else 1
+        out = np.empty((rows, nb), dtype=np.int32)
+        self._call("download_block_ints", int(which), np_ptr(out, C.c_int), nb, nb)
+        return out if rows > 1 else out[0]
+
+    def coords(self, n_blocks=None) -> np.ndarray:
+        nb = self.n_blocks() if n_blocks is None else int(n_blocks)
+        out = np.empty((self.sizes.n_dim, nb), dtype=np.float32)
+        self._call("download_coords", np_ptr(out, C.c_float), nb, nb)
+        return out
+
+    def id_set(self, L) -> np.ndarray:
+        n = int(self.refresh_sizes().n_ids[L])
+        out = np.empty(n, dtype=np.int32)
+        self._call("download_id_set", int(L), np_ptr(out, C.c_int), n)
+        return out
+
+    def gap_set(self) -> np.ndarray:
+        n = int(self.refresh_sizes().n_gaps)
+        out = np.empty(n, dtype=np.int32)
+        self._call("download_gap_set", np_ptr(out, C.c_int), n)
+        return out
+
+    def state(self) -> dict:
+        """Everything the parity tests compare, restricted to active blocks where it matters."""
+        s = self.refresh_sizes()
+        ML = s.max_levels
+        st = {
+            "n_ids": [int(s.n_ids[L]) for L in range(ML)],
+            "id_max": int(s.id_max[ML]),
+            "n_gaps": int(s.n_gaps),
+            "id_sets": [self.id_set(L) for L in range(ML)],
+            "gap_set": self.gap_set(),
+        }
+        nb = st["id_max"]
+        st["ref"] = self.block_ints(capi.BLK_REF, nb)
+        st["level"] = self.block_ints(capi.BLK_LEVEL, nb)
+        st["nbr"] = self.block_ints(capi.BLK_NBR, nb)
+        st["nbr_child"] = self.block_ints(capi.BLK_NBR_CHILD, nb)
+        st["onb"] = self.block_ints(capi.BLK_ONB, nb)
+        st["blk_mask"] = self.block_ints(capi.BLK_MASK, nb)
+        st["coords"] = self.coords(nb)
+        st["cell_mask"] = self.cell_masks(nb)
+        st["f"] = self.populations(nb)
+        return st

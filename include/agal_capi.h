@@ -1,0 +1,218 @@
+/*
+ * agal_capi.h -- C ABI of the MI355X-native octree-AMR lattice-Boltzmann step.
+ *
+ * This is the drop-in boundary.  The reference wires its LBM solver into the mesh through the
+ * abstract class `Solver<ufloat_t,ufloat_g_t,AP>` (inc/solver.h:62-71) and keeps all device state
+ * in `Mesh<...>` (inc/mesh.h:137-765).  Here one opaque context owns the device state of one mesh
+ * and one solver on one GPU; every entry point below replaces one method of those classes and is
+ * annotated with the reference declaration it stands for.  Plain C types only: pointers, sizes,
+ * ints and doubles.  All functions return an `int` status (AGAL_OK == 0) instead of calling
+ * exit() as the reference's gpuErrchk does (inc/cppspec.h:146-155).
+ *
+ * Ordering: every call is stream-ordered on the context's single HIP stream (the reference uses
+ * one stream per device, inc/mesh.h:482, src/mesh/mesh_init.cu:236).  Calls on one context are
+ * not thread-safe.  Download functions synchronise the stream before returning.
+ *
+ * Populations are exchanged in physical order: entry q of a cell is the pre-collision
+ * population of direction q.  (The reference stores direction p in slot pb(p) between steps,
+ * src/solver_lbm/solver_lbm_kernels.cu:251; that is an internal convention, not part of the ABI.)
+ */
+#ifndef AGAL_CAPI_H
+#define AGAL_CAPI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------------------------- */
+#define AGAL_OK                 0
+#define AGAL_ERR_ARG            1   /* invalid argument / unsupported combination               */
+#define AGAL_ERR_HIP            2   /* a HIP runtime call failed                                */
+#define AGAL_ERR_ALLOC          3   /* device or host allocation failed                         */
+#define AGAL_ERR_CAPACITY       4   /* root grid does not fit the block pool (mesh_init.cu:171) */
+#define AGAL_ERR_POOL_EXHAUSTED 5   /* refinement cancelled: not enough free block IDs
+                                       (src/mesh/mesh_amr.cu:2190-2206).  The mesh is left
+                                       consistent (marks reverted); the call still completes. */
+#define AGAL_ERR_STATE          6   /* call out of order (e.g. step before set_ic)              */
+#define AGAL_ERR_OVERFLOW       7   /* an internal bounded list overflowed                      */
+#define AGAL_ERR_NOT_BUILT      8   /* library built without the requested variant            */
+
+/* ---- enumerations (values follow the reference where it has them) ---------------------------- */
+#define AGAL_D2Q9   9
+#define AGAL_D3Q19 19
+#define AGAL_D3Q27 27
+
+#define AGAL_FP32      0    /* populations stored and computed in FP32                         */
+#define AGAL_FP64      1    /* populations stored and computed in FP64                         */
+
+#define AGAL_BGK       0    /* CollisionOperator::BGK (inc/solver_lbm.h:19-24)                  */
+
+#define AGAL_INTERP_LINEAR 0 /* InterpOrder::Linear (inc/solver_lbm.h:25-30)                    */
+
+/* Boundary-condition presets of the reference's device hook Cu_ImposeBC (inc/custom.h:66-123). */
+#define AGAL_BC_NONE    0   /* hook returns f unchanged on every face: plain bounce-back        */
+#define AGAL_BC_LDC2D   1   /* moving lid on face id -4 (custom.h:80-84)                         */
+#define AGAL_BC_LDC3D   2   /* moving lid on face id -6 (custom.h:100-104)                       */
+#define AGAL_BC_FPSC2D  3   /* inflow on -1,-3,-4, anti-bounce-back outflow on -2 (custom.h:87-97)*/
+#define AGAL_BC_FPSC3D  4   /* inflow on -1,-3..-6, anti-bounce-back outflow on -2 (:107-117)     */
+
+/* Interpolation / averaging modes (inc/mesh.h:58-62). */
+#define AGAL_INTERP_INTERFACE 0
+#define AGAL_INTERP_ADDED     1
+#define AGAL_AVERAGE_INTERFACE 0
+#define AGAL_AVERAGE_BLOCK     1
+#define AGAL_AVERAGE_GRID      2
+
+/* Refinement states (inc/mesh.h:34-46) and special neighbour IDs (inc/cppspec.h:124-125). */
+#define AGAL_REF_UNREFINED          0
+#define AGAL_REF_REFINED            1
+#define AGAL_REF_REFINED_WCHILD     2
+#define AGAL_REF_REFINED_PERM       3
+#define AGAL_REF_MARK_REFINE        4
+#define AGAL_REF_MARK_COARSEN       5
+#define AGAL_REF_NEW                6
+#define AGAL_REF_REMOVE             7
+#define AGAL_REF_INACTIVE           8
+#define AGAL_REF_UNREFINED_VIO      9
+#define AGAL_REF_REFINED_WCHILD_PERM 10
+#define AGAL_N_SKIPID  (-999)
+#define AGAL_N_SPECID  (-998)
+
+/* Cell masks (inc/mesh.h:82-98). */
+#define AGAL_CELLMASK_INTERIOR  0
+#define AGAL_CELLMASK_INTERFACE 1
+#define AGAL_CELLMASK_GHOST     2
+#define AGAL_CELLMASK_SOLID    (-1)
+#define AGAL_CELLMASK_BOUNDARY (-6)
+
+/* ---- parameters: the input.txt keys the step needs (inc/parser.h:62-144) ----------------------
+ * Field names keep the reference's key names.  Values not listed in input.txt by the reference
+ * (velocity set, precision, presets) are compile-time packs there (src/main.cu:48-57) and runtime
+ * fields here.  agal_params_default() fills the reference's parser defaults. */
+typedef struct agal_params {
+    int    velocity_set;          /* AGAL_D2Q9 | AGAL_D3Q19 | AGAL_D3Q27 (VSET, src/Makefile:160)   */
+    int    precision;             /* AGAL_FP32 | AGAL_FP64 (REAL_s, src/main.cu:60)                */
+    int    collision;             /* AGAL_BGK (S_COLLISION)                                       */
+    int    interp_order;          /* AGAL_INTERP_LINEAR (S_INTERP)                                */
+    int    Nx;                    /* root-grid cells along x (Nx)                                 */
+    double L_c, L_fy, L_fz;       /* domain length and aspect ratios                              */
+    int    PERIODIC_X, PERIODIC_Y, PERIODIC_Z;
+    int    MAX_LEVELS_WALL, MAX_LEVELS_INTERIOR, N_LEVEL_START;
+    double v0;                    /* kinematic viscosity in the reference's units (tau = 3 v0 + dx/2) */
+    int    S_CRITERION;           /* 0: |vorticity|, 1: Q-like invariant                          */
+    double N_REFINE_START, N_REFINE_INC, N_REFINE_MAX;
+    int    P_REFINE;              /* coarse steps between solution-based rebuilds                 */
+    int    bc_preset;             /* AGAL_BC_* : the Cu_ImposeBC hook                             */
+    double bc_velocity;           /* wall / inflow speed used by the hook (0.05 in custom.h)      */
+    double ic_rho, ic_u, ic_v, ic_w; /* Cu_ComputeIC hook: uniform state (custom.h:48-60)         */
+} agal_params;
+
+typedef struct agal_ctx agal_ctx;
+
+/* Sizes of the block pool and its per-level ID sets (inc/mesh.h:430-462). */
+typedef struct agal_sizes {
+    int  n_dim, n_q, n_q_max, cells_per_block, n_children, max_levels;
+    long long n_maxcblocks, n_maxcells;
+    int  n_coarsecblocks;
+    int  n_ids[8];                /* active blocks per level                                       */
+    int  id_max[9];               /* 1 + largest active ID per level; [max_levels] = overall       */
+    int  n_gaps;                  /* free block IDs on the gap stack                               */
+} agal_sizes;
+
+/* Fill `p` with the reference parser's defaults (inc/parser.h:62-144). */
+void agal_params_default(agal_params *p);
+
+/* Mesh<...>::M_Init (src/mesh/mesh_init.cu:10-567) + Solver_LBM::S_Init (solver_lbm_init.cu:4-102).
+ * `block_capacity` replaces the reference's pool sizing from free memory (mesh_init.cu:155-169):
+ * the pool holds exactly that many cell-blocks.  Builds the root grid, boundary IDs and the gap
+ * stack; allocates device memory.  Returns AGAL_ERR_CAPACITY if the root grid does not fit. */
+int  agal_create(const agal_params *p, long long block_capacity, int device, agal_ctx **out);
+/* Mesh<...>::M_Dest (src/mesh/mesh_dest.cu:11-98). */
+int  agal_destroy(agal_ctx *ctx);
+
+int  agal_get_sizes(agal_ctx *ctx, agal_sizes *out);
+int  agal_sync(agal_ctx *ctx);
+const char *agal_status_string(int status);
+/* Last error text recorded by the context (HIP error name, argument that failed). */
+const char *agal_last_error(agal_ctx *ctx);
+
+/* ---- the solver interface (inc/solver.h:62-71) ------------------------------------------------ */
+/* Solver::S_SetIC (solver_lbm_kernels.cu:79-98). */
+int  agal_set_ic(agal_ctx *ctx, int L);
+/* Solver::S_Advance (solver_lbm_advance.cu:30-174): one coarse step of level L0 with recursive
+ * sub-cycling of the finer levels. */
+int  agal_advance(agal_ctx *ctx, int L0);
+/* One fused collide + boundary + stream sub-step of level L: S_Collide, S_ImposeBC and S_Stream
+ * (solver_lbm_kernels.cu:464-510, 890-920, 635-660) in one HBM pass. */
+int  agal_collide_stream(agal_ctx *ctx, int L);
+/* Solver::S_Interpolate (solver_lbm_kernels_comm.cu:141-155), var = AGAL_INTERP_*. */
+int  agal_interpolate(agal_ctx *ctx, int L, int var);
+/* Solver::S_Average (solver_lbm_kernels_comm.cu:351-370), var = AGAL_AVERAGE_*. */
+int  agal_average(agal_ctx *ctx, int L, int var);
+/* Solver::S_RefreshVariables (solver_lbm_kernels.cu:161-175): rho,u,v,w of level L into the
+ * macroscopic arrays. */
+int  agal_refresh(agal_ctx *ctx, int L);
+/* Solver::S_ComputeRefCriteria (solver_lbm_criterion.cu:594-613): marks blocks of every level. */
+int  agal_ref_criteria(agal_ctx *ctx);
+
+/* ---- the mesh interface ---------------------------------------------------------------------- */
+/* Mesh::M_RefineAndCoarsenBlocks (src/mesh/mesh_amr.cu:2161-2744).  out_counts (may be NULL)
+ * receives {marked for refinement, marked for coarsening after quality control, blocks added,
+ * blocks removed}.  Returns AGAL_ERR_POOL_EXHAUSTED when refinement was cancelled. */
+int  agal_refine_coarsen(agal_ctx *ctx, int out_counts[4]);
+/* Mesh::M_Advance_RefineWithSolution (src/mesh/mesh_advance.cu:184-243). */
+int  agal_refine_with_solution(agal_ctx *ctx, int out_counts[4]);
+/* Mesh::M_ComputeRefCriteria(.., V_MESH_REF_UNIFORM) (src/mesh/mesh_criterion.cu:102-111). */
+int  agal_mark_uniform(agal_ctx *ctx, int L);
+/* Sets cblock_ID_ref of the listed blocks to `mark` (the reference's refine-region hook
+ * Cu_RefineRegion, inc/custom.h:15-42, marks blocks the same way). */
+int  agal_mark_blocks(agal_ctx *ctx, const int *ids, int n, int mark);
+/* Mesh::M_FreezeRefinedCells (src/mesh/mesh_amr.cu:2049-2074), var 0 freeze / 1 unfreeze. */
+int  agal_freeze_refined(agal_ctx *ctx, int var);
+/* Mesh::M_Advance_Step (src/mesh/mesh_advance.cu:155-182) followed, when (i+1)%P_REFINE==0, by
+ * M_Advance_RefineWithSolution -- the body of M_AdvanceLoop (mesh_advance.cu:421-463) for
+ * coarse steps [i0, i0+n).  Rebuild time is reported separately in out_rebuild_ms (may be NULL). */
+int  agal_advance_loop(agal_ctx *ctx, int i0, int n, double *out_step_ms, double *out_rebuild_ms);
+
+/* ---- data exchange (M_RetrieveFromGPU / M_LoadToGPU, src/mesh/mesh_output.cu:4-82) ---------
+ * Arrays are copied for block IDs [0, n_blocks).  Element layouts follow the reference's SoA:
+ *   populations: Q arrays of n_blocks*cells_per_block values (dtype = precision), physical order
+ *   macros:      4 arrays (rho,u,v,w) of n_blocks*cells_per_block
+ *   cell masks:  n_blocks*cells_per_block ints
+ *   nbr, nbr_child: n_q_max arrays of n_blocks ints (velocity-ordered, self at p=0)
+ *   per-block ints (ref, level, onb, block mask): n_blocks
+ *   coords: n_dim arrays of n_blocks floats
+ * `dst_stride` / `src_stride` is the distance in elements between consecutive arrays. */
+int  agal_download_populations(agal_ctx *ctx, void *dst, long long n_blocks, long long dst_stride);
+int  agal_upload_populations(agal_ctx *ctx, const void *src, long long n_blocks, long long src_stride);
+int  agal_download_macros(agal_ctx *ctx, void *dst, long long n_blocks, long long dst_stride);
+int  agal_download_cell_masks(agal_ctx *ctx, int *dst, long long n_blocks);
+/* M_LoadToGPU (src/mesh/mesh_output.cu:4-42) for one array: overwrite cell masks / a block table
+ * (e.g. a restart or a voxelised geometry); per-level lists are rebuilt afterwards. */
+int  agal_upload_cell_masks(agal_ctx *ctx, const int *src, long long n_blocks);
+int  agal_upload_block_ints(agal_ctx *ctx, int which, const int *src, long long n_blocks, long long src_stride);
+int  agal_download_block_ints(agal_ctx *ctx, int which, int *dst, long long n_blocks, long long dst_stride);
+int  agal_download_coords(agal_ctx *ctx, float *dst, long long n_blocks, long long dst_stride);
+int  agal_download_id_set(agal_ctx *ctx, int L, int *dst, int n);
+int  agal_download_gap_set(agal_ctx *ctx, int *dst, int n);
+/* `which` values for agal_download_block_ints. */
+#define AGAL_BLK_NBR        0
+#define AGAL_BLK_NBR_CHILD  1
+#define AGAL_BLK_REF        2
+#define AGAL_BLK_LEVEL      3
+#define AGAL_BLK_ONB        4
+#define AGAL_BLK_MASK       5
+
+/* ---- measurement ----------------------------------------------------------------------------- */
+/* Average device time (ms) per launch of the fused level kernel since the last reset, measured
+ * with HIP events on the context stream; `launches` receives the launch count. */
+int  agal_timing_reset(agal_ctx *ctx, int enable);
+int  agal_timing_get(agal_ctx *ctx, double *fused_ms_total, long long *fused_launches,
+                     double *fused_cells_total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AGAL_CAPI_H */

@@ -1,0 +1,1200 @@
+/*
+ * agal_oracle.hpp -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+ *
+ * CPU restatement of KhodrJ/AGAL's octree-AMR lattice-Boltzmann step, used only as the parity
+ * checker of the HIP library (tests/, __graft_entry__.smoke(), bench.py's cpu_baseline leg).
+ * Nothing in agal_amd/ links, loads or calls it.
+ *
+ * It follows the reference expression for expression: the same slot convention (direction p of
+ * a pre-collision population lives in slot pb(p) between steps), the same p-order sums, the same
+ * guards (solid cells hold -1, the stream swaps only when the partner value is > 0) and the same
+ * integer bookkeeping of the block pool (gap stack, sorted per-level ID sets, neighbour and
+ * neighbour-child tables).  Each function cites the reference lines it restates.
+ *
+ * Pinning (see tests/): velocity tables against the reference's text (tests/golden), isotropy
+ * closed forms, equilibrium fixed point and conservation of BGK, stream as a permutation and
+ * single-population translation, Couette closed form for the moving-wall hook, linear
+ * reproduction of interpolation, average of constants, criterion on analytic shear fields, child
+ * connectivity and cell-mask rules against tables parsed from the reference source, and AMR
+ * invariants plus brute-force neighbour reconstruction from block coordinates.
+ *
+ * Deliberate difference from the reference, kept identical in the HIP library:
+ *   - floor(log2(x)) in the refinement criterion is evaluated exactly (ilogb) instead of through
+ *     log2f, whose last-bit rounding differs between math libraries; the two differ only when x
+ *     lies within one ulp below a power of two.
+ *   - id_max of an empty level is 0 (the reference dereferences an empty thrust range there,
+ *     src/mesh/mesh_amr.cu:2629); id_max is only a loop bound over inactive IDs.
+ */
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "../include/agal_capi.h"
+
+namespace agal_oracle {
+
+constexpr int N_SKIPID = -999;
+
+constexpr int REF_UNREFINED = 0, REF_UNREFINED_VIO = 9, REF_REFINED = 1, REF_REFINED_WCHILD = 2,
+              REF_REFINED_PERM = 3, REF_REFINED_WCHILD_PERM = 10, REF_MARK_REFINE = 4,
+              REF_MARK_COARSEN = 5, REF_NEW = 6, REF_REMOVE = 7, REF_INACTIVE = 8;
+constexpr int CELL_INTERIOR = 0, CELL_INTERFACE = 1, CELL_GHOST = 2, CELL_SOLID = -1;
+constexpr int BLK_REGULAR = 0, BLK_INTERFACE = 1, BLK_SOLID = -3, BLK_SOLIDA = -2;
+constexpr int INTERP_INTERFACE = 0, INTERP_ADDED = 1;
+constexpr int AVERAGE_INTERFACE = 0, AVERAGE_BLOCK = 1, AVERAGE_GRID = 2;
+
+/* Velocity sets: V_CONN_ID / V_CONN_MAP (inc/util.h:94-101), LBMw / LBMpb
+ * (src/solver_lbm/solver_lbm_init.cu:79-88).  The neighbour tables of the mesh always use the
+ * full 9 (2D) or 27 (3D) directions in D2Q9 / D3Q27 order. */
+struct VelSet {
+    int Q = 0, DIM = 0, Q_max = 0;
+    int c[3][27] = {};
+    double w[27] = {};
+    int pb[27] = {};
+    int conn_map[27] = {};
+};
+
+inline VelSet make_velset(int Q)
+{
+    VelSet V;
+    static const int c2[3][9] = {{0, 1, 0, -1, 0, 1, -1, -1, 1},
+                                 {0, 0, 1, 0, -1, 1, 1, -1, -1},
+                                 {0, 0, 0, 0, 0, 0, 0, 0, 0}};
+    static const int c3[3][27] = {
+        {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1},
+        {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1},
+        {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1}};
+    V.Q = Q;
+    V.DIM = (Q == 9) ? 2 : 3;
+    V.Q_max = (V.DIM == 2) ? 9 : 27;
+    for (int d = 0; d < 3; d++)
+        for (int p = 0; p < V.Q_max; p++)
+            V.c[d][p] = (V.DIM == 2) ? c2[d][p] : c3[d][p];
+    // Opposite direction: the unique index with c = -c_p.
+    for (int p = 0; p < V.Q_max; p++)
+        for (int q = 0; q < V.Q_max; q++)
+            if (V.c[0][q] == -V.c[0][p] && V.c[1][q] == -V.c[1][p] && V.c[2][q] == -V.c[2][p])
+                V.pb[p] = q;
+    // conn_map[S] = velocity index of the 3x3(x3) slot S = (dx+1) + 3(dy+1) + 9(dz+1).
+    for (int p = 0; p < V.Q_max; p++) {
+        int S = (V.c[0][p] + 1) + 3 * (V.c[1][p] + 1) + (V.DIM == 3 ? 9 * (V.c[2][p] + 1) : 0);
+        V.conn_map[S] = p;
+    }
+    if (Q == 9) {
+        double w[9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
+                       1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+        for (int p = 0; p < 9; p++) V.w[p] = w[p];
+    } else if (Q == 19) {
+        V.w[0] = 1.0 / 3.0;
+        for (int p = 1; p < 7; p++) V.w[p] = 1.0 / 18.0;
+        for (int p = 7; p < 19; p++) V.w[p] = 1.0 / 36.0;
+    } else {
+        V.w[0] = 8.0 / 27.0;
+        for (int p = 1; p < 7; p++) V.w[p] = 2.0 / 27.0;
+        for (int p = 7; p < 19; p++) V.w[p] = 1.0 / 54.0;
+        for (int p = 19; p < 27; p++) V.w[p] = 1.0 / 216.0;
+    }
+    return V;
+}
+
+/* Half set of directions that initiate a swap in Cu_Stream / pair in Cu_ImposeBC
+ * (solver_lbm_kernels.cu:597,769). */
+inline bool in_half_set(int dim, int p)
+{
+    if (dim == 2) return p == 1 || (p + 1) % 3 == 0;
+    return p == 26 || ((p - 1) % 2 == 0 && p < 25);
+}
+
+/* Cu_NbrMap (inc/util_index.h:43-66): 3x3(x3) slot of the block holding local cell (I,J,K). */
+inline int nbr_map(int dim, int I, int J, int K)
+{
+    int S = 0;
+    if (I >= 4) S = 2;
+    if (I >= 0 && I < 4) S = 1;
+    if (J >= 4) S += 3 * 2;
+    if (J >= 0 && J < 4) S += 3 * 1;
+    if (dim == 3) {
+        if (K >= 4) S += 9 * 2;
+        if (K >= 0 && K < 4) S += 9 * 1;
+    }
+    return S;
+}
+
+/* Cu_NbrCellId (inc/util_index.h:88-98). */
+inline int nbr_cell_id(int dim, int I, int J, int K)
+{
+    I = (4 + (I % 4)) % 4;
+    J = (4 + (J % 4)) % 4;
+    if (dim == 3) K = (4 + (K % 4)) % 4;
+    return (dim == 2) ? I + 4 * J : I + 4 * J + 16 * K;
+}
+
+/* Exact floor(log2(x)) for float/double x (see header note). */
+template <typename T>
+inline T floor_log2_exact(T x)
+{
+    if (std::isnan(x)) return x;
+    if (x == T(0)) return -std::numeric_limits<T>::infinity();
+    if (std::isinf(x)) return x;  // +inf (x >= 0 here)
+    return T(std::ilogb(x));
+}
+
+template <typename T>
+struct Oracle {
+    agal_params P{};
+    VelSet V;
+    int DIM = 3, Q = 19, Q_max = 27, M_CBLOCK = 64, N_CHILDREN = 8, N_U = 23;
+    long n_maxcblocks = 0, n_maxcells = 0;
+    int Nxi[3] = {0, 0, 0};
+    int MAX_LEVELS = 1, MAX_LEVELS_INTERIOR = 1, MAX_LEVELS_WALL = 1, N_LEVEL_START = 0;
+    T dx = 1, dx_cblock = 1;
+    std::vector<T> dxf_vec, tau_vec;
+    int n_coarsecblocks = 0;
+
+    // Mesh-owned arrays (inc/mesh.h:367-462).  F holds N_Q populations then rho,u,v,w.
+    std::vector<T> F;
+    std::vector<int> cells_mask;
+    std::vector<float> X;
+    std::vector<int> blk_mask, nbr, nbr_child, onb, ref, level;
+    std::vector<int> id_set, n_ids, id_max, gap_set;
+    int n_gaps = 0;
+    int global_Liter = 0;
+
+    // ---------------------------------------------------------------------------------------
+    // Mesh<..>::M_Init (src/mesh/mesh_init.cu:10-567) and Solver_LBM::S_Init (solver_lbm_init.cu).
+    // ---------------------------------------------------------------------------------------
+    int init(const agal_params &p, long long capacity)
+    {
+        P = p;
+        V = make_velset(P.velocity_set);
+        Q = V.Q;
+        DIM = V.DIM;
+        Q_max = V.Q_max;
+        M_CBLOCK = (DIM == 2) ? 16 : 64;
+        N_CHILDREN = (DIM == 2) ? 4 : 8;
+        N_U = Q + 4;
+        // mesh_init.cu:22-29: Lx is a float member (inc/mesh.h:292).
+        float Lx = (float)P.L_c;
+        float Ly = (float)(P.L_fy * Lx);
+        float Lz = (float)(P.L_fz * Lx);
+        Nxi[0] = P.Nx;
+        Nxi[1] = (int)(P.Nx * (Ly / Lx));
+        Nxi[2] = (int)(P.Nx * (Lz / Lx));
+        dx = (T)(Lx / P.Nx);
+        dx_cblock = 4 * dx;
+        MAX_LEVELS_WALL = P.MAX_LEVELS_WALL;
+        MAX_LEVELS_INTERIOR = P.MAX_LEVELS_INTERIOR;
+        N_LEVEL_START = P.N_LEVEL_START;
+        if (DIM == 2) Nxi[2] = 1;
+        MAX_LEVELS = std::max(MAX_LEVELS_WALL, MAX_LEVELS_INTERIOR);
+        if (MAX_LEVELS == 1) N_LEVEL_START = 0;
+        if (N_LEVEL_START > MAX_LEVELS - 1) N_LEVEL_START = MAX_LEVELS - 1;
+        if (MAX_LEVELS > 8) return AGAL_ERR_ARG;
+
+        n_maxcblocks = capacity;
+        n_maxcells = n_maxcblocks * M_CBLOCK;
+        n_coarsecblocks = (Nxi[0] * Nxi[1] * Nxi[2]) / M_CBLOCK;
+        if ((long)Nxi[0] * Nxi[1] * Nxi[2] >= n_maxcells) return AGAL_ERR_CAPACITY;
+
+        dxf_vec.assign(MAX_LEVELS, 0);
+        for (int L = 0; L < MAX_LEVELS; L++)
+            dxf_vec[L] = (T)((double)dx / std::pow(2.0, (double)(T)L));
+        // solver_lbm_init.cu:47-52
+        tau_vec.assign(MAX_LEVELS, 0);
+        for (int L = 0; L < MAX_LEVELS; L++)
+            tau_vec[L] = (T)(P.v0 * 3.0 + 0.5 * (double)dxf_vec[L]);
+
+        F.assign((size_t)N_U * n_maxcells, T(0));
+        cells_mask.assign(n_maxcells, CELL_INTERIOR);
+        X.assign((size_t)DIM * n_maxcblocks, -1.0f);
+        blk_mask.assign(n_maxcblocks, BLK_REGULAR);
+        nbr.assign((size_t)Q_max * n_maxcblocks, N_SKIPID);
+        nbr_child.assign((size_t)Q_max * n_maxcblocks, N_SKIPID);
+        ref.assign(n_maxcblocks, REF_INACTIVE);
+        level.assign(n_maxcblocks, 0);
+        onb.assign(n_maxcblocks, 0);
+        id_set.assign((size_t)MAX_LEVELS * n_maxcblocks, 0);
+        n_ids.assign(MAX_LEVELS + 1, 0);
+        id_max.assign(MAX_LEVELS + 1, 0);
+        gap_set.assign(n_maxcblocks, -1);
+
+        // Root grid over a padded structured index grid (mesh_init.cu:325-501).
+        int bx = Nxi[0] / 4, by = Nxi[1] / 4;
+        int gh = (DIM == 2) ? 1 : Nxi[2] / 4;
+        int ghm = (DIM == 2) ? 0 : 1;
+        int px = bx + 2, py = by + 2, pz = gh + 2;
+        std::vector<int> grid((size_t)px * py * pz, 0);
+        auto G = [&](int i, int j, int k) -> int & { return grid[(size_t)i + px * ((size_t)j + py * k)]; };
+        int ID_counter = 0;
+        for (int k = 1 * ghm; k < gh * ghm + 2 - 1; k++)
+            for (int j = 1; j < by + 2 - 1; j++)
+                for (int i = 1; i < bx + 2 - 1; i++)
+                    G(i, j, k) = ID_counter++;
+        if (DIM == 3) {
+            for (int j = 0; j < by + 2; j++)
+                for (int i = 0; i < bx + 2; i++) {
+                    if (P.PERIODIC_Z == 1) {
+                        G(i, j, 0) = G(i, j, gh + 2 - 1 - 1);
+                        G(i, j, gh + 2 - 1) = G(i, j, 1);
+                    } else {
+                        G(i, j, 0) = -5;
+                        G(i, j, gh + 2 - 1) = -6;
+                    }
+                }
+        }
+        for (int k = 0; k < gh + 2 * ghm; k++)
+            for (int i = 0; i < bx + 2; i++) {
+                if (P.PERIODIC_Y == 1) {
+                    G(i, 0, k) = G(i, by + 2 - 1 - 1, k);
+                    G(i, by + 2 - 1, k) = G(i, 1, k);
+                } else {
+                    G(i, 0, k) = -3;
+                    G(i, by + 2 - 1, k) = -4;
+                }
+            }
+        for (int k = 0; k < gh + 2 * ghm; k++)
+            for (int j = 0; j < by + 2; j++) {
+                if (P.PERIODIC_X == 1) {
+                    G(0, j, k) = G(bx + 2 - 1 - 1, j, k);
+                    G(bx + 2 - 1, j, k) = G(1, j, k);
+                } else {
+                    G(0, j, k) = -1;
+                    G(bx + 2 - 1, j, k) = -2;
+                }
+            }
+        for (int k = 1 * ghm; k < gh * ghm + 2 - 1; k++)
+            for (int j = 1; j < by + 2 - 1; j++)
+                for (int i = 1; i < bx + 2 - 1; i++) {
+                    int b = G(i, j, k);
+                    if (b < 0) continue;
+                    X[b + 0 * n_maxcblocks] = (float)((T)(i - 1) * dx_cblock);
+                    X[b + 1 * n_maxcblocks] = (float)((T)(j - 1) * dx_cblock);
+                    if (DIM == 3) X[b + 2 * n_maxcblocks] = (float)((T)(k - 1) * dx_cblock);
+                    int on_b = 0;
+                    for (int p = 0; p < Q_max; p++) {
+                        int nb = G(i + V.c[0][p], j + V.c[1][p], ghm * (k + V.c[2][p]));
+                        nbr[b + p * n_maxcblocks] = nb;
+                        if (nb < 0) on_b = 1;
+                        nbr_child[b + p * n_maxcblocks] = nb < 0 ? nb : N_SKIPID;
+                    }
+                    onb[b] = on_b;
+                }
+        for (int kap = 0; kap < n_coarsecblocks; kap++) ref[kap] = REF_UNREFINED;
+        n_ids[0] = n_coarsecblocks;
+        id_max[0] = id_max[MAX_LEVELS] = n_ids[0];
+        for (int xi = 0; xi < n_ids[0]; xi++) id_set[xi] = xi;
+        for (long xi = 0; xi < n_maxcblocks - n_ids[0]; xi++) gap_set[xi] = (int)((n_maxcblocks - 1) - xi);
+        n_gaps = (int)(n_maxcblocks - n_ids[0]);
+        return AGAL_OK;
+    }
+
+    T *Fp(int slot) { return &F[(size_t)slot * n_maxcells]; }
+    const int *idset(int L) const { return &id_set[(size_t)L * n_maxcblocks]; }
+    void cell_ijk(int t, int &I, int &J, int &K) const
+    {
+        I = t % 4;
+        J = (t / 4) % 4;
+        K = (DIM == 3) ? (t / 4) / 4 : 0;
+    }
+
+    // Cu_ComputeIC hook (inc/custom.h:48-60): a uniform state from the parameters.
+    void ic_state(T &rho, T &u, T &v, T &w) const
+    {
+        rho = (T)P.ic_rho;
+        u = (T)P.ic_u;
+        v = (T)P.ic_v;
+        w = (T)P.ic_w;
+    }
+
+    // Cu_ImposeBC hook (inc/custom.h:66-123) for the named presets.
+    T bc_hook(int nbr_id, T f, T rho, T u, T v, T w, T wp, T cxp, T cyp, T czp) const
+    {
+        (void)rho;
+        const T U = (T)P.bc_velocity;
+        int pre = P.bc_preset;
+        bool wall = false, outflow = false;
+        if (pre == AGAL_BC_LDC2D) wall = (nbr_id == -4);
+        if (pre == AGAL_BC_LDC3D) wall = (nbr_id == -6);
+        if (pre == AGAL_BC_FPSC2D) { wall = (nbr_id == -1 || nbr_id == -3 || nbr_id == -4); outflow = (nbr_id == -2); }
+        if (pre == AGAL_BC_FPSC3D) {
+            wall = (nbr_id == -1 || nbr_id == -3 || nbr_id == -4 || nbr_id == -5 || nbr_id == -6);
+            outflow = (nbr_id == -2);
+        }
+        if (wall) {
+            T cdotu = (T)(3.0) * (cxp * U);
+            return f - (T)(2.0) * wp * cdotu;
+        }
+        if (outflow) {
+            T cdotu = cxp * u + cyp * v + czp * w;
+            cdotu = (T)(1.0) + (T)(4.5) * cdotu * cdotu - (T)(1.5) * (u * u + v * v + w * w);
+            return -f + (T)(2.0) * wp * cdotu;
+        }
+        return f;
+    }
+
+    // Equilibrium in the reference's operation order (solver_lbm_kernels.cu:282).
+    T feq(int p, T rho, T u, T v, T w, T udotu) const
+    {
+        T cdotu = (T)V.c[0][p] * u + (T)V.c[1][p] * v + (T)V.c[2][p] * w;
+        return (T)V.w[p] * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
+    }
+
+    // ---------------------------------------------------------------------------------------
+    // Cu_SetInitialConditions (solver_lbm_kernels.cu:13-77).
+    // ---------------------------------------------------------------------------------------
+    void set_ic(int L)
+    {
+        const int nL = n_ids[L];
+        const int *ids = idset(L);
+#pragma omp parallel for schedule(static)
+        for (int bi = 0; bi < nL; bi++) {
+            int b = ids[bi];
+            if (b < 0) continue;
+            for (int t = 0; t < M_CBLOCK; t++) {
+                T rho = 0, u = 0, v = 0, w = 0;
+                ic_state(rho, u, v, w);
+                T udotu = u * u + v * v + w * w;
+                long cell = (long)b * M_CBLOCK + t;
+                int valid_mask = cells_mask[cell];
+                for (int p = 0; p < Q; p++) {
+                    T cdotu = (T)V.c[0][p] * u + (T)V.c[1][p] * v + (T)V.c[2][p] * w;
+                    T f_p = (T)(-1.0);
+                    if (valid_mask != CELL_SOLID)
+                        f_p = rho * (T)V.w[p] * ((T)1.0 + (T)3.0 * cdotu + (T)4.5 * cdotu * cdotu - (T)1.5 * udotu);
+                    Fp(V.pb[p])[cell] = f_p;
+                }
+                Fp(Q + 0)[cell] = rho;
+                Fp(Q + 1)[cell] = u;
+                Fp(Q + 2)[cell] = v;
+                if (DIM == 3) Fp(Q + 3)[cell] = w;
+            }
+        }
+    }
+
+    // Moments of the pre-collision populations (read from slots pb(p)), p order.
+    void moments(long cell, T &rho, T &u, T &v, T &w)
+    {
+        rho = 0; u = 0; v = 0; w = 0;
+        for (int p = 0; p < Q; p++) {
+            T f = Fp(V.pb[p])[cell];
+            rho += f;
+            u += V.c[0][p] * f;
+            v += V.c[1][p] * f;
+            if (DIM == 3) w += V.c[2][p] * f;
+        }
+        u /= rho;
+        v /= rho;
+        if (DIM == 3) w /= rho;
+    }
+
+    // Cu_RefreshVariables (solver_lbm_kernels.cu:110-159).
+    void refresh(int L)
+    {
+        const int nL = n_ids[L];
+        const int *ids = idset(L);
+#pragma omp parallel for schedule(static)
+        for (int bi = 0; bi < nL; bi++) {
+            int b = ids[bi];
+            if (b < 0) continue;
+            for (int t = 0; t < M_CBLOCK; t++) {
+                long cell = (long)b * M_CBLOCK + t;
+                T rho, u, v, w;
+                moments(cell, rho, u, v, w);
+                Fp(Q + 0)[cell] = rho;
+                Fp(Q + 1)[cell] = u;
+                Fp(Q + 2)[cell] = v;
+                if (DIM == 3) Fp(Q + 3)[cell] = w;
+            }
+        }
+    }
+
+    bool block_streams(int b) const
+    {
+        // solver_lbm_kernels.cu:236,570: skipped only when the block has children and is solid.
+        return nbr_child[b] < 0 || blk_mask[b] != BLK_SOLID;
+    }
+
+    // Cu_Collide (solver_lbm_kernels.cu:204-289) + S_Collide (:464-510).
+    void collide(int L)
+    {
+        if (n_ids[L] <= 0) return;
+        if (MAX_LEVELS == 1) refresh(L);
+        const int nL = n_ids[L];
+        const int *ids = idset(L);
+        const T dx_L = dxf_vec[L], tau_L = tau_vec[L];
+#pragma omp parallel for schedule(static)
+        for (int bi = 0; bi < nL; bi++) {
+            int b = ids[bi];
+            if (b < 0 || !block_streams(b)) continue;
+            for (int t = 0; t < M_CBLOCK; t++) {
+                long cell = (long)b * M_CBLOCK + t;
+                int valid_mask = cells_mask[cell];
+                T rho = 0, u = 0, v = 0, w = 0;
+                T f_p[27];
+                for (int p = 0; p < Q; p++) {
+                    T f_pi = Fp(V.pb[p])[cell];
+                    rho += f_pi;
+                    u += V.c[0][p] * f_pi;
+                    v += V.c[1][p] * f_pi;
+                    if (DIM == 3) w += V.c[2][p] * f_pi;
+                    f_p[p] = f_pi;
+                }
+                u /= rho;
+                v /= rho;
+                if (DIM == 3) w /= rho;
+                T udotu = u * u + v * v + w * w;
+                Fp(Q + 0)[cell] = rho;
+                Fp(Q + 1)[cell] = u;
+                Fp(Q + 2)[cell] = v;
+                if (DIM == 3) Fp(Q + 3)[cell] = w;
+                T omeg = dx_L / tau_L;
+                T omegp = (T)(1.0) - omeg;
+                for (int p = 0; p < Q; p++) {
+                    T cdotu = (T)V.c[0][p] * u + (T)V.c[1][p] * v + (T)V.c[2][p] * w;
+                    T f_pi = f_p[p] * omegp +
+                             ((T)V.w[p] * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu)) * omeg;
+                    if (valid_mask != CELL_SOLID) Fp(p)[cell] = f_pi;
+                }
+            }
+        }
+    }
+
+    void load_block_nbrs(int b, int *s_nbr) const
+    {
+        // s_ID_nbr[S] = cblock_ID_nbr[b + V_CONN_MAP[S]*n] (solver_lbm_kernels.cu:579-583).
+        for (int S = 0; S < Q_max; S++) s_nbr[S] = nbr[b + (size_t)V.conn_map[S] * n_maxcblocks];
+    }
+
+    // Cu_ImposeBC (solver_lbm_kernels.cu:691-887), curved-boundary variants excluded (S_BC_TYPE 0).
+    void impose_bc(int L)
+    {
+        if (n_ids[L] <= 0) return;
+        const int nL = n_ids[L];
+        const int *ids = idset(L);
+#pragma omp parallel for schedule(static)
+        for (int bi = 0; bi < nL; bi++) {
+            int b = ids[bi];
+            if (b < 0 || onb[b] != 1) continue;
+            int s_nbr[27];
+            load_block_nbrs(b, s_nbr);
+            for (int t = 0; t < M_CBLOCK; t++) {
+                long cell = (long)b * M_CBLOCK + t;
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                int valid_mask = cells_mask[cell];
+                T rho = Fp(Q + 0)[cell], u = Fp(Q + 1)[cell], v = Fp(Q + 2)[cell];
+                T w = (DIM == 3) ? Fp(Q + 3)[cell] : (T)0;
+                for (int p = 1; p < Q; p++) {
+                    if (!in_half_set(DIM, p)) continue;
+                    int pb = V.pb[p];
+                    T f_p = Fp(p)[cell];
+                    T f_q = Fp(pb)[cell];
+                    {
+                        int nb = s_nbr[nbr_map(DIM, I + V.c[0][p], J + V.c[1][p], (DIM == 3) ? K + V.c[2][p] : 0)];
+                        if (nb < 0 && nb != N_SKIPID)
+                            f_p = bc_hook(nb, f_p, rho, u, v, w, (T)V.w[p], (T)V.c[0][p], (T)V.c[1][p], (T)V.c[2][p]);
+                    }
+                    {
+                        int nb = s_nbr[nbr_map(DIM, I + V.c[0][pb], J + V.c[1][pb], (DIM == 3) ? K + V.c[2][pb] : 0)];
+                        if (nb < 0 && nb != N_SKIPID)
+                            f_q = bc_hook(nb, f_q, rho, u, v, w, (T)V.w[pb], (T)V.c[0][pb], (T)V.c[1][pb], (T)V.c[2][pb]);
+                    }
+                    if (valid_mask != CELL_SOLID) {
+                        Fp(p)[cell] = f_p;
+                        Fp(pb)[cell] = f_q;
+                    }
+                }
+            }
+        }
+    }
+
+    // Cu_Stream (solver_lbm_kernels.cu:532-633): in-place swap over the half set.
+    void stream(int L)
+    {
+        if (n_ids[L] <= 0) return;
+        const int nL = n_ids[L];
+        const int *ids = idset(L);
+#pragma omp parallel for schedule(static)
+        for (int bi = 0; bi < nL; bi++) {
+            int b = ids[bi];
+            if (b < 0 || !block_streams(b)) continue;
+            int s_nbr[27];
+            load_block_nbrs(b, s_nbr);
+            for (int t = 0; t < M_CBLOCK; t++) {
+                long cell = (long)b * M_CBLOCK + t;
+                int valid_mask = cells_mask[cell];
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                for (int p = 1; p < Q; p++) {
+                    if (!in_half_set(DIM, p)) continue;
+                    T f_p = Fp(p)[cell];
+                    T dQ = (T)(-1.0);
+                    int Ip = I + V.c[0][p], Jp = J + V.c[1][p], Kp = (DIM == 3) ? K + V.c[2][p] : 0;
+                    int nb = s_nbr[nbr_map(DIM, Ip, Jp, Kp)];
+                    int nc = nbr_cell_id(DIM, Ip, Jp, Kp);
+                    T f_pb = (T)(-1.0);
+                    if (nb > -1) f_pb = Fp(V.pb[p])[(long)nb * M_CBLOCK + nc];
+                    if (valid_mask != CELL_SOLID && f_pb > 0 && dQ < 0) {
+                        Fp(V.pb[p])[(long)nb * M_CBLOCK + nc] = f_p;
+                        Fp(p)[cell] = f_pb;
+                    }
+                }
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------------------------------
+    // Cu_Interpolate_Linear (solver_lbm_kernels_comm.cu:13-138) + S_Interpolate_Linear (:141-155).
+    // ---------------------------------------------------------------------------------------
+    void interpolate(int L, int var)
+    {
+        if (L >= MAX_LEVELS - 1) return;  // M_Interpolate guard (src/mesh/mesh_comm.cu:9)
+        refresh(L);
+        if (n_ids[L] <= 0) return;
+        const T tau_ratio = tau_vec[L + 1] / tau_vec[L];
+        const int nL = n_ids[L];
+        const int *ids = idset(L);
+#pragma omp parallel for schedule(static)
+        for (int bi = 0; bi < nL; bi++) {
+            int b = ids[bi];
+            if (b < 0) continue;
+            int i_kap_bc = nbr_child[b];
+            bool b1 = (var == INTERP_INTERFACE) && (blk_mask[b] == BLK_INTERFACE);
+            bool b2 = (var == INTERP_ADDED) && (ref[b] == REF_MARK_REFINE);
+            if (!(b1 || b2) || i_kap_bc < 0) continue;
+            T s_F[64];
+            T Ixs[64], Iys[64], Izs[64];
+            T rho[64], uu[64], vv[64], ww[64];
+            for (int t = 0; t < M_CBLOCK; t++) {
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                Ixs[t] = (T)(-0.25) + (T)(0.5) * (T)I;
+                Iys[t] = (T)(-0.25) + (T)(0.5) * (T)J;
+                Izs[t] = (DIM == 3) ? (T)(-0.25) + (T)(0.5) * (T)K : (T)0;
+                long cell = (long)b * M_CBLOCK + t;
+                rho[t] = Fp(Q + 0)[cell];
+                uu[t] = Fp(Q + 1)[cell];
+                vv[t] = Fp(Q + 2)[cell];
+                ww[t] = (DIM == 3) ? Fp(Q + 3)[cell] : (T)0;
+            }
+            for (int p = 0; p < Q; p++) {
+                for (int t = 0; t < M_CBLOCK; t++) {
+                    long cell = (long)b * M_CBLOCK + t;
+                    T udotu = uu[t] * uu[t] + vv[t] * vv[t] + ww[t] * ww[t];
+                    T f_p = Fp(V.pb[p])[cell];
+                    T fe = feq(p, rho[t], uu[t], vv[t], ww[t], udotu);
+                    s_F[t] = fe + (f_p - fe) * tau_ratio;
+                }
+                for (int t = 0; t < M_CBLOCK; t++) {
+                    T Ix = Ixs[t], Iy = Iys[t], Iz = Izs[t];
+                    if (DIM == 2) {
+                        for (int cj = 0; cj < 2; cj++)
+                            for (int ci = 0; ci < 2; ci++) {
+                                int c = ci + 2 * cj;
+                                int s = 2 * ci + 2 * 4 * cj;
+                                long ccell = (long)(i_kap_bc + c) * M_CBLOCK + t;
+                                if ((var == 0 && cells_mask[ccell] == CELL_GHOST) || (var == 1))
+                                    Fp(V.pb[p])[ccell] = s_F[0 + s] + (s_F[1 + s] - s_F[0 + s]) * Ix +
+                                                         (s_F[4 + s] - s_F[0 + s]) * Iy +
+                                                         (s_F[5 + s] - s_F[4 + s] - s_F[1 + s] + s_F[0 + s]) * Ix * Iy;
+                            }
+                    } else {
+                        for (int ck = 0; ck < 2; ck++)
+                            for (int cj = 0; cj < 2; cj++)
+                                for (int ci = 0; ci < 2; ci++) {
+                                    int c = ci + 2 * cj + 4 * ck;
+                                    int s = 2 * ci + 2 * 4 * cj + 2 * 4 * 4 * ck;
+                                    long ccell = (long)(i_kap_bc + c) * M_CBLOCK + t;
+                                    if ((var == 0 && cells_mask[ccell] == CELL_GHOST) || (var == 1))
+                                        Fp(V.pb[p])[ccell] =
+                                            s_F[0 + s] + (s_F[1 + s] - s_F[0 + s]) * Ix + (s_F[4 + s] - s_F[0 + s]) * Iy +
+                                            (s_F[16 + s] - s_F[0 + s]) * Iz +
+                                            (s_F[5 + s] - s_F[1 + s] - s_F[4 + s] + s_F[0 + s]) * Ix * Iy +
+                                            (s_F[17 + s] - s_F[1 + s] - s_F[16 + s] + s_F[0 + s]) * Ix * Iz +
+                                            (s_F[20 + s] - s_F[4 + s] - s_F[16 + s] + s_F[0 + s]) * Iy * Iz +
+                                            (s_F[21 + s] + s_F[1 + s] + s_F[4 + s] + s_F[16 + s] - s_F[5 + s] - s_F[17 + s] -
+                                             s_F[20 + s] - s_F[0 + s]) * Ix * Iy * Iz;
+                                }
+                    }
+                }
+            }
+        }
+    }
+
+    // Cu_Average (solver_lbm_kernels_comm.cu:174-347) + S_Average (:351-370).
+    void average(int L, int var)
+    {
+        if (L >= MAX_LEVELS - 1) return;  // M_Average guard (src/mesh/mesh_comm.cu:21)
+        refresh(L + 1);
+        if (n_ids[L] <= 0) return;
+        const T tau_ratio = tau_vec[L] / tau_vec[L + 1];
+        const int nL = n_ids[L];
+        const int *ids = idset(L);
+#pragma omp parallel for schedule(static)
+        for (int bi = 0; bi < nL; bi++) {
+            int b = ids[bi];
+            if (b < 0) continue;
+            int i_kap_bc = nbr_child[b];
+            bool b1 = i_kap_bc > -1;
+            bool b2 = (var == AVERAGE_GRID || blk_mask[b] == BLK_INTERFACE);
+            if (!(b1 && b2)) continue;
+            int nck = (DIM == 3) ? 2 : 1;
+            for (int ck = 0; ck < nck; ck++)
+                for (int cj = 0; cj < 2; cj++)
+                    for (int ci = 0; ci < 2; ci++) {
+                        int c = ci + 2 * cj + 4 * ck;
+                        long cb = (long)(i_kap_bc + c) * M_CBLOCK;
+                        int s_mask[64];
+                        T s_Fc[64];
+                        for (int t = 0; t < M_CBLOCK; t++) {
+                            s_mask[t] = cells_mask[cb + t];
+                            if (var > 0 && s_mask[t] < 2) s_mask[t] = 1;
+                        }
+                        for (int p = 0; p < Q; p++) {
+                            for (int t = 0; t < M_CBLOCK; t++) {
+                                T rho = Fp(Q + 0)[cb + t], u = Fp(Q + 1)[cb + t], v = Fp(Q + 2)[cb + t];
+                                T w = (DIM == 3) ? Fp(Q + 3)[cb + t] : (T)0;
+                                T udotu = (DIM == 3) ? u * u + v * v + w * w : u * u + v * v;
+                                T f_p = Fp(V.pb[p])[cb + t];
+                                T cdotu = (DIM == 3) ? (T)V.c[0][p] * u + (T)V.c[1][p] * v + (T)V.c[2][p] * w
+                                                     : (T)V.c[0][p] * u + (T)V.c[1][p] * v;
+                                T fe = (T)V.w[p] * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
+                                s_Fc[t] = fe + (f_p - fe) * tau_ratio;
+                            }
+                            for (int t = 0; t < M_CBLOCK; t++) {
+                                int I, J, K;
+                                cell_ijk(t, I, J, K);
+                                int c0 = 2 * (I % 2) + 4 * (2 * (J % 2));
+                                if (DIM == 3) c0 += 4 * 4 * (2 * (K % 2));
+                                bool inq = (I >= 2 * ci) && (I < 2 + 2 * ci) && (J >= 2 * cj) && (J < 2 + 2 * cj);
+                                if (DIM == 3) inq = inq && (K >= 2 * ck) && (K < 2 + 2 * ck);
+                                if (s_mask[c0] == 1 && inq) {
+                                    long pcell = (long)b * M_CBLOCK + t;
+                                    if (DIM == 2)
+                                        Fp(V.pb[p])[pcell] = (T)(0.25) * (s_Fc[c0 + 0 + 4 * 0] + s_Fc[c0 + 1 + 4 * 0] +
+                                                                          s_Fc[c0 + 0 + 4 * 1] + s_Fc[c0 + 1 + 4 * 1]);
+                                    else
+                                        Fp(V.pb[p])[pcell] = (T)(0.125) * (s_Fc[c0 + 0 + 4 * 0 + 16 * 0] + s_Fc[c0 + 1 + 4 * 0 + 16 * 0] +
+                                                                           s_Fc[c0 + 0 + 4 * 1 + 16 * 0] + s_Fc[c0 + 1 + 4 * 1 + 16 * 0] +
+                                                                           s_Fc[c0 + 0 + 4 * 0 + 16 * 1] + s_Fc[c0 + 1 + 4 * 0 + 16 * 1] +
+                                                                           s_Fc[c0 + 0 + 4 * 1 + 16 * 1] + s_Fc[c0 + 1 + 4 * 1 + 16 * 1]);
+                                }
+                            }
+                        }
+                    }
+        }
+    }
+
+    void collide_stream(int L)
+    {
+        collide(L);
+        impose_bc(L);
+        stream(L);
+    }
+
+    // Solver_LBM::S_Advance (solver_lbm_advance.cu:30-174), forces excluded (they do not alter
+    // the populations).
+    void advance(int L)
+    {
+        if (MAX_LEVELS > 1 && (MAX_LEVELS != N_LEVEL_START + 1)) {
+            if (L == N_LEVEL_START) {
+                global_Liter = 0;
+                interpolate(L, INTERP_INTERFACE);
+                collide_stream(L);
+                advance(L + 1);
+                average(L, AVERAGE_INTERFACE);
+            } else {
+                if (L < MAX_LEVELS - 1) interpolate(L, INTERP_INTERFACE);
+                collide_stream(L);
+                if (L == MAX_LEVELS - 1) global_Liter++;
+                if (L < MAX_LEVELS - 1) {
+                    advance(L + 1);
+                    average(L, AVERAGE_INTERFACE);
+                    interpolate(L, INTERP_INTERFACE);
+                }
+                collide_stream(L);
+                if (L == MAX_LEVELS - 1) global_Liter++;
+                if (L < MAX_LEVELS - 1) {
+                    advance(L + 1);
+                    average(L, AVERAGE_INTERFACE);
+                }
+            }
+        } else {
+            collide_stream(L);
+        }
+    }
+
+    // ---------------------------------------------------------------------------------------
+    // Cu_ComputeRefCriteria (solver_lbm_criterion.cu:438-592) with ExtrapolateToHalo
+    // (inc/util_math.h:34-112) and BlockwiseMaximum (inc/util.h:786-799); dx of level 0 for all
+    // levels as the caller passes L = 0 (src/mesh/mesh_advance.cu:228).
+    // ---------------------------------------------------------------------------------------
+    static T extrap4(T d1, T d2, T d3, T d4)
+    {
+        return (T)(+4.0) * d1 + (T)(-6.0) * d2 + (T)(+4.0) * d3 + (T)(-1.0) * d4;
+    }
+
+    void ref_criteria()
+    {
+        const int idm = id_max[MAX_LEVELS];
+        const T dx_L = dxf_vec[0];
+        const T NRS = (T)P.N_REFINE_START, NRI = (T)P.N_REFINE_INC, NRM = (T)P.N_REFINE_MAX;
+        const int MLI = MAX_LEVELS_INTERIOR;
+#pragma omp parallel for schedule(static)
+        for (int b = 0; b < idm; b++) {
+            if (ref[b] == REF_INACTIVE) continue;
+            T su[216], sv[216], sw[216];
+            auto H = [&](int I, int J, int K) { return (DIM == 2) ? (I + 1) + 6 * (J + 1) : (I + 1) + 6 * (J + 1) + 36 * (K + 1); };
+            for (int t = 0; t < M_CBLOCK; t++) {
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                long cell = (long)b * M_CBLOCK + t;
+                su[H(I, J, K)] = Fp(Q + 1)[cell];
+                sv[H(I, J, K)] = Fp(Q + 2)[cell];
+                sw[H(I, J, K)] = (DIM == 3) ? Fp(Q + 3)[cell] : (T)0;
+            }
+            auto halo = [&](T *s) {
+                for (int t = 0; t < M_CBLOCK; t++) {
+                    int I, J, K;
+                    cell_ijk(t, I, J, K);
+                    if (I == 0) {
+                        s[H(-1, J, K)] = extrap4(s[H(0, J, K)], s[H(1, J, K)], s[H(2, J, K)], s[H(3, J, K)]);
+                        s[H(4, J, K)] = extrap4(s[H(3, J, K)], s[H(2, J, K)], s[H(1, J, K)], s[H(0, J, K)]);
+                    }
+                    if (J == 0) {
+                        s[H(I, -1, K)] = extrap4(s[H(I, 0, K)], s[H(I, 1, K)], s[H(I, 2, K)], s[H(I, 3, K)]);
+                        s[H(I, 4, K)] = extrap4(s[H(I, 3, K)], s[H(I, 2, K)], s[H(I, 1, K)], s[H(I, 0, K)]);
+                    }
+                    if (DIM == 3 && K == 0) {
+                        s[H(I, J, -1)] = extrap4(s[H(I, J, 0)], s[H(I, J, 1)], s[H(I, J, 2)], s[H(I, J, 3)]);
+                        s[H(I, J, 4)] = extrap4(s[H(I, J, 3)], s[H(I, J, 2)], s[H(I, J, 1)], s[H(I, J, 0)]);
+                    }
+                }
+            };
+            halo(su);
+            halo(sv);
+            if (DIM == 3) halo(sw);
+            T Wmax = 0;
+            bool first = true;
+            for (int t = 0; t < M_CBLOCK; t++) {
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                T uX = 0, uY = 0, uZ = 0, vX = 0, vY = 0, vZ = 0, wX = 0, wY = 0, wZ = 0;
+                if (DIM == 2) {
+                    uX = (T)((double)(su[H(I + 1, J, K)] - su[H(I - 1, J, K)]) / (2.0 * (double)dx_L));
+                    uY = (T)((double)(su[H(I, J + 1, K)] - su[H(I, J - 1, K)]) / (2.0 * (double)dx_L));
+                    vX = (T)((double)(sv[H(I + 1, J, K)] - sv[H(I - 1, J, K)]) / (2.0 * (double)dx_L));
+                    vY = (T)((double)(sv[H(I, J + 1, K)] - sv[H(I, J - 1, K)]) / (2.0 * (double)dx_L));
+                } else {
+                    T d2 = 2 * dx_L;
+                    uX = (su[H(I + 1, J, K)] - su[H(I - 1, J, K)]) / d2;
+                    uY = (su[H(I, J + 1, K)] - su[H(I, J - 1, K)]) / d2;
+                    uZ = (su[H(I, J, K + 1)] - su[H(I, J, K - 1)]) / d2;
+                    vX = (sv[H(I + 1, J, K)] - sv[H(I - 1, J, K)]) / d2;
+                    vY = (sv[H(I, J + 1, K)] - sv[H(I, J - 1, K)]) / d2;
+                    vZ = (sv[H(I, J, K + 1)] - sv[H(I, J, K - 1)]) / d2;
+                    wX = (sw[H(I + 1, J, K)] - sw[H(I - 1, J, K)]) / d2;
+                    wY = (sw[H(I, J + 1, K)] - sw[H(I, J - 1, K)]) / d2;
+                    wZ = (sw[H(I, J, K + 1)] - sw[H(I, J, K - 1)]) / d2;
+                }
+                T W = 0;
+                if (P.S_CRITERION == 0)
+                    W = floor_log2_exact<T>(std::sqrt((wY - vZ) * (wY - vZ) + (uZ - wX) * (uZ - wX) + (vX - uY) * (vX - uY)));
+                if (P.S_CRITERION == 1)
+                    W = floor_log2_exact<T>(std::fabs((T)0.5 * (uX * uX + vY * vY + wZ * wZ + (T)2.0 * (uY * vX + uZ * wX + wY * vZ))));
+                if (cells_mask[(long)b * M_CBLOCK + t] == CELL_GHOST) W = (T)0.0;
+                // fmax semantics: a NaN operand is ignored.
+                if (first) { Wmax = W; first = false; }
+                else Wmax = std::fmax(Wmax, W);
+            }
+            if (Wmax > NRM) Wmax = NRM;
+            int ref_kap = ref[b];
+            int level_kap = level[b];
+            int L_desired = MLI - 1;
+            for (int p = 1; p <= MLI - 1; p++)
+                if (Wmax < NRS - NRI * (T)p) L_desired = (MLI - 1) - p;
+            bool eligible = true;
+            for (int p = 0; p < Q_max; p++)
+                if (nbr[b + (size_t)p * n_maxcblocks] == N_SKIPID) eligible = false;
+            if (eligible && level_kap != MLI - 1 && ref_kap == REF_UNREFINED && L_desired > level_kap)
+                ref[b] = REF_MARK_REFINE;
+            if (ref_kap == REF_REFINED && L_desired < level_kap + 1)
+                ref[b] = REF_MARK_COARSEN;
+        }
+    }
+
+    // Cu_ComputeRefCriteria_Uniform (src/mesh/mesh_criterion.cu:67-85).
+    void mark_uniform(int L)
+    {
+        for (int k = 0; k < n_ids[L]; k++) {
+            int b = idset(L)[k];
+            if (ref[b] == REF_UNREFINED) ref[b] = REF_MARK_REFINE;
+        }
+    }
+
+    // Cu_FreezeRefined / Cu_UnfreezeRefined (src/mesh/mesh_amr.cu:1887-1923).
+    void freeze_refined(int var)
+    {
+        for (int kap = 0; kap < id_max[MAX_LEVELS]; kap++) {
+            if (var == 0) {
+                if (ref[kap] == REF_REFINED) ref[kap] = REF_REFINED_PERM;
+                if (ref[kap] == REF_REFINED_WCHILD) ref[kap] = REF_REFINED_WCHILD_PERM;
+            } else {
+                if (ref[kap] == REF_REFINED_PERM) ref[kap] = REF_REFINED;
+                if (ref[kap] == REF_REFINED_WCHILD_PERM) ref[kap] = REF_REFINED_WCHILD;
+            }
+        }
+    }
+
+    int N(int b, int p) const { return nbr[b + (size_t)p * n_maxcblocks]; }
+    int &Nn(int b, int p) { return nbr[b + (size_t)p * n_maxcblocks]; }
+    int &NC(int b, int p) { return nbr_child[b + (size_t)p * n_maxcblocks]; }
+
+    /* Velocity index of a direction vector (components in {-1,0,1}). */
+    int dir_index(int dx_, int dy_, int dz_) const
+    {
+        int S = (dx_ + 1) + 3 * (dy_ + 1) + (DIM == 3 ? 9 * (dz_ + 1) : 0);
+        return V.conn_map[S];
+    }
+
+    // ---------------------------------------------------------------------------------------
+    // Mesh::M_RefineAndCoarsenBlocks (src/mesh/mesh_amr.cu:2161-2744), kernels at :26-1923.
+    // Returns AGAL_ERR_POOL_EXHAUSTED when the refinement was cancelled.
+    // ---------------------------------------------------------------------------------------
+    int refine_coarsen(int out_counts[4])
+    {
+        int status = AGAL_OK;
+        const int NCH = N_CHILDREN;
+        int id_max_curr = id_max[MAX_LEVELS];
+        int n_ref = 0, n_coa = 0;
+        for (int k = 0; k < id_max_curr; k++) {
+            if (ref[k] == REF_MARK_REFINE) n_ref++;
+            if (ref[k] == REF_MARK_COARSEN) n_coa++;
+        }
+        int id_max_curr_wnew = id_max_curr + NCH * n_ref;
+        bool proceed_ref = n_ref > 0 && NCH * n_ref <= n_gaps;
+        bool proceed_coa = n_coa > 0;
+        if (!proceed_ref && n_ref > 0) {
+            status = AGAL_ERR_POOL_EXHAUSTED;
+            n_ref = 0;
+            id_max_curr_wnew = id_max_curr;
+            for (int k = 0; k < id_max_curr_wnew; k++)   // Cu_RefineCells_Cancel
+                if (ref[k] == REF_MARK_REFINE) ref[k] = REF_UNREFINED;
+        }
+        int n_added = 0, n_removed = 0;
+        // S1: map of marked blocks and their neighbours (c_tmp_8).
+        std::vector<int> map(n_maxcblocks, -1);
+        if (proceed_ref || proceed_coa) {
+            for (int k = 0; k < id_max_curr; k++) {
+                if (ref[k] == REF_MARK_REFINE || ref[k] == REF_MARK_COARSEN) {
+                    int self = N(k, 0);
+                    if (self < 0) continue;
+                    map[self] = self;
+                    for (int p = 1; p < Q_max; p++) {
+                        int nb = N(self, p);
+                        if (nb >= 0) map[nb] = nb;
+                    }
+                }
+            }
+        }
+        // S2: first-child IDs from the reversed tail of the gap stack.
+        std::vector<int> first_child(n_maxcblocks, -1);
+        if (proceed_ref) {
+            std::vector<int> list1;
+            for (int k = 0; k < id_max_curr; k++)
+                if (ref[k] == REF_MARK_REFINE) list1.push_back(k);
+            std::vector<int> tmp3(NCH * n_ref);
+            for (int j = 0; j < NCH * n_ref; j++) tmp3[j] = gap_set[n_gaps - 1 - j];
+            id_max_curr_wnew = std::max(id_max_curr_wnew, *std::max_element(tmp3.begin(), tmp3.end()) + 1);
+            for (int k = 0; k < n_ref; k++) first_child[list1[k]] = tmp3[NCH * k];
+            // Cu_RefineCells_S2_1
+            for (int kap = 0; kap < id_max_curr; kap++) {
+                if (map[kap] < 0) continue;
+                for (int p = 0; p < Q_max; p++) {
+                    int ip = N(kap, p);
+                    if (ip > -1 && ref[ip] == REF_MARK_REFINE) NC(kap, p) = first_child[ip];
+                }
+            }
+        }
+        // S3: quality control on coarsening.
+        if (proceed_coa) {
+            for (int kap = 0; kap < id_max_curr; kap++) {   // Cu_RefineCells_Q1_1
+                if (ref[kap] != REF_UNREFINED) continue;
+                bool has_refined_nbr = false;
+                for (int p = 1; p < Q_max; p++)
+                    if (NC(kap, p) >= 0) has_refined_nbr = true;
+                if (has_refined_nbr) ref[kap] = REF_UNREFINED_VIO;
+            }
+            std::vector<int> ref_snapshot(ref.begin(), ref.begin() + id_max_curr);
+            for (int kap = 0; kap < id_max_curr; kap++) {   // Cu_RefineCells_Q1_2
+                if (ref_snapshot[kap] != REF_MARK_COARSEN) continue;
+                int i_c = NC(kap, 0);
+                if (i_c < 0) continue;  // a coarsening mark on a block without children is ignored
+                bool viol = false;
+                for (int xc = 0; xc < NCH; xc++) {
+                    int r = ref[i_c + xc];
+                    if (r == REF_UNREFINED_VIO || r == REF_MARK_REFINE) viol = true;
+                }
+                if (viol) ref[kap] = REF_REFINED;
+            }
+            n_coa = 0;
+            for (int k = 0; k < id_max_curr; k++)
+                if (ref[k] == REF_MARK_COARSEN) n_coa++;
+            proceed_coa = n_coa > 0;
+            if (proceed_coa) {
+                for (int kap = 0; kap < id_max_curr; kap++) {   // Cu_RefineCells_S2_2
+                    if (map[kap] < 0) continue;
+                    for (int p = 1; p < Q_max; p++) {
+                        int ip = N(kap, p);
+                        if (ip > -1 && ref[ip] == REF_MARK_COARSEN) NC(kap, p) = N_SKIPID;
+                    }
+                }
+            }
+        }
+        // S4: Cu_AddRemoveBlocks.
+        if (proceed_ref || proceed_coa) {
+            for (int kap = 0; kap < id_max_curr; kap++) {
+                if (ref[kap] == REF_MARK_REFINE) {
+                    int fc = first_child[kap];
+                    NC(kap, 0) = fc;
+                    int level_q = level[kap] + 1;
+                    T xs[3] = {(T)X[kap], (T)X[kap + n_maxcblocks], DIM == 3 ? (T)X[kap + 2 * n_maxcblocks] : (T)0};
+                    for (int c = 0; c < NCH; c++) {
+                        int id = fc + c;
+                        int ci = c % 2, cj = (c / 2) % 2, ck = (c / 4) % 2;
+                        level[id] = level_q;
+                        ref[id] = REF_NEW;
+                        T h = dx_cblock / (T)(1 << level_q);
+                        X[id] = (float)(xs[0] + (T)ci * h);
+                        X[id + n_maxcblocks] = (float)(xs[1] + (T)cj * h);
+                        if (DIM == 3) X[id + 2 * n_maxcblocks] = (float)(xs[2] + (T)ck * h);
+                        NC(id, 0) = N_SKIPID;
+                    }
+                } else if (ref[kap] == REF_MARK_COARSEN) {
+                    int fc = NC(kap, 0);
+                    NC(kap, 0) = N_SKIPID;
+                    if (fc >= 0)
+                        for (int c = 0; c < NCH; c++) ref[fc + c] = REF_REMOVE;
+                }
+            }
+        }
+        // S5: remove children of coarsened blocks from the ID sets, push them on the gap stack.
+        if (proceed_coa) {
+            std::vector<std::pair<int, int>> rem;  // (level, id), ascending id
+            for (int k = 0; k < id_max_curr_wnew; k++)
+                if (ref[k] == REF_REMOVE) rem.push_back({level[k], k});
+            std::stable_sort(rem.begin(), rem.end(), [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
+            int n_rem_prev = 0;
+            for (int L = 1; L < MAX_LEVELS; L++) {
+                int n_rem_L = 0;
+                for (auto &r : rem)
+                    if (r.first == L) n_rem_L++;
+                if (n_rem_L > 0) {
+                    std::vector<char> kill(n_maxcblocks, 0);
+                    for (int k = n_rem_prev; k < n_rem_prev + n_rem_L; k++) kill[rem[k].second] = 1;
+                    int *s = &id_set[(size_t)L * n_maxcblocks];
+                    int w = 0;
+                    for (int k = 0; k < n_ids[L]; k++)
+                        if (!kill[s[k]]) s[w++] = s[k];
+                    n_ids[L] -= n_rem_L;
+                    n_rem_prev += n_rem_L;
+                }
+            }
+            for (int k = 0; k < n_rem_prev; k++)   // Cu_ConcatReverse
+                gap_set[n_gaps - NCH * n_ref + k] = rem[n_rem_prev - 1 - k].second;
+            n_gaps += n_rem_prev;
+            n_removed = n_rem_prev;
+            for (int kap = 0; kap < id_max_curr; kap++) {   // Cu_CoarsenCells_S2
+                if (ref[kap] != REF_REMOVE) continue;
+                for (int p = 0; p < Q_max; p++) {
+                    if (p > 0) Nn(kap, p) = N_SKIPID;
+                    NC(kap, p) = N_SKIPID;
+                }
+            }
+        }
+        // S6: insert new children in the ID sets.
+        if (proceed_ref) {
+            std::vector<std::pair<int, int>> nw;
+            for (int k = 0; k < id_max_curr_wnew; k++)
+                if (ref[k] == REF_NEW) nw.push_back({level[k], k});
+            std::stable_sort(nw.begin(), nw.end(), [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
+            int prev = 0;
+            for (int L = 1; L < MAX_LEVELS; L++) {
+                int n_new_L = 0;
+                for (auto &r : nw)
+                    if (r.first == L) n_new_L++;
+                if (n_new_L > 0) {
+                    int *s = &id_set[(size_t)L * n_maxcblocks];
+                    for (int k = 0; k < n_new_L; k++) s[n_ids[L] + k] = nw[prev + k].second;
+                    n_ids[L] += n_new_L;
+                    std::sort(s, s + n_ids[L]);
+                }
+                prev += n_new_L;
+            }
+            n_gaps -= NCH * n_ref;
+            n_added = NCH * n_ref;
+        }
+        // S7: ID maxima, block masks, interpolation to new blocks, ref-state bookkeeping.
+        if (proceed_ref || proceed_coa) {
+            for (int L = 1; L < MAX_LEVELS; L++) {
+                int *s = &id_set[(size_t)L * n_maxcblocks];
+                id_max[L] = n_ids[L] > 0 ? *std::max_element(s, s + n_ids[L]) + 1 : 0;
+            }
+            id_max[MAX_LEVELS] = *std::max_element(id_max.begin(), id_max.begin() + MAX_LEVELS);
+            for (int kap = 0; kap < id_max[MAX_LEVELS]; kap++) {   // Cu_UpdateMasks_1
+                if (ref[kap] == REF_INACTIVE) continue;
+                int mask_ID = BLK_REGULAR;
+                if (NC(kap, 0) > -1)
+                    for (int p = 1; p < Q_max; p++)
+                        if (NC(kap, p) == N_SKIPID) mask_ID = BLK_INTERFACE;
+                if (blk_mask[kap] > -1) blk_mask[kap] = mask_ID;
+            }
+            for (int L = 0; L < MAX_LEVELS - 1; L++) interpolate(L, INTERP_ADDED);
+            for (int kap = 0; kap < id_max_curr_wnew; kap++) {   // Cu_RefineCells_S3
+                int r = ref[kap];
+                if (r == REF_MARK_REFINE) ref[kap] = REF_REFINED;
+                if (r == REF_NEW || r == REF_MARK_COARSEN) ref[kap] = REF_UNREFINED;
+                if (r == REF_REMOVE) ref[kap] = REF_INACTIVE;
+                if (r == REF_UNREFINED_VIO) ref[kap] = REF_UNREFINED;
+            }
+            std::vector<int> newref(id_max_curr_wnew, -1);
+            for (int kap = 0; kap < id_max_curr_wnew; kap++) {   // Cu_RefineCells_S4
+                int r = ref[kap];
+                if (r == REF_REFINED || r == REF_REFINED_WCHILD) {
+                    int i_c = NC(kap, 0);
+                    if (i_c < 0) continue;
+                    int nr = REF_REFINED;
+                    for (int xc = 0; xc < NCH; xc++) {
+                        int rc = ref[i_c + xc];
+                        if (rc == REF_REFINED || rc == REF_REFINED_WCHILD) nr = REF_REFINED_WCHILD;
+                    }
+                    newref[kap] = nr;
+                }
+            }
+            for (int kap = 0; kap < id_max_curr_wnew; kap++)
+                if (newref[kap] > -1) ref[kap] = newref[kap];
+        }
+        // S8: connectivity of children, boundary flags, cell masks.
+        if (proceed_ref || proceed_coa) {
+            const int idm = id_max[MAX_LEVELS];
+            for (int kap = 0; kap < idm; kap++) {   // Cu_UpdateConnectivity
+                if (map[kap] < 0 || NC(kap, 0) < 0) continue;
+                int fc = NC(kap, 0);
+                for (int c = 0; c < NCH; c++) {
+                    int cc[3] = {c % 2, (c / 2) % 2, (c / 4) % 2};
+                    for (int p = 0; p < Q_max; p++) {
+                        int d[3], o[3];
+                        for (int a = 0; a < 3; a++) {
+                            int tgt = cc[a] + V.c[a][p];
+                            d[a] = tgt < 0 ? -1 : (tgt > 1 ? 1 : 0);
+                            o[a] = (tgt + 2) % 2;
+                        }
+                        int src = NC(kap, dir_index(d[0], d[1], d[2]));
+                        int off = o[0] + 2 * o[1] + 4 * o[2];
+                        Nn(fc + c, p) = src < 0 ? src : src + off;
+                    }
+                }
+            }
+            for (int kap = 0; kap < idm; kap++) {   // Cu_UpdateBoundaries
+                int on_b = 0;
+                for (int p = 1; p < Q_max; p++) {
+                    int nb = N(kap, p);
+                    if (nb < 0) {
+                        NC(kap, p) = nb;
+                        if (nb != N_SKIPID) on_b = 1;
+                    }
+                }
+                if (blk_mask[kap] == BLK_SOLIDA) on_b = 1;
+                onb[kap] = on_b;
+            }
+            update_cell_masks(idm);
+        }
+        if (out_counts) {
+            out_counts[0] = n_ref;
+            out_counts[1] = proceed_coa ? n_coa : 0;
+            out_counts[2] = n_added;
+            out_counts[3] = n_removed;
+        }
+        return status;
+    }
+
+    // Cu_UpdateMasks_2 (src/mesh/mesh_amr.cu:981-1297) with one sub-block per block (Nqx = 1).
+    void update_cell_masks(int idm)
+    {
+        for (int kap = 0; kap < idm; kap++) {
+            if (ref[kap] == REF_INACTIVE) continue;
+            for (int t = 0; t < M_CBLOCK; t++) {
+                long cell = (long)kap * M_CBLOCK + t;
+                int m = cells_mask[cell];
+                if (m < 0) continue;
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                int ijk[3] = {I, J, K};
+                m = 0;
+                for (int p = 1; p < Q_max; p++)
+                    if (N(kap, p) == N_SKIPID) m = 1;
+                for (int p = 1; p < Q_max; p++) {
+                    if (N(kap, p) != N_SKIPID) continue;
+                    bool outer = true;
+                    for (int a = 0; a < DIM; a++) {
+                        if (V.c[a][p] == 1 && !(ijk[a] >= 2)) outer = false;
+                        if (V.c[a][p] == -1 && !(ijk[a] < 2)) outer = false;
+                    }
+                    if (outer) m = 2;
+                }
+                cells_mask[cell] = m;
+            }
+        }
+    }
+
+    // Mesh::M_Advance_RefineWithSolution (src/mesh/mesh_advance.cu:184-243).
+    int refine_with_solution(int out_counts[4])
+    {
+        if (!(MAX_LEVELS_INTERIOR > 1)) {
+            if (out_counts) out_counts[0] = out_counts[1] = out_counts[2] = out_counts[3] = 0;
+            return AGAL_OK;
+        }
+        for (int L = MAX_LEVELS - 2; L >= N_LEVEL_START; L--) average(L, AVERAGE_GRID);
+        for (int L = N_LEVEL_START; L < MAX_LEVELS - 1; L++) interpolate(L, INTERP_INTERFACE);
+        ref_criteria();
+        return refine_coarsen(out_counts);
+    }
+
+    // Mesh::M_Advance_RefineNearWall (src/mesh/mesh_advance.cu:53-121) without geometry: the
+    // near-wall hook Cu_RefineRegion returns false (inc/custom.h:41).
+    int refine_near_wall()
+    {
+        for (int L = 0; L < N_LEVEL_START; L++) {
+            mark_uniform(L);
+            refine_coarsen(nullptr);
+        }
+        for (int L = N_LEVEL_START; L < MAX_LEVELS_WALL; L++) {
+            if (MAX_LEVELS > 1 && L < MAX_LEVELS_WALL - 1) refine_coarsen(nullptr);
+            set_ic(L);
+        }
+        if (MAX_LEVELS > 1) freeze_refined(0);
+        return AGAL_OK;
+    }
+
+    // The body of Mesh::M_AdvanceLoop (src/mesh/mesh_advance.cu:421-463) for coarse steps [i0,i0+n).
+    int advance_loop(int i0, int n)
+    {
+        int iter_mult = 1 << N_LEVEL_START;
+        int st = AGAL_OK;
+        for (int i = i0; i < i0 + n; i++) {
+            for (int j = 0; j < iter_mult; j++) advance(N_LEVEL_START);
+            if (MAX_LEVELS > 1 && (MAX_LEVELS != N_LEVEL_START + 1) && (i + 1) % P.P_REFINE == 0 && i > 0) {
+                int s = refine_with_solution(nullptr);
+                if (s != AGAL_OK) st = s;
+            }
+        }
+        return st;
+    }
+};
+
+}  // namespace agal_oracle

@@ -1,0 +1,291 @@
+/*
+ * oracle_capi.cpp -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+ *
+ * C wrappers around the CPU restatement in agal_oracle.hpp so that tests/ (via ctypes),
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg can drive it with the same call
+ * sequence they use for the HIP library.  Signatures mirror include/agal_capi.h with an
+ * `oracle_` prefix; populations are exchanged in physical order (entry q = direction q).
+ */
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <string>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "agal_oracle.hpp"
+
+using namespace agal_oracle;
+
+namespace {
+struct Handle {
+    int precision = AGAL_FP32;
+    Oracle<float> *f = nullptr;
+    Oracle<double> *d = nullptr;
+};
+
+template <typename Fn>
+int dispatch(void *h, Fn fn)
+{
+    if (!h) return AGAL_ERR_ARG;
+    Handle *H = static_cast<Handle *>(h);
+    if (H->f) return fn(*H->f);
+    if (H->d) return fn(*H->d);
+    return AGAL_ERR_STATE;
+}
+}  // namespace
+
+extern "C" {
+
+int oracle_create(const agal_params *p, long long capacity, int device, void **out)
+{
+    (void)device;
+    if (!p || !out || capacity <= 0) return AGAL_ERR_ARG;
+    if (p->velocity_set != 9 && p->velocity_set != 19 && p->velocity_set != 27) return AGAL_ERR_ARG;
+    Handle *H = new (std::nothrow) Handle;
+    if (!H) return AGAL_ERR_ALLOC;
+    H->precision = p->precision;
+    int st;
+    try {
+        if (p->precision == AGAL_FP64) {
+            H->d = new Oracle<double>;
+            st = H->d->init(*p, capacity);
+        } else {
+            H->f = new Oracle<float>;
+            st = H->f->init(*p, capacity);
+        }
+    } catch (const std::bad_alloc &) {
+        st = AGAL_ERR_ALLOC;
+    }
+    if (st != AGAL_OK) {
+        delete H->f;
+        delete H->d;
+        delete H;
+        return st;
+    }
+    *out = H;
+    return AGAL_OK;
+}
+
+int oracle_destroy(void *h)
+{
+    if (!h) return AGAL_ERR_ARG;
+    Handle *H = static_cast<Handle *>(h);
+    delete H->f;
+    delete H->d;
+    delete H;
+    return AGAL_OK;
+}
+
+int oracle_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+int oracle_sync(void *h) { return h ? AGAL_OK : AGAL_ERR_ARG; }
+
+int oracle_get_sizes(void *h, agal_sizes *s)
+{
+    return dispatch(h, [&](auto &O) {
+        std::memset(s, 0, sizeof(*s));
+        s->n_dim = O.DIM;
+        s->n_q = O.Q;
+        s->n_q_max = O.Q_max;
+        s->cells_per_block = O.M_CBLOCK;
+        s->n_children = O.N_CHILDREN;
+        s->max_levels = O.MAX_LEVELS;
+        s->n_maxcblocks = O.n_maxcblocks;
+        s->n_maxcells = O.n_maxcells;
+        s->n_coarsecblocks = O.n_coarsecblocks;
+        for (int L = 0; L < O.MAX_LEVELS; L++) s->n_ids[L] = O.n_ids[L];
+        for (int L = 0; L <= O.MAX_LEVELS; L++) s->id_max[L] = O.id_max[L];
+        s->n_gaps = O.n_gaps;
+        return AGAL_OK;
+    });
+}
+
+int oracle_set_ic(void *h, int L) { return dispatch(h, [&](auto &O) { O.set_ic(L); return AGAL_OK; }); }
+int oracle_advance(void *h, int L) { return dispatch(h, [&](auto &O) { O.advance(L); return AGAL_OK; }); }
+int oracle_collide(void *h, int L) { return dispatch(h, [&](auto &O) { O.collide(L); return AGAL_OK; }); }
+int oracle_impose_bc(void *h, int L) { return dispatch(h, [&](auto &O) { O.impose_bc(L); return AGAL_OK; }); }
+int oracle_stream(void *h, int L) { return dispatch(h, [&](auto &O) { O.stream(L); return AGAL_OK; }); }
+int oracle_collide_stream(void *h, int L) { return dispatch(h, [&](auto &O) { O.collide_stream(L); return AGAL_OK; }); }
+int oracle_interpolate(void *h, int L, int var) { return dispatch(h, [&](auto &O) { O.interpolate(L, var); return AGAL_OK; }); }
+int oracle_average(void *h, int L, int var) { return dispatch(h, [&](auto &O) { O.average(L, var); return AGAL_OK; }); }
+int oracle_refresh(void *h, int L) { return dispatch(h, [&](auto &O) { O.refresh(L); return AGAL_OK; }); }
+int oracle_ref_criteria(void *h) { return dispatch(h, [&](auto &O) { O.ref_criteria(); return AGAL_OK; }); }
+int oracle_refine_coarsen(void *h, int *counts) { return dispatch(h, [&](auto &O) { return O.refine_coarsen(counts); }); }
+int oracle_refine_with_solution(void *h, int *counts) { return dispatch(h, [&](auto &O) { return O.refine_with_solution(counts); }); }
+int oracle_refine_near_wall(void *h) { return dispatch(h, [&](auto &O) { return O.refine_near_wall(); }); }
+int oracle_mark_uniform(void *h, int L) { return dispatch(h, [&](auto &O) { O.mark_uniform(L); return AGAL_OK; }); }
+int oracle_freeze_refined(void *h, int var) { return dispatch(h, [&](auto &O) { O.freeze_refined(var); return AGAL_OK; }); }
+
+int oracle_mark_blocks(void *h, const int *ids, int n, int mark)
+{
+    return dispatch(h, [&](auto &O) {
+        for (int k = 0; k < n; k++) {
+            if (ids[k] < 0 || ids[k] >= O.n_maxcblocks) return AGAL_ERR_ARG;
+            O.ref[ids[k]] = mark;
+        }
+        return AGAL_OK;
+    });
+}
+
+/* Wall time (ms) of the coarse steps and of the rebuilds, as agal_advance_loop reports. */
+int oracle_advance_loop(void *h, int i0, int n, double *step_ms, double *rebuild_ms)
+{
+    return dispatch(h, [&](auto &O) {
+        using clk = std::chrono::steady_clock;
+        double ts = 0, tr = 0;
+        int st = AGAL_OK;
+        int iter_mult = 1 << O.N_LEVEL_START;
+        for (int i = i0; i < i0 + n; i++) {
+            auto a = clk::now();
+            for (int j = 0; j < iter_mult; j++) O.advance(O.N_LEVEL_START);
+            auto b = clk::now();
+            ts += std::chrono::duration<double, std::milli>(b - a).count();
+            if (O.MAX_LEVELS > 1 && (O.MAX_LEVELS != O.N_LEVEL_START + 1) && (i + 1) % O.P.P_REFINE == 0 && i > 0) {
+                int s = O.refine_with_solution(nullptr);
+                if (s != AGAL_OK) st = s;
+                tr += std::chrono::duration<double, std::milli>(clk::now() - b).count();
+            }
+        }
+        if (step_ms) *step_ms = ts;
+        if (rebuild_ms) *rebuild_ms = tr;
+        return st;
+    });
+}
+
+/* Populations in physical order: dst[q*dst_stride + cell] = pre-collision f_q. */
+int oracle_download_populations(void *h, void *dst, long long n_blocks, long long dst_stride)
+{
+    return dispatch(h, [&](auto &O) {
+        using T = typename std::decay<decltype(O.F[0])>::type;
+        if (n_blocks < 0 || n_blocks > O.n_maxcblocks) return AGAL_ERR_ARG;
+        T *out = static_cast<T *>(dst);
+        long long nc = n_blocks * O.M_CBLOCK;
+        for (int q = 0; q < O.Q; q++)
+            std::memcpy(out + q * dst_stride, O.Fp(O.V.pb[q]), sizeof(T) * nc);
+        return AGAL_OK;
+    });
+}
+
+int oracle_upload_populations(void *h, const void *src, long long n_blocks, long long src_stride)
+{
+    return dispatch(h, [&](auto &O) {
+        using T = typename std::decay<decltype(O.F[0])>::type;
+        if (n_blocks < 0 || n_blocks > O.n_maxcblocks) return AGAL_ERR_ARG;
+        const T *in = static_cast<const T *>(src);
+        long long nc = n_blocks * O.M_CBLOCK;
+        for (int q = 0; q < O.Q; q++)
+            std::memcpy(O.Fp(O.V.pb[q]), in + q * src_stride, sizeof(T) * nc);
+        return AGAL_OK;
+    });
+}
+
+/* rho,u,v,w as stored by the last refresh/collide (the reference's fields N_Q..N_Q+3). */
+int oracle_download_macros(void *h, void *dst, long long n_blocks, long long dst_stride)
+{
+    return dispatch(h, [&](auto &O) {
+        using T = typename std::decay<decltype(O.F[0])>::type;
+        if (n_blocks < 0 || n_blocks > O.n_maxcblocks) return AGAL_ERR_ARG;
+        T *out = static_cast<T *>(dst);
+        long long nc = n_blocks * O.M_CBLOCK;
+        for (int k = 0; k < 4; k++) {
+            if (k == 3 && O.DIM == 2) {
+                for (long long i = 0; i < nc; i++) out[k * dst_stride + i] = 0;
+                continue;
+            }
+            std::memcpy(out + k * dst_stride, O.Fp(O.Q + k), sizeof(T) * nc);
+        }
+        return AGAL_OK;
+    });
+}
+
+int oracle_download_cell_masks(void *h, int *dst, long long n_blocks)
+{
+    return dispatch(h, [&](auto &O) {
+        if (n_blocks < 0 || n_blocks > O.n_maxcblocks) return AGAL_ERR_ARG;
+        std::memcpy(dst, O.cells_mask.data(), sizeof(int) * n_blocks * O.M_CBLOCK);
+        return AGAL_OK;
+    });
+}
+
+int oracle_upload_cell_masks(void *h, const int *src, long long n_blocks)
+{
+    return dispatch(h, [&](auto &O) {
+        if (n_blocks < 0 || n_blocks > O.n_maxcblocks) return AGAL_ERR_ARG;
+        std::memcpy(O.cells_mask.data(), src, sizeof(int) * n_blocks * O.M_CBLOCK);
+        return AGAL_OK;
+    });
+}
+
+int oracle_download_block_ints(void *h, int which, int *dst, long long n_blocks, long long dst_stride)
+{
+    return dispatch(h, [&](auto &O) {
+        if (n_blocks < 0 || n_blocks > O.n_maxcblocks) return AGAL_ERR_ARG;
+        const std::vector<int> *src = nullptr;
+        int rows = 1;
+        switch (which) {
+            case AGAL_BLK_NBR: src = &O.nbr; rows = O.Q_max; break;
+            case AGAL_BLK_NBR_CHILD: src = &O.nbr_child; rows = O.Q_max; break;
+            case AGAL_BLK_REF: src = &O.ref; break;
+            case AGAL_BLK_LEVEL: src = &O.level; break;
+            case AGAL_BLK_ONB: src = &O.onb; break;
+            case AGAL_BLK_MASK: src = &O.blk_mask; break;
+            default: return AGAL_ERR_ARG;
+        }
+        for (int r = 0; r < rows; r++)
+            std::memcpy(dst + r * dst_stride, src->data() + (size_t)r * O.n_maxcblocks, sizeof(int) * n_blocks);
+        return AGAL_OK;
+    });
+}
+
+int oracle_download_coords(void *h, float *dst, long long n_blocks, long long dst_stride)
+{
+    return dispatch(h, [&](auto &O) {
+        if (n_blocks < 0 || n_blocks > O.n_maxcblocks) return AGAL_ERR_ARG;
+        for (int d = 0; d < O.DIM; d++)
+            std::memcpy(dst + d * dst_stride, O.X.data() + (size_t)d * O.n_maxcblocks, sizeof(float) * n_blocks);
+        return AGAL_OK;
+    });
+}
+
+int oracle_download_id_set(void *h, int L, int *dst, int n)
+{
+    return dispatch(h, [&](auto &O) {
+        if (L < 0 || L >= O.MAX_LEVELS || n < 0 || n > O.n_ids[L]) return AGAL_ERR_ARG;
+        std::memcpy(dst, O.idset(L), sizeof(int) * n);
+        return AGAL_OK;
+    });
+}
+
+int oracle_download_gap_set(void *h, int *dst, int n)
+{
+    return dispatch(h, [&](auto &O) {
+        if (n < 0 || n > O.n_gaps) return AGAL_ERR_ARG;
+        std::memcpy(dst, O.gap_set.data(), sizeof(int) * n);
+        return AGAL_OK;
+    });
+}
+
+/* Restates the tau / dx ladders (solver_lbm_init.cu:47-52) for tests. */
+int oracle_get_level_params(void *h, double *dxf, double *tau, int n)
+{
+    return dispatch(h, [&](auto &O) {
+        for (int L = 0; L < n && L < O.MAX_LEVELS; L++) {
+            dxf[L] = (double)O.dxf_vec[L];
+            tau[L] = (double)O.tau_vec[L];
+        }
+        return AGAL_OK;
+    });
+}
+}

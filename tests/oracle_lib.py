@@ -1,0 +1,79 @@
+"""Binding of the CPU restatement (oracle/liboracle.so) -- the parity checker.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from agal_amd import capi
+from agal_amd.mesh import LbmMesh
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_NAMES = [k for k in capi.PROTOTYPES if not k.startswith("timing")]
+_lib = None
+
+
+def oracle_library():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise FileNotFoundError(f"{ORACLE_PATH} missing: run `make -C oracle`")
+        _lib = C.CDLL(ORACLE_PATH)
+        for extra, (res, args) in {
+            "set_threads": (C.c_int, [C.c_int]),
+            "collide": (C.c_int, [C.c_void_p, C.c_int]),
+            "impose_bc": (C.c_int, [C.c_void_p, C.c_int]),
+            "stream": (C.c_int, [C.c_void_p, C.c_int]),
+            "refine_near_wall": (C.c_int, [C.c_void_p]),
+            "upload_cell_masks": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_longlong]),
+            "get_level_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
+        }.items():
+            f = getattr(_lib, "oracle_" + extra)
+            f.restype, f.argtypes = res, args
+    return _lib
+
+
+def set_threads(n: int) -> int:
+    return oracle_library().oracle_set_threads(int(n))
+
+
+class OracleMesh(LbmMesh):
+    def __init__(self, params, capacity):
+        lib = oracle_library()
+        fns = capi.bind(lib, "oracle_", _NAMES)
+        super().__init__(params, capacity, fns=fns, lib=lib, prefix="oracle_")
+
+    def collide(self, L):
+        self._call_raw("collide", L)
+
+    def impose_bc(self, L):
+        self._call_raw("impose_bc", L)
+
+    def stream(self, L):
+        self._call_raw("stream", L)
+
+    def refine_near_wall(self):
+        self._call_raw("refine_near_wall")
+
+    def level_params(self):
+        n = self.sizes.max_levels
+        dxf = (C.c_double * n)()
+        tau = (C.c_double * n)()
+        oracle_library().oracle_get_level_params(self.h, dxf, tau, n)
+        return list(dxf), list(tau)
+
+    def set_cell_masks(self, m):
+        import numpy as np
+        m = np.ascontiguousarray(m, dtype=np.int32)
+        nb = m.size // self.sizes.cells_per_block
+        oracle_library().oracle_upload_cell_masks(self.h, m.ctypes.data_as(C.POINTER(C.c_int)), nb)
+
+    def _call_raw(self, name, *args):
+        st = getattr(oracle_library(), "oracle_" + name)(self.h, *args)
+        if st != capi.AGAL_OK:
+            raise capi.AgalError(st, "oracle_" + name)
+        return st
