@@ -134,6 +134,43 @@ inline int nbr_cell_id(int dim, int I, int J, int K)
     return (dim == 2) ? I + 4 * J : I + 4 * J + 16 * K;
 }
 
+/* Neighbour of child c (= ci + 2cj + 4ck) in direction p, read from the parent's neighbour-child
+ * table: the parent-level direction whose block contains the target position, and the child
+ * offset inside that block (the table of Cu_UpdateConnectivity, src/mesh/mesh_amr.cu:154-944). */
+inline void child_neighbor_rule(const VelSet &V, int c, int p, int &src_dir, int &off)
+{
+    int cc[3] = {c % 2, (c / 2) % 2, (c / 4) % 2};
+    int d[3], o[3];
+    for (int a = 0; a < 3; a++) {
+        int tgt = cc[a] + V.c[a][p];
+        d[a] = tgt < 0 ? -1 : (tgt > 1 ? 1 : 0);
+        o[a] = (tgt + 2) % 2;
+    }
+    int S = (d[0] + 1) + 3 * (d[1] + 1) + (V.DIM == 3 ? 9 * (d[2] + 1) : 0);
+    src_dir = V.conn_map[S];
+    off = o[0] + 2 * o[1] + 4 * o[2];
+}
+
+/* Cell mask of local cell t of a block whose direction-p neighbour is missing when skip[p] != 0
+ * (Cu_UpdateMasks_2, src/mesh/mesh_amr.cu:981-1297, one sub-block per block). */
+inline int cell_mask_rule(const VelSet &V, const int *skip, int t)
+{
+    int ijk[3] = {t % 4, (t / 4) % 4, (V.DIM == 3) ? t / 16 : 0};
+    int m = 0;
+    for (int p = 1; p < V.Q_max; p++)
+        if (skip[p]) m = 1;
+    for (int p = 1; p < V.Q_max; p++) {
+        if (!skip[p]) continue;
+        bool outer = true;
+        for (int a = 0; a < V.DIM; a++) {
+            if (V.c[a][p] == 1 && !(ijk[a] >= 2)) outer = false;
+            if (V.c[a][p] == -1 && !(ijk[a] < 2)) outer = false;
+        }
+        if (outer) m = 2;
+    }
+    return m;
+}
+
 /* Exact floor(log2(x)) for float/double x (see header note). */
 template <typename T>
 inline T floor_log2_exact(T x)
@@ -1086,16 +1123,10 @@ struct Oracle {
                 if (map[kap] < 0 || NC(kap, 0) < 0) continue;
                 int fc = NC(kap, 0);
                 for (int c = 0; c < NCH; c++) {
-                    int cc[3] = {c % 2, (c / 2) % 2, (c / 4) % 2};
                     for (int p = 0; p < Q_max; p++) {
-                        int d[3], o[3];
-                        for (int a = 0; a < 3; a++) {
-                            int tgt = cc[a] + V.c[a][p];
-                            d[a] = tgt < 0 ? -1 : (tgt > 1 ? 1 : 0);
-                            o[a] = (tgt + 2) % 2;
-                        }
-                        int src = NC(kap, dir_index(d[0], d[1], d[2]));
-                        int off = o[0] + 2 * o[1] + 4 * o[2];
+                        int src_dir, off;
+                        child_neighbor_rule(V, c, p, src_dir, off);
+                        int src = NC(kap, src_dir);
                         Nn(fc + c, p) = src < 0 ? src : src + off;
                     }
                 }
@@ -1128,26 +1159,12 @@ struct Oracle {
     {
         for (int kap = 0; kap < idm; kap++) {
             if (ref[kap] == REF_INACTIVE) continue;
+            int skip[27];
+            for (int p = 0; p < Q_max; p++) skip[p] = (N(kap, p) == N_SKIPID);
             for (int t = 0; t < M_CBLOCK; t++) {
                 long cell = (long)kap * M_CBLOCK + t;
-                int m = cells_mask[cell];
-                if (m < 0) continue;
-                int I, J, K;
-                cell_ijk(t, I, J, K);
-                int ijk[3] = {I, J, K};
-                m = 0;
-                for (int p = 1; p < Q_max; p++)
-                    if (N(kap, p) == N_SKIPID) m = 1;
-                for (int p = 1; p < Q_max; p++) {
-                    if (N(kap, p) != N_SKIPID) continue;
-                    bool outer = true;
-                    for (int a = 0; a < DIM; a++) {
-                        if (V.c[a][p] == 1 && !(ijk[a] >= 2)) outer = false;
-                        if (V.c[a][p] == -1 && !(ijk[a] < 2)) outer = false;
-                    }
-                    if (outer) m = 2;
-                }
-                cells_mask[cell] = m;
+                if (cells_mask[cell] < 0) continue;
+                cells_mask[cell] = cell_mask_rule(V, skip, t);
             }
         }
     }

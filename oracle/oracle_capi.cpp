@@ -277,6 +277,51 @@ int oracle_download_gap_set(void *h, int *dst, int n)
     });
 }
 
+/* The restated velocity set: c (3 rows of 27), w, pb, conn_map (all of length Q_max). */
+int oracle_velset(int Q, int *c, double *w, int *pb, int *conn_map)
+{
+    if (Q != 9 && Q != 19 && Q != 27) return AGAL_ERR_ARG;
+    VelSet V = make_velset(Q);
+    for (int d = 0; d < 3; d++)
+        for (int p = 0; p < 27; p++) c[d * 27 + p] = V.c[d][p];
+    for (int p = 0; p < 27; p++) {
+        w[p] = V.w[p];
+        pb[p] = V.pb[p];
+        conn_map[p] = V.conn_map[p];
+    }
+    return V.Q_max;
+}
+
+int oracle_child_neighbor_rule(int Q, int c, int p, int *src_dir, int *off)
+{
+    if (Q != 9 && Q != 19 && Q != 27) return AGAL_ERR_ARG;
+    VelSet V = make_velset(Q);
+    child_neighbor_rule(V, c, p, *src_dir, *off);
+    return AGAL_OK;
+}
+
+int oracle_cell_mask_rule(int Q, const int *skip, int t)
+{
+    VelSet V = make_velset(Q);
+    return cell_mask_rule(V, skip, t);
+}
+
+int oracle_in_half_set(int dim, int p) { return in_half_set(dim, p) ? 1 : 0; }
+
+/* Overwrite the stored rho,u,v,w fields (the reference's fields N_Q..N_Q+3). */
+int oracle_upload_macros(void *h, const void *src, long long n_blocks, long long src_stride)
+{
+    return dispatch(h, [&](auto &O) {
+        using T = typename std::decay<decltype(O.F[0])>::type;
+        if (n_blocks < 0 || n_blocks > O.n_maxcblocks) return AGAL_ERR_ARG;
+        const T *in = static_cast<const T *>(src);
+        long long nc = n_blocks * O.M_CBLOCK;
+        for (int k = 0; k < (O.DIM == 3 ? 4 : 3); k++)
+            std::memcpy(O.Fp(O.Q + k), in + k * src_stride, sizeof(T) * nc);
+        return AGAL_OK;
+    });
+}
+
 /* Restates the tau / dx ladders (solver_lbm_init.cu:47-52) for tests. */
 int oracle_get_level_params(void *h, double *dxf, double *tau, int n)
 {

@@ -31,6 +31,11 @@ def oracle_library():
             "refine_near_wall": (C.c_int, [C.c_void_p]),
             "upload_cell_masks": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_longlong]),
             "get_level_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
+            "velset": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "child_neighbor_rule": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "cell_mask_rule": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.c_int]),
+            "in_half_set": (C.c_int, [C.c_int, C.c_int]),
+            "upload_macros": (C.c_int, [C.c_void_p, C.c_void_p, C.c_longlong, C.c_longlong]),
         }.items():
             f = getattr(_lib, "oracle_" + extra)
             f.restype, f.argtypes = res, args
@@ -72,8 +77,41 @@ class OracleMesh(LbmMesh):
         nb = m.size // self.sizes.cells_per_block
         oracle_library().oracle_upload_cell_masks(self.h, m.ctypes.data_as(C.POINTER(C.c_int)), nb)
 
+    def set_macros(self, m):
+        import numpy as np
+        m = np.ascontiguousarray(m, dtype=self.dtype)
+        nc = m.shape[1]
+        self._call_raw("upload_macros", m.ctypes.data_as(C.c_void_p), nc // self.sizes.cells_per_block, nc)
+
     def _call_raw(self, name, *args):
         st = getattr(oracle_library(), "oracle_" + name)(self.h, *args)
         if st != capi.AGAL_OK:
             raise capi.AgalError(st, "oracle_" + name)
         return st
+
+
+def velset(Q: int):
+    """The restated velocity set as numpy arrays: c (3 x Qmax), w, pb, conn_map."""
+    import numpy as np
+    c = (C.c_int * 81)()
+    w = (C.c_double * 27)()
+    pb = (C.c_int * 27)()
+    cm = (C.c_int * 27)()
+    qmax = oracle_library().oracle_velset(Q, c, w, pb, cm)
+    cc = np.array(list(c), dtype=np.int64).reshape(3, 27)[:, :qmax]
+    return cc, np.array(list(w))[:qmax], np.array(list(pb))[:qmax], np.array(list(cm))[:qmax]
+
+
+def child_neighbor_rule(Q: int, c: int, p: int):
+    a, b = C.c_int(), C.c_int()
+    oracle_library().oracle_child_neighbor_rule(Q, c, p, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def cell_mask_rule(Q: int, skip, t: int) -> int:
+    arr = (C.c_int * 27)(*([int(x) for x in skip] + [0] * (27 - len(skip))))
+    return oracle_library().oracle_cell_mask_rule(Q, arr, t)
+
+
+def in_half_set(dim: int, p: int) -> bool:
+    return bool(oracle_library().oracle_in_half_set(dim, p))
