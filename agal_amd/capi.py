@@ -37,6 +37,7 @@ REF_UNREFINED_VIO, REF_REFINED_WCHILD_PERM = 9, 10
 N_SKIPID = -999
 CELL_INTERIOR, CELL_INTERFACE, CELL_GHOST, CELL_SOLID = 0, 1, 2, -1
 BLK_NBR, BLK_NBR_CHILD, BLK_REF, BLK_LEVEL, BLK_ONB, BLK_MASK = 0, 1, 2, 3, 4, 5
+OPT_GRAPHS = 1
 
 
 class Params(C.Structure):
@@ -147,10 +148,13 @@ PROTOTYPES = {
     "upload_populations": (_I, [_P, _P, _LL, _LL]),
     "download_macros": (_I, [_P, _P, _LL, _LL]),
     "download_cell_masks": (_I, [_P, C.POINTER(_I), _LL]),
+    "upload_cell_masks": (_I, [_P, C.POINTER(_I), _LL]),
+    "upload_block_ints": (_I, [_P, _I, C.POINTER(_I), _LL, _LL]),
     "download_block_ints": (_I, [_P, _I, C.POINTER(_I), _LL, _LL]),
     "download_coords": (_I, [_P, C.POINTER(C.c_float), _LL, _LL]),
     "download_id_set": (_I, [_P, _I, C.POINTER(_I), _I]),
     "download_gap_set": (_I, [_P, C.POINTER(_I), _I]),
+    "set_option": (_I, [_P, _I, _LL]),
     "timing_reset": (_I, [_P, _I]),
     "timing_get": (_I, [_P, C.POINTER(C.c_double), C.POINTER(_LL), C.POINTER(C.c_double)]),
 }

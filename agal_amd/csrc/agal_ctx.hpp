@@ -77,6 +77,17 @@ struct Ctx {
 
     LevelTiming timing;
 
+    // One coarse step recorded as a HIP graph, per starting buffer parity of the levels.  A graph
+    // is valid for one mesh epoch (bumped whenever the level lists change).
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        unsigned cur_before = 0, cur_after = 0;
+        unsigned epoch = 0;
+    };
+    StepGraph graphs[4];
+    unsigned mesh_epoch = 1;
+    bool use_graphs = true;
+
     int *id_set_L(int L) const { return id_set + (size_t)L * n_maxcblocks; }
     int *iface_L(int L) const { return iface + (size_t)L * n_maxcblocks; }
 };

@@ -651,6 +651,7 @@ int rebuild_level_lists(Ctx *c, long long range_hint)
             AGAL_HIP(c, hipMemcpyAsync(&last[L], c->id_set_L(L) + c->n_ids[L] - 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     if ((st = read_scalars(c, 16))) return st;
     for (int L = 0; L < ML; L++) c->n_iface[L] = c->hscal[8 + L];
+    c->mesh_epoch++;  // recorded step graphs carry the old list sizes
     for (int L = 1; L < ML; L++) c->id_max[L] = c->n_ids[L] > 0 ? last[L] + 1 : 0;
     c->id_max[ML] = *std::max_element(c->id_max.begin(), c->id_max.begin() + ML);
     int n = c->id_max[ML];

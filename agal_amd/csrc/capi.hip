@@ -114,6 +114,8 @@ void free_ctx(Ctx *c)
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->hscal) (void)hipHostFree(c->hscal);
+    for (auto &g : c->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (hipEvent_t e : c->timing.ev_pairs) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -246,6 +248,67 @@ int advance_rec(Ctx *c, int L)
     } else {
         if ((st = launch_collide_stream(c, L))) return st;
     }
+    return AGAL_OK;
+}
+
+unsigned cur_mask(const Ctx *c)
+{
+    unsigned m = 0;
+    for (int L = 0; L < c->MAX_LEVELS; L++) m |= (unsigned)(c->cur[L] & 1) << L;
+    return m;
+}
+
+void set_cur_mask(Ctx *c, unsigned m)
+{
+    for (int L = 0; L < c->MAX_LEVELS; L++) c->cur[L] = (m >> L) & 1;
+}
+
+// One coarse step from level L0.  With graphs enabled the launch sequence of advance_rec is
+// recorded once per (mesh epoch, starting buffer parity) and replayed; the host-side bookkeeping
+// (buffer parity per level) is advanced exactly as a direct run would.
+int step_once(Ctx *c, int L0)
+{
+    if (!c->use_graphs || c->timing.enabled || L0 != c->N_LEVEL_START) return advance_rec(c, L0);
+    const unsigned before = cur_mask(c);
+    for (auto &g : c->graphs)
+        if (g.exec && g.epoch == c->mesh_epoch && g.cur_before == before) {
+            AGAL_HIP(c, hipGraphLaunch(g.exec, c->stream));
+            set_cur_mask(c, g.cur_after);
+            c->global_Liter = 1 << (c->MAX_LEVELS - 1 - L0);
+            return AGAL_OK;
+        }
+    // Record a new graph (evicting a stale or the oldest entry).
+    Ctx::StepGraph *slot = &c->graphs[0];
+    for (auto &g : c->graphs)
+        if (!g.exec || g.epoch != c->mesh_epoch) {
+            slot = &g;
+            break;
+        }
+    if (slot->exec) {
+        (void)hipGraphExecDestroy(slot->exec);
+        slot->exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    AGAL_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int st = advance_rec(c, L0);
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (st != AGAL_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        set_cur_mask(c, before);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+    const unsigned after = cur_mask(c);
+    e = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        slot->exec = nullptr;
+        return hip_fail(c, e, "hipGraphInstantiate");
+    }
+    slot->cur_before = before;
+    slot->cur_after = after;
+    slot->epoch = c->mesh_epoch;
+    AGAL_HIP(c, hipGraphLaunch(slot->exec, c->stream));
     return AGAL_OK;
 }
 
@@ -541,7 +604,7 @@ int agal_advance(agal_ctx *ctx, int L0)
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (L0 < 0 || L0 >= c->MAX_LEVELS) return AGAL_ERR_ARG;
     if (!c->ic_done) return AGAL_ERR_STATE;
-    return advance_rec(c, L0);
+    return step_once(c, L0);
 }
 
 int agal_collide_stream(agal_ctx *ctx, int L)
@@ -654,7 +717,7 @@ int agal_advance_loop(agal_ctx *ctx, int i0, int n, double *out_step_ms, double 
     AGAL_HIP(c, hipEventRecord(e0, c->stream));
     for (int i = i0; i < i0 + n; i++) {
         for (int j = 0; j < iter_mult; j++)
-            if ((st = advance_rec(c, c->N_LEVEL_START))) return st;
+            if ((st = step_once(c, c->N_LEVEL_START))) return st;
         if (ML > 1 && ML != c->N_LEVEL_START + 1 && (i + 1) % c->P.P_REFINE == 0 && i > 0) {
             float ms = 0;
             AGAL_HIP(c, hipEventRecord(e1, c->stream));
@@ -851,6 +914,17 @@ int agal_download_gap_set(agal_ctx *ctx, int *dst, int n)
     AGAL_HIP(c, hipMemcpyAsync(dst, c->gap_set, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
     AGAL_HIP(c, hipStreamSynchronize(c->stream));
     return AGAL_OK;
+}
+
+int agal_set_option(agal_ctx *ctx, int option, long long value)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (option == AGAL_OPT_GRAPHS) {
+        c->use_graphs = value != 0;
+        return AGAL_OK;
+    }
+    return AGAL_ERR_ARG;
 }
 
 int agal_timing_reset(agal_ctx *ctx, int enable)

@@ -116,6 +116,15 @@ struct StepArgs {
     long long exc_cap;
 };
 
+// Workgroup -> position in the level's block list.  Workgroups are dealt round-robin to the
+// eight XCDs (each with its own L2); giving every XCD one contiguous range of the sorted ID list
+// keeps a block and most of its neighbours -- whose pushes share cache lines -- on one L2.
+__device__ __forceinline__ int xcd_local_wg(int nwg_padded)
+{
+    const int per = nwg_padded / 8;
+    return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+}
+
 template <typename T, int Q>
 __global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T> a)
 {
@@ -124,35 +133,30 @@ __global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T> a)
     const int t = threadIdx.x % CELLS;
     const int lane = threadIdx.x % 64;
     const int gbase = lane - t;  // lane of cell 0 of this block within the wavefront
-    const int bi = blockIdx.x * (WG / CELLS) + threadIdx.x / CELLS;
+    const int bi = xcd_local_wg(gridDim.x) * (WG / CELLS) + threadIdx.x / CELLS;
     const bool active = bi < a.n;
     const int b = active ? a.ids[bi] : 0;
-    // Lane-parallel load of the block's neighbour table; distributed below with ds_bpermute.
+    // Lane-parallel load of the block's neighbour table; distributed with ds_bpermute (every
+    // __shfl below sits outside divergent control flow).
     const int my_nb = (active && t < QMAX) ? a.nbr_slot[(long long)b * QMAX + t] : N_SKIPID;
     const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
     const long long cell = (long long)b * CELLS + t;
 
-    // Neighbour block and cell of (I,J,K) + c_q for every direction (all lanes participate).
-    int tb[Q], tcell[Q];
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        const int Ip = I + VS::cx(q), Jp = J + VS::cy(q), Kp = K + VS::cz(q);
-        const int S = (Ip < 0 ? 0 : (Ip >= 4 ? 2 : 1)) + 3 * (Jp < 0 ? 0 : (Jp >= 4 ? 2 : 1)) +
-                      (DIM == 3 ? 9 * (Kp < 0 ? 0 : (Kp >= 4 ? 2 : 1)) : 0);
-        tb[q] = __shfl(my_nb, gbase + S, 64);
-        tcell[q] = (Ip & 3) + 4 * (Jp & 3) + (DIM == 3 ? 16 * (Kp & 3) : 0);
-    }
-    if (!active) return;
-
-    const bool simple = a.simple[b] != 0;
-    const bool part = a.part[b] != 0;
+    bool simple = true, inert = false, onb = false;
     int mask = CELL_INTERIOR;
-    if (!simple) mask = a.cells_mask[cell];
-    const bool inert = !part || mask == CELL_SOLID;
-
     T f[Q];
+    if (active) {
+        simple = a.simple[b] != 0;
+        const bool part = a.part[b] != 0;
+        if (!simple) mask = a.cells_mask[cell];
+        inert = !part || mask == CELL_SOLID;
+        onb = a.onb[b] == 1 && mask != CELL_SOLID;
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = a.A[(long long)q * a.ncm + cell];
+        for (int q = 0; q < Q; q++) f[q] = a.A[(long long)q * a.ncm + cell];
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; q++) f[q] = (T)1;
+    }
 
     T fs[Q];
     T rho, u, v, w;
@@ -169,20 +173,18 @@ __global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T> a)
         for (int p = 0; p < Q; p++) fs[p] = f[VS::pb(p)];
     }
 
-    // Domain-face hook on links that leave the domain (Cu_ImposeBC, onb blocks, non-solid cells).
-    if (a.onb[b] == 1 && mask != CELL_SOLID) {
-#pragma unroll
-        for (int q = 1; q < Q; q++) {
-            if (tb[q] < 0 && tb[q] != N_SKIPID)
-                fs[q] = bc_hook<T>(a.bc_preset, a.U, tb[q], fs[q], u, v, w, (T)VS::w(q), (T)VS::cx(q), (T)VS::cy(q), (T)VS::cz(q));
-        }
-    }
-
-    a.B[cell] = fs[0];
+    if (active) a.B[cell] = fs[0];
 #pragma unroll
     for (int q = 1; q < Q; q++) {
-        const int nbk = tb[q];
-        const long long tgt = (long long)nbk * CELLS + tcell[q];
+        const int Ip = I + VS::cx(q), Jp = J + VS::cy(q), Kp = K + VS::cz(q);
+        const int S = (Ip < 0 ? 0 : (Ip >= 4 ? 2 : 1)) + 3 * (Jp < 0 ? 0 : (Jp >= 4 ? 2 : 1)) +
+                      (DIM == 3 ? 9 * (Kp < 0 ? 0 : (Kp >= 4 ? 2 : 1)) : 0);
+        const int nbk = __shfl(my_nb, gbase + S, 64);
+        if (!active) continue;
+        const long long tgt = (long long)nbk * CELLS + ((Ip & 3) + 4 * (Jp & 3) + (DIM == 3 ? 16 * (Kp & 3) : 0));
+        // Domain-face hook on links that leave the domain (Cu_ImposeBC, onb blocks, non-solid cells).
+        if (onb && nbk < 0 && nbk != N_SKIPID)
+            fs[q] = bc_hook<T>(a.bc_preset, a.U, nbk, fs[q], u, v, w, (T)VS::w(q), (T)VS::cx(q), (T)VS::cy(q), (T)VS::cz(q));
         bool push;
         bool exc = false;
         if (VS::half(q)) {
@@ -243,7 +245,8 @@ __global__ __launch_bounds__(WG) void k_fixup(T *__restrict__ B, long long ncm, 
 
 // ---------------------------------------------------------------------------------------------
 // Coarse -> fine: Cu_Interpolate_Linear (solver_lbm_kernels_comm.cu:13-138).  One wavefront per
-// parent block (3D); the rescaled parent populations of one direction are shared through LDS.
+// parent block (3D).  All Q rescaled parent populations go to LDS behind one barrier; each lane
+// then writes its cell of the eight children (row-contiguous stores).
 // ---------------------------------------------------------------------------------------------
 template <typename T, int Q, bool ADDED>
 __global__ __launch_bounds__(WG) void k_interp(const int *__restrict__ ids, int n, const T *__restrict__ Fp,
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(WG) void k_interp(const int *__restrict__ ids, int 
 {
     using VS = VelSet<Q>;
     constexpr int CELLS = VS::CELLS, DIM = VS::DIM, NCH = VS::CHILDREN;
-    __shared__ T s_F[WG];
+    __shared__ T s_all[Q * WG];
     const int t = threadIdx.x % CELLS;
     const int base = threadIdx.x - t;
     const int bi = blockIdx.x * (WG / CELLS) + threadIdx.x / CELLS;
@@ -282,16 +285,20 @@ __global__ __launch_bounds__(WG) void k_interp(const int *__restrict__ ids, int 
             ghost[c] = ADDED || cells_mask[(long long)(fc + c) * CELLS + t] == CELL_GHOST;
     }
     const T udotu = u * u + v * v + w * w;
+    if (work) {
 #pragma unroll
-    for (int p = 0; p < Q; p++) {
-        if (work) {
+        for (int p = 0; p < Q; p++) {
             T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
             T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
-            s_F[threadIdx.x] = fe + (f[p] - fe) * tau_ratio;
+            s_all[p * WG + threadIdx.x] = fe + (f[p] - fe) * tau_ratio;
         }
-        __syncthreads();
-        if (work) {
-            const T *s = s_F + base;
+    }
+    __syncthreads();
+    if (!work) return;
+#pragma unroll
+    for (int p = 0; p < Q; p++) {
+        {
+            const T *s = s_all + p * WG + base;
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
                 const int ci = c % 2, cj = (c / 2) % 2, ck = (c / 4) % 2;
@@ -312,12 +319,13 @@ __global__ __launch_bounds__(WG) void k_interp(const int *__restrict__ ids, int 
                 Fc[(long long)p * ncm + (long long)(fc + c) * CELLS + t] = val;
             }
         }
-        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------------------------
-// Fine -> coarse: Cu_Average (solver_lbm_kernels_comm.cu:174-347).
+// Fine -> coarse: Cu_Average (solver_lbm_kernels_comm.cu:174-347).  One lane per parent cell
+// gathers the 2x2(x2) child cells of its quadrant's child block, rescales their populations and
+// sums them in the reference's order (no LDS, no barriers).
 // ---------------------------------------------------------------------------------------------
 template <typename T, int Q>
 __global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int n, T *__restrict__ Fp,
@@ -326,63 +334,46 @@ __global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int
                                                 int ave_type, T tau_ratio)
 {
     using VS = VelSet<Q>;
-    constexpr int CELLS = VS::CELLS, DIM = VS::DIM, NCH = VS::CHILDREN;
-    __shared__ T s_Fc[WG];
-    __shared__ int s_mask[WG];
+    constexpr int CELLS = VS::CELLS, DIM = VS::DIM;
+    constexpr int NG = (DIM == 3) ? 8 : 4;  // child cells per parent cell
     const int t = threadIdx.x % CELLS;
-    const int base = threadIdx.x - t;
     const int bi = blockIdx.x * (WG / CELLS) + threadIdx.x / CELLS;
-    int b = -1;
-    if (bi < n) b = ids[bi];
-    bool work = false;
-    int fc = 0;
-    if (b >= 0) {
-        fc = nbr_child[b];
-        work = (fc > -1) && (ave_type == 2 || blk_mask[b] == BLK_INTERFACE);
-    }
+    if (bi >= n) return;
+    const int b = ids[bi];
+    const int fc = nbr_child[b];
+    if (!((fc > -1) && (ave_type == 2 || blk_mask[b] == BLK_INTERFACE))) return;
     const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
+    // Child block of this cell's quadrant and the first cell of its 2x2(x2) group.
+    const int c = (I / 2) + 2 * (J / 2) + 4 * (K / 2);
     int c0 = 2 * (I % 2) + 4 * (2 * (J % 2));
     if (DIM == 3) c0 += 4 * 4 * (2 * (K % 2));
-#pragma unroll 1
-    for (int c = 0; c < NCH; c++) {
-        const int ci = c % 2, cj = (c / 2) % 2, ck = (c / 4) % 2;
-        bool inq = (I >= 2 * ci) && (I < 2 + 2 * ci) && (J >= 2 * cj) && (J < 2 + 2 * cj);
-        if (DIM == 3) inq = inq && (K >= 2 * ck) && (K < 2 + 2 * ck);
-        T f[Q];
-        T rho = 0, u = 0, v = 0, w = 0;
-        if (work) {
-            const long long ccell = (long long)(fc + c) * CELLS + t;
-            int m = cells_mask[ccell];
-            if (ave_type > 0 && m < 2) m = 1;
-            s_mask[threadIdx.x] = m;
+    const long long cb = (long long)(fc + c) * CELLS;
+    int m = cells_mask[cb + c0];
+    if (ave_type > 0 && m < 2) m = 1;
+    if (m != 1) return;
+    T acc[Q];
 #pragma unroll
-            for (int p = 0; p < Q; p++) f[p] = Fc[(long long)p * ncm + ccell];
-            moments<T, Q>(f, rho, u, v, w);
-        }
+    for (int g = 0; g < NG; g++) {
+        const int off = (g & 1) + 4 * ((g >> 1) & 1) + 16 * ((g >> 2) & 1);
+        const long long ccell = cb + c0 + off;
+        T f[Q];
+#pragma unroll
+        for (int p = 0; p < Q; p++) f[p] = Fc[(long long)p * ncm + ccell];
+        T rho, u, v, w;
+        moments<T, Q>(f, rho, u, v, w);
         const T udotu = (DIM == 3) ? u * u + v * v + w * w : u * u + v * v;
-        __syncthreads();
 #pragma unroll
         for (int p = 0; p < Q; p++) {
-            if (work) {
-                T cdotu = (DIM == 3) ? (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w
-                                     : (T)VS::cx(p) * u + (T)VS::cy(p) * v;
-                T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
-                s_Fc[threadIdx.x] = fe + (f[p] - fe) * tau_ratio;
-            }
-            __syncthreads();
-            if (work && inq && s_mask[base + c0] == 1) {
-                const T *s = s_Fc + base;
-                T val;
-                if (DIM == 2)
-                    val = (T)(0.25) * (s[c0 + 0] + s[c0 + 1] + s[c0 + 4] + s[c0 + 5]);
-                else
-                    val = (T)(0.125) * (s[c0 + 0] + s[c0 + 1] + s[c0 + 4] + s[c0 + 5] + s[c0 + 16] + s[c0 + 17] +
-                                        s[c0 + 20] + s[c0 + 21]);
-                Fp[(long long)p * ncm + (long long)b * CELLS + t] = val;
-            }
-            __syncthreads();
+            T cdotu = (DIM == 3) ? (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w
+                                 : (T)VS::cx(p) * u + (T)VS::cy(p) * v;
+            T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
+            T s = fe + (f[p] - fe) * tau_ratio;
+            acc[p] = (g == 0) ? s : acc[p] + s;
         }
     }
+    const T scale = (DIM == 2) ? (T)(0.25) : (T)(0.125);
+#pragma unroll
+    for (int p = 0; p < Q; p++) Fp[(long long)p * ncm + (long long)b * CELLS + t] = scale * acc[p];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -447,7 +438,8 @@ static int collide_stream_t(Ctx *c, int L)
         AGAL_HIP(c, hipEventCreate(&e1));
         AGAL_HIP(c, hipEventRecord(e0, c->stream));
     }
-    hipLaunchKernelGGL((k_collide_stream<T, Q>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, a);
+    const int grid = (nwg(n, VelSet<Q>::CELLS) + 7) / 8 * 8;  // a multiple of the XCD count
+    hipLaunchKernelGGL((k_collide_stream<T, Q>), dim3(grid), dim3(WG), 0, c->stream, a);
     AGAL_HIP(c, hipGetLastError());
     if (c->timing.enabled) {
         AGAL_HIP(c, hipEventRecord(e1, c->stream));

@@ -66,6 +66,9 @@ class LbmMesh:
     def sync(self):
         self._call("sync")
 
+    def set_option(self, option, value):
+        self._call("set_option", int(option), int(value))
+
     # -- solver interface (inc/solver.h:62-71) -------------------------------------------------
     def S_SetIC(self, L):
         self._call("set_ic", int(L))

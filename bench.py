@@ -1,0 +1,215 @@
+"""Benchmark of the octree-AMR lattice-Boltzmann step on one MI355X.
+
+Workload (one "step" = one coarse time step, S_Advance of the root level with recursive
+sub-cycling): D3Q19 BGK lid-driven cavity, root grid 64^3 in 4x4x4 cell blocks, 3 refinement
+levels marked by the vorticity criterion and re-evaluated every 100 coarse steps, Re = 1000, lid
+velocity 0.05, FP32 populations in structure-of-arrays layout.  The warm-up runs the real driver
+(M_AdvanceLoop: steps plus solution-based rebuilds) from rest so the mesh is the one the flow
+produces; the timed window then runs K coarse steps.
+
+value: MLUPS counting every cell update of every active block (2^L sub-steps on level L) divided
+by the step time of the window, rebuild time excluded (reported separately and end to end).
+roofline: the fused collide+boundary+stream kernel, algorithmic bytes 2*Q*4 per cell update
+divided by its average launch time from HIP events on the context stream.
+cpu_baseline: the CPU restatement (OpenMP) stepping a copy of the same mesh for a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS + %HBM roofline, D3Q19 AMR; FP16-store vs FP32 at ≤1e-3 vel err"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=99)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--capacity", type=int, default=400000, help="block pool size")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-steps", type=int, default=20)
+    ap.add_argument("--out", default=None, help="also write the JSON line to this file")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device copy (torch) to calibrate HBM byte counters")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 28  # 2^28 float32 = 1 GiB read + 1 GiB written
+        a = torch.ones(n, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
+              "described in DESIGN.md but not built.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    from agal_amd import capi
+    from agal_amd.mesh import LbmMesh
+
+    Q, b = 19, 4
+    N, U, Re, levels = 64, 0.05, 1000.0, 3
+    p = capi.default_params(velocity_set=capi.D3Q19, precision=capi.FP32, Nx=N, MAX_LEVELS_WALL=1,
+                            MAX_LEVELS_INTERIOR=levels, v0=(U * N / Re) / N, bc_preset=capi.BC_LDC3D,
+                            bc_velocity=U, P_REFINE=100, N_REFINE_START=-2.0, N_REFINE_INC=1.0,
+                            N_REFINE_MAX=1.0, ic_rho=1.0)
+    m = LbmMesh(p, args.capacity)
+    m.S_SetIC(0)
+    m.M_FreezeRefinedCells(0)
+    t0 = time.time()
+    st_w, _, reb_w = m.advance_loop(0, args.warmup)
+    m.sync()
+    warm_s = time.time() - t0
+    s = m.refresh_sizes()
+    n_ids = [int(s.n_ids[L]) for L in range(levels)]
+
+    # Roofline pass on the same mesh: HIP events around every fused launch (direct launches; the
+    # stepping state advances but no rebuild is triggered).
+    import ctypes as C
+    m._f["timing_reset"](m.h, 1)
+    for _ in range(args.roofline_steps):
+        m.S_Advance(0)
+    m.sync()
+    fms, nl, fcells = C.c_double(), C.c_longlong(), C.c_double()
+    m._f["timing_get"](m.h, C.byref(fms), C.byref(nl), C.byref(fcells))
+    m._f["timing_reset"](m.h, 0)
+
+    # Timed window: K coarse steps of the driver (each step replayed from a recorded HIP graph),
+    # bracketed by HIP events on the context stream.
+    m.sync()
+    st, step_ms, rebuild_ms = m.advance_loop(args.warmup, args.steps)
+    m.sync()
+    s = m.refresh_sizes()
+    n_ids_end = [int(s.n_ids[L]) for L in range(levels)]
+
+    # Cell updates per coarse step (the mesh is fixed between rebuilds).
+    cells = 64
+    upd_actual = sum(n_ids_end[L] * cells * (1 << L) for L in range(levels))
+    leaves = [n_ids_end[L] - (n_ids_end[L + 1] // 8 if L + 1 < levels else 0) for L in range(levels)]
+    upd_reference = sum(leaves[L] * cells * (1 << L) for L in range(levels))
+    upd_fine_equiv = (N << (levels - 1)) ** 3 * (1 << (levels - 1))
+    K = args.steps
+    step_s = step_ms * 1e-3
+    mlups = upd_actual * K / step_s / 1e6
+    mlups_ref = upd_reference * K / step_s / 1e6
+    mlups_fine = upd_fine_equiv * K / step_s / 1e6
+    mlups_e2e = upd_actual * K / ((step_ms + rebuild_ms) * 1e-3) / 1e6
+    fused_ms = fms.value
+    launches = nl.value
+    alg_bytes = 2 * Q * b * fcells.value
+    achieved = alg_bytes / (fused_ms * 1e-3) / 1e9 if fused_ms > 0 else 0.0
+
+    # Rebuild cost on this mesh (one solution-based rebuild after the window).
+    t1 = time.time()
+    m.M_Advance_RefineWithSolution()
+    m.sync()
+    rebuild_one_ms = (time.time() - t1) * 1e3
+
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "fused_step_cfg2_traffic.json")
+    if os.path.exists(tp):
+        try:
+            traffic = json.load(open(tp)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(m, p, args.capacity, upd_actual, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms / K, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: lid-driven cavity from rest, mesh produced by the solver's own refinement",
+        "config": {
+            "workload": "D3Q19 BGK lid-driven cavity, root 64^3 (4x4x4 blocks), 3 levels, Re 1000, "
+                        "U 0.05, rebuild every 100 coarse steps, FP32 SoA",
+            "blocks_per_level": n_ids_end,
+            "cell_updates_per_step": upd_actual,
+            "parallelism": "single GPU",
+        },
+        "mlups_reference_formula": round(mlups_ref, 1),
+        "mlups_fine_equivalent": round(mlups_fine, 1),
+        "mlups_end_to_end": round(mlups_e2e, 1),
+        "rebuild_ms_in_window": round(rebuild_ms, 3),
+        "rebuild_ms_one": round(rebuild_one_ms, 3),
+        "warmup_seconds": round(warm_s, 3),
+        "step_hbm_gbs_algorithmic": round(2 * Q * b * upd_actual * K / step_s / 1e9, 1),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "k_collide_stream<float,19>",
+            "launches": launches,
+            "avg_launch_ms": round(fused_ms / max(1, launches), 5),
+            "bytes_per_launch_algorithmic": round(alg_bytes / max(1, launches)),
+        },
+        "cpu_baseline": cpu,
+    }
+    txt = json.dumps(line)
+    print(txt, flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(txt + "\n")
+
+
+def cpu_baseline(m, p, capacity, upd_per_step, budget_s):
+    """The CPU restatement stepping the same mesh (state copied from the GPU) on the host cores."""
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import OracleMesh, set_threads
+    except Exception as e:  # the checker is not built
+        return {"value": None, "unit": "MLUPS", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = set_threads(max(1, min(ncpu, 16)))
+    o = OracleMesh(p, capacity)
+    o.import_state(m.state())
+    o.S_Advance(0)  # first touch
+    steps, t = 0, 0.0
+    while t < budget_s and steps < 200:
+        t0 = time.time()
+        o.S_Advance(0)
+        t += time.time() - t0
+        steps += 1
+    o.close()
+    return {
+        "value": round(upd_per_step * steps / t / 1e6, 2),
+        "unit": "MLUPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} coarse steps of the same 3-level mesh (FP32, OpenMP over blocks), {t:.1f} s",
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -205,6 +205,12 @@ int  agal_download_gap_set(agal_ctx *ctx, int *dst, int n);
 #define AGAL_BLK_ONB        4
 #define AGAL_BLK_MASK       5
 
+/* ---- options (no counterpart in the reference, which fixes these at compile time) ------------
+ * AGAL_OPT_GRAPHS: 1 (default) records each coarse step of agal_advance / agal_advance_loop as a
+ * HIP graph and replays it until the mesh changes; 0 launches every kernel directly. */
+#define AGAL_OPT_GRAPHS 1
+int  agal_set_option(agal_ctx *ctx, int option, long long value);
+
 /* ---- measurement ----------------------------------------------------------------------------- */
 /* Average device time (ms) per launch of the fused level kernel since the last reset, measured
  * with HIP events on the context stream; `launches` receives the launch count. */

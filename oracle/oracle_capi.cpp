@@ -277,6 +277,40 @@ int oracle_download_gap_set(void *h, int *dst, int n)
     });
 }
 
+/* Replace the whole block-pool state (tables for blocks [0, n_blocks), per-level ID sets given
+ * concatenated in level order, the gap stack and the ID maxima) -- used to time the restatement on
+ * a mesh produced elsewhere. */
+int oracle_import_pool(void *h, long long n_blocks, const int *nbr, const int *nbr_child, const int *ref,
+                       const int *level, const int *onb, const int *blk_mask, const float *coords,
+                       const int *cell_mask, const int *n_ids, const int *id_max, const int *id_sets,
+                       int n_gaps, const int *gap_set)
+{
+    return dispatch(h, [&](auto &O) {
+        if (n_blocks < 0 || n_blocks > O.n_maxcblocks || n_gaps < 0 || n_gaps > O.n_maxcblocks) return AGAL_ERR_ARG;
+        for (int p = 0; p < O.Q_max; p++) {
+            std::memcpy(&O.nbr[(size_t)p * O.n_maxcblocks], nbr + p * n_blocks, sizeof(int) * n_blocks);
+            std::memcpy(&O.nbr_child[(size_t)p * O.n_maxcblocks], nbr_child + p * n_blocks, sizeof(int) * n_blocks);
+        }
+        std::memcpy(O.ref.data(), ref, sizeof(int) * n_blocks);
+        std::memcpy(O.level.data(), level, sizeof(int) * n_blocks);
+        std::memcpy(O.onb.data(), onb, sizeof(int) * n_blocks);
+        std::memcpy(O.blk_mask.data(), blk_mask, sizeof(int) * n_blocks);
+        for (int d = 0; d < O.DIM; d++)
+            std::memcpy(&O.X[(size_t)d * O.n_maxcblocks], coords + d * n_blocks, sizeof(float) * n_blocks);
+        std::memcpy(O.cells_mask.data(), cell_mask, sizeof(int) * n_blocks * O.M_CBLOCK);
+        long long off = 0;
+        for (int L = 0; L < O.MAX_LEVELS; L++) {
+            O.n_ids[L] = n_ids[L];
+            std::memcpy(&O.id_set[(size_t)L * O.n_maxcblocks], id_sets + off, sizeof(int) * n_ids[L]);
+            off += n_ids[L];
+        }
+        for (int L = 0; L <= O.MAX_LEVELS; L++) O.id_max[L] = id_max[L];
+        O.n_gaps = n_gaps;
+        std::memcpy(O.gap_set.data(), gap_set, sizeof(int) * n_gaps);
+        return AGAL_OK;
+    });
+}
+
 /* The restated velocity set: c (3 rows of 27), w, pb, conn_map (all of length Q_max). */
 int oracle_velset(int Q, int *c, double *w, int *pb, int *conn_map)
 {

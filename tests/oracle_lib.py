@@ -13,7 +13,8 @@ from agal_amd.mesh import LbmMesh
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
-_NAMES = [k for k in capi.PROTOTYPES if not k.startswith("timing")]
+_NAMES = [k for k in capi.PROTOTYPES
+          if not k.startswith("timing") and k not in ("set_option", "upload_cell_masks", "upload_block_ints")]
 _lib = None
 
 
@@ -82,6 +83,41 @@ class OracleMesh(LbmMesh):
         m = np.ascontiguousarray(m, dtype=self.dtype)
         nc = m.shape[1]
         self._call_raw("upload_macros", m.ctypes.data_as(C.c_void_p), nc // self.sizes.cells_per_block, nc)
+
+    def import_state(self, st: dict):
+        """Adopt a block pool and populations downloaded from another context (LbmMesh.state())."""
+        import numpy as np
+        lib = oracle_library()
+        nb = int(st["id_max"])
+        P = C.POINTER(C.c_int)
+
+        def ip(a):
+            return np.ascontiguousarray(a, dtype=np.int32).ctypes.data_as(P)
+
+        keep = []
+
+        def arr(a, dt=np.int32):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_float) if dt == np.float32 else P)
+
+        s = self.sizes
+        id_max = [0] * (s.max_levels + 1)
+        id_max[s.max_levels] = nb
+        for L in range(s.max_levels):
+            ids = st["id_sets"][L]
+            id_max[L] = int(ids.max()) + 1 if ids.size else 0
+        id_max[0] = s.n_coarsecblocks
+        f = lib.oracle_import_pool
+        f.restype = C.c_int
+        st_ = f(self.h, C.c_longlong(nb), arr(st["nbr"]), arr(st["nbr_child"]), arr(st["ref"]), arr(st["level"]),
+                arr(st["onb"]), arr(st["blk_mask"]), arr(st["coords"], np.float32), arr(st["cell_mask"]),
+                arr(np.array(st["n_ids"])), arr(np.array(id_max)), arr(np.concatenate(st["id_sets"])),
+                C.c_int(len(st["gap_set"])), arr(st["gap_set"]))
+        if st_ != capi.AGAL_OK:
+            raise capi.AgalError(st_, "oracle_import_pool")
+        self.refresh_sizes()
+        self.set_populations(st["f"])
 
     def _call_raw(self, name, *args):
         st = getattr(oracle_library(), "oracle_" + name)(self.h, *args)

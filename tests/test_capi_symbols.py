@@ -31,9 +31,7 @@ def test_every_declared_symbol_is_exported(lib):
 
 
 def test_python_bindings_cover_the_header():
-    assert sorted(capi.EXPORTED) == sorted(n for n in declared() if not n.startswith("agal_upload_cell")
-                                           and n not in ("agal_upload_block_ints",)) or \
-        set(capi.EXPORTED) <= set(declared())
+    assert sorted(capi.EXPORTED) == declared()
 
 
 def test_params_default_matches_host_defaults(lib):

@@ -130,6 +130,31 @@ def test_stream_guard_exceptions():
     assert_same(g, o)
 
 
+def test_graph_replays_match_direct_launches():
+    """Each coarse step is replayed from a recorded HIP graph; interleaved with rebuilds, direct
+    calls and host synchronisations, every replay must equal the directly launched run."""
+    p = ldc_params(vs=capi.D3Q19, N=16, Re=40.0, levels=3, P_REFINE=7)
+    a, b = LbmMesh(p, 8192), LbmMesh(p, 8192)
+    b.set_option(capi.OPT_GRAPHS, 0)
+    for m in (a, b):
+        m.S_SetIC(0)
+    for k in range(30):
+        a.S_Advance(0)
+        b.S_Advance(0)
+        if k % 7 == 6:
+            for m in (a, b):
+                m.M_Advance_RefineWithSolution()
+        if k % 3 == 0:
+            a.sync()
+            bad = compare_states(a.state(), b.state())
+            assert not bad, (k, bad)
+        if k == 11:
+            for m in (a, b):  # direct launches between replays
+                m.collide_stream(1)
+                m.collide_stream(1)
+    assert not compare_states(a.state(), b.state())
+
+
 def test_pool_exhausted_cancels_refinement():
     p = ldc_params(vs=capi.D3Q19, N=8, levels=2)
     g, o = pair(p, 8 + 12)  # room for one refined block only
