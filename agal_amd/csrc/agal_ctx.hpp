@@ -69,11 +69,12 @@ struct Ctx {
     int *first_child = nullptr, *scan_a = nullptr, *scan_b = nullptr, *partials = nullptr;
     int *dscal = nullptr;     // small device scalars (counters)
     int *hscal = nullptr;     // pinned host mirror
-    // Exceptions of the stream guard: (initiator cell, partner cell, direction) + value.
+    // Exceptions of the stream guard: (initiator cell, partner cell, direction) + value, one list
+    // of exc_cap entries per level.
     long long exc_cap = 0;
     int *exc_idx = nullptr;   // 3 ints per entry
     void *exc_val = nullptr;  // one value per entry
-    int *exc_count = nullptr; // [0] count, [1] overflow flag
+    int *exc_count = nullptr; // per level: [2L] count, [2L+1] overflow flag
 
     LevelTiming timing;
 
@@ -88,6 +89,21 @@ struct Ctx {
     unsigned mesh_epoch = 1;
     bool use_graphs = true;
 
+    // Level scheduler used while a step is being recorded: each kernel goes to the stream of the
+    // level it writes and waits only for the last writer / readers of the levels it touches, so
+    // the recorded graph lets coarse-level work overlap the finer levels' sub-cycles.
+    struct LevelDeps {
+        int w_ev = -1, w_stream = -1;            // last writer
+        std::vector<std::pair<int, int>> readers; // (event, stream) since the last write
+    };
+    bool sched_on = false;
+    hipStream_t lstream[8] = {};
+    bool lstream_joined[8] = {};
+    hipEvent_t sched_start = nullptr;
+    std::vector<hipEvent_t> evpool;
+    int ev_next = 0;
+    LevelDeps deps[8];
+
     int *id_set_L(int L) const { return id_set + (size_t)L * n_maxcblocks; }
     int *iface_L(int L) const { return iface + (size_t)L * n_maxcblocks; }
 };
@@ -99,6 +115,12 @@ int hip_fail(Ctx *c, hipError_t e, const char *what);
         hipError_t _e = (call);                             \
         if (_e != hipSuccess) return ::agal::hip_fail((c), _e, #call); \
     } while (0)
+
+// Stream for a kernel that reads levels `rmask` and writes levels `wmask` (bit L = level L),
+// placed on the stream of level `home`; inserts the waits it needs while recording.  Without a
+// recording in progress this is the context stream.  sched_done() records the kernel's event.
+hipStream_t sched_begin(Ctx *c, int home, unsigned rmask, unsigned wmask);
+int sched_done(Ctx *c, hipStream_t s, int home, unsigned rmask, unsigned wmask);
 
 // Kernel launchers (lbm_kernels.hip).
 int launch_set_ic(Ctx *c, int L);

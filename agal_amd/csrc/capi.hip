@@ -27,6 +27,57 @@ int hip_fail(Ctx *c, hipError_t e, const char *what)
     return AGAL_ERR_HIP;
 }
 
+static hipError_t sched_event(Ctx *c, int &idx)
+{
+    if (c->ev_next >= (int)c->evpool.size()) {
+        hipEvent_t e;
+        hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        if (r != hipSuccess) return r;
+        c->evpool.push_back(e);
+    }
+    idx = c->ev_next++;
+    return hipSuccess;
+}
+
+hipStream_t sched_begin(Ctx *c, int home, unsigned rmask, unsigned wmask)
+{
+    if (!c->sched_on) return c->stream;
+    hipStream_t s = c->lstream[home];
+    if (!c->lstream_joined[home]) {
+        (void)hipStreamWaitEvent(s, c->sched_start, 0);
+        c->lstream_joined[home] = true;
+    }
+    const unsigned all = rmask | wmask;
+    for (int L = 0; L < 8; L++) {
+        if (!(all >> L & 1)) continue;
+        const auto &d = c->deps[L];
+        if (d.w_ev >= 0 && d.w_stream != home) (void)hipStreamWaitEvent(s, c->evpool[d.w_ev], 0);
+        if (wmask >> L & 1)
+            for (auto &r : d.readers)
+                if (r.second != home) (void)hipStreamWaitEvent(s, c->evpool[r.first], 0);
+    }
+    return s;
+}
+
+int sched_done(Ctx *c, hipStream_t s, int home, unsigned rmask, unsigned wmask)
+{
+    if (!c->sched_on) return AGAL_OK;
+    int e;
+    AGAL_HIP(c, sched_event(c, e));
+    AGAL_HIP(c, hipEventRecord(c->evpool[e], s));
+    for (int L = 0; L < 8; L++) {
+        auto &d = c->deps[L];
+        if (wmask >> L & 1) {
+            d.w_ev = e;
+            d.w_stream = home;
+            d.readers.clear();
+        } else if (rmask >> L & 1) {
+            d.readers.push_back({e, home});
+        }
+    }
+    return AGAL_OK;
+}
+
 // Element-wise gather (dir 0) or scatter (dir 1) between a block's current population buffer
 // and a contiguous staging array, Q rows of n_blocks*cells.
 template <typename T>
@@ -116,6 +167,10 @@ void free_ctx(Ctx *c)
     if (c->hscal) (void)hipHostFree(c->hscal);
     for (auto &g : c->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+    if (c->sched_start) (void)hipEventDestroy(c->sched_start);
+    for (hipStream_t s : c->lstream)
+        if (s) (void)hipStreamDestroy(s);
     for (hipEvent_t e : c->timing.ev_pairs) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -290,8 +345,28 @@ int step_once(Ctx *c, int L0)
     }
     hipGraph_t graph = nullptr;
     AGAL_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int st = advance_rec(c, L0);
-    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    // Record with the level scheduler: fork the level streams from the capture stream, join
+    // every used one back before the capture ends.
+    c->ev_next = 0;
+    for (int L = 0; L < 8; L++) {
+        c->deps[L] = Ctx::LevelDeps();
+        c->lstream_joined[L] = false;
+    }
+    int st = AGAL_OK;
+    hipError_t e = hipEventRecord(c->sched_start, c->stream);
+    if (e != hipSuccess) st = hip_fail(c, e, "hipEventRecord");
+    c->sched_on = (st == AGAL_OK);
+    if (st == AGAL_OK) st = advance_rec(c, L0);
+    c->sched_on = false;
+    for (int L = 0; L < 8; L++) {  // joined also after a failure so that the capture can end
+        if (!c->lstream_joined[L]) continue;
+        int ev;
+        e = sched_event(c, ev);
+        if (e == hipSuccess) e = hipEventRecord(c->evpool[ev], c->lstream[L]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->evpool[ev], 0);
+        if (e != hipSuccess && st == AGAL_OK) st = hip_fail(c, e, "join level stream");
+    }
+    e = hipStreamEndCapture(c->stream, &graph);
     if (st != AGAL_OK) {
         if (graph) (void)hipGraphDestroy(graph);
         set_cur_mask(c, before);
@@ -335,13 +410,14 @@ int refine_with_solution_impl(Ctx *c, int out_counts[4])
 
 int check_exceptions(Ctx *c)
 {
-    int v[2] = {0, 0};
-    AGAL_HIP(c, hipMemcpyAsync(v, c->exc_count, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+    int v[16] = {};
+    AGAL_HIP(c, hipMemcpyAsync(v, c->exc_count, sizeof(int) * 2 * c->MAX_LEVELS, hipMemcpyDeviceToHost, c->stream));
     AGAL_HIP(c, hipStreamSynchronize(c->stream));
-    if (v[1]) {
-        c->last_error = "stream-guard exception list overflowed";
-        return AGAL_ERR_OVERFLOW;
-    }
+    for (int L = 0; L < c->MAX_LEVELS; L++)
+        if (v[2 * L + 1]) {
+            c->last_error = "stream-guard exception list of level " + std::to_string(L) + " overflowed";
+            return AGAL_ERR_OVERFLOW;
+        }
     return AGAL_OK;
 }
 
@@ -471,6 +547,9 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
         return st;
     }
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int L = 0; L < c->MAX_LEVELS && e == hipSuccess; L++)
+        e = hipStreamCreateWithFlags(&c->lstream[L], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->sched_start, hipEventDisableTiming);
     if (e != hipSuccess) {
         st = hip_fail(c, e, "hipStreamCreate");
         free_ctx(c);
@@ -507,9 +586,9 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     ALLOC(c->scan_b, sizeof(int) * nmax);
     ALLOC(c->partials, sizeof(int) * ((nmax + 2047) / 2048 + 64));
     ALLOC(c->dscal, sizeof(int) * 64);
-    ALLOC(c->exc_idx, sizeof(int) * 3 * c->exc_cap);
-    ALLOC(c->exc_val, ts * c->exc_cap);
-    ALLOC(c->exc_count, sizeof(int) * 4);
+    ALLOC(c->exc_idx, sizeof(int) * 3 * c->exc_cap * c->MAX_LEVELS);
+    ALLOC(c->exc_val, ts * c->exc_cap * c->MAX_LEVELS);
+    ALLOC(c->exc_count, sizeof(int) * 2 * c->MAX_LEVELS);
 #undef ALLOC
     e = hipHostMalloc((void **)&c->hscal, sizeof(int) * 64, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -523,7 +602,7 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     if ((st = fill(c->cells_mask, ncm, CELL_INTERIOR)) || (st = fill(c->blk_mask, nmax, BLK_REGULAR)) ||
         (st = fill(c->nbr, c->QMAX * nmax, N_SKIPID)) || (st = fill(c->nbr_child, c->QMAX * nmax, N_SKIPID)) ||
         (st = fill(c->ref, nmax, REF_INACTIVE)) || (st = fill(c->level, nmax, 0)) || (st = fill(c->onb, nmax, 0)) ||
-        (st = fill((int *)c->map_stamp, nmax, 0)) || (st = fill(c->exc_count, 4, 0)) || (st = fill(c->part, nmax, 1)) ||
+        (st = fill((int *)c->map_stamp, nmax, 0)) || (st = fill(c->exc_count, 2 * c->MAX_LEVELS, 0)) || (st = fill(c->part, nmax, 1)) ||
         (st = fill(c->simple, nmax, 0)) || (st = fill((int *)c->X, c->DIM * nmax, 0))) {
         free_ctx(c);
         return st;

@@ -244,22 +244,27 @@ __global__ __launch_bounds__(WG) void k_fixup(T *__restrict__ B, long long ncm, 
 }
 
 // ---------------------------------------------------------------------------------------------
-// Coarse -> fine: Cu_Interpolate_Linear (solver_lbm_kernels_comm.cu:13-138).  One wavefront per
-// parent block (3D).  All Q rescaled parent populations go to LDS behind one barrier; each lane
-// then writes its cell of the eight children (row-contiguous stores).
+// Coarse -> fine: Cu_Interpolate_Linear (solver_lbm_kernels_comm.cu:13-138).  A 512-thread
+// workgroup owns one parent block in 3D (eight 4x4 parents in 2D): the first CELLS lanes of each
+// parent put the Q rescaled parent populations in LDS, then every child cell gets one lane, so the
+// eight children are written by eight wavefronts in parallel with row-contiguous stores.
 // ---------------------------------------------------------------------------------------------
+constexpr int IWG = 512;
+
 template <typename T, int Q, bool ADDED>
-__global__ __launch_bounds__(WG) void k_interp(const int *__restrict__ ids, int n, const T *__restrict__ Fp,
-                                               T *__restrict__ Fc, long long ncm, const int *__restrict__ nbr_child,
-                                               const int *__restrict__ blk_mask, const int *__restrict__ ref,
-                                               const int *__restrict__ cells_mask, T tau_ratio)
+__global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int n, const T *__restrict__ Fp,
+                                                T *__restrict__ Fc, long long ncm, const int *__restrict__ nbr_child,
+                                                const int *__restrict__ blk_mask, const int *__restrict__ ref,
+                                                const int *__restrict__ cells_mask, T tau_ratio)
 {
     using VS = VelSet<Q>;
     constexpr int CELLS = VS::CELLS, DIM = VS::DIM, NCH = VS::CHILDREN;
-    __shared__ T s_all[Q * WG];
-    const int t = threadIdx.x % CELLS;
-    const int base = threadIdx.x - t;
-    const int bi = blockIdx.x * (WG / CELLS) + threadIdx.x / CELLS;
+    constexpr int GROUP = NCH * CELLS;   // lanes per parent: one per child cell
+    constexpr int PPW = IWG / GROUP;     // parents per workgroup
+    __shared__ T s_all[Q * PPW * CELLS];
+    const int g = threadIdx.x / GROUP, r = threadIdx.x % GROUP;
+    const int c = r / CELLS, t = r % CELLS;
+    const int bi = blockIdx.x * PPW + g;
     int b = -1;
     if (bi < n) b = ids[bi];
     bool work = false;
@@ -268,64 +273,56 @@ __global__ __launch_bounds__(WG) void k_interp(const int *__restrict__ ids, int 
         fc = nbr_child[b];
         work = fc >= 0 && (ADDED ? (ref[b] == REF_MARK_REFINE) : (blk_mask[b] == BLK_INTERFACE));
     }
-    const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
-    const T Ix = (T)(-0.25) + (T)(0.5) * (T)I;
-    const T Iy = (T)(-0.25) + (T)(0.5) * (T)J;
-    const T Iz = (DIM == 3) ? (T)(-0.25) + (T)(0.5) * (T)K : (T)0;
-    T f[Q];
-    T rho = 0, u = 0, v = 0, w = 0;
-    bool ghost[NCH];
-    if (work) {
+    T *sg = s_all + g * CELLS;
+    if (work && c == 0) {
         const long long cell = (long long)b * CELLS + t;
+        T f[Q];
 #pragma unroll
         for (int p = 0; p < Q; p++) f[p] = Fp[(long long)p * ncm + cell];
+        T rho, u, v, w;
         moments<T, Q>(f, rho, u, v, w);
-#pragma unroll
-        for (int c = 0; c < NCH; c++)
-            ghost[c] = ADDED || cells_mask[(long long)(fc + c) * CELLS + t] == CELL_GHOST;
-    }
-    const T udotu = u * u + v * v + w * w;
-    if (work) {
+        const T udotu = u * u + v * v + w * w;
 #pragma unroll
         for (int p = 0; p < Q; p++) {
             T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
             T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
-            s_all[p * WG + threadIdx.x] = fe + (f[p] - fe) * tau_ratio;
+            sg[p * PPW * CELLS + t] = fe + (f[p] - fe) * tau_ratio;
         }
     }
     __syncthreads();
     if (!work) return;
+    const long long ccell = (long long)(fc + c) * CELLS + t;
+    if (!ADDED && cells_mask[ccell] != CELL_GHOST) return;
+    const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
+    const T Ix = (T)(-0.25) + (T)(0.5) * (T)I;
+    const T Iy = (T)(-0.25) + (T)(0.5) * (T)J;
+    const T Iz = (DIM == 3) ? (T)(-0.25) + (T)(0.5) * (T)K : (T)0;
+    const int ci = c % 2, cj = (c / 2) % 2, ck = (c / 4) % 2;
 #pragma unroll
     for (int p = 0; p < Q; p++) {
-        {
-            const T *s = s_all + p * WG + base;
-#pragma unroll
-            for (int c = 0; c < NCH; c++) {
-                const int ci = c % 2, cj = (c / 2) % 2, ck = (c / 4) % 2;
-                if (!ghost[c]) continue;
-                T val;
-                if (DIM == 2) {
-                    const int o = 2 * ci + 2 * 4 * cj;
-                    val = s[0 + o] + (s[1 + o] - s[0 + o]) * Ix + (s[4 + o] - s[0 + o]) * Iy +
-                          (s[5 + o] - s[4 + o] - s[1 + o] + s[0 + o]) * Ix * Iy;
-                } else {
-                    const int o = 2 * ci + 2 * 4 * cj + 2 * 4 * 4 * ck;
-                    val = s[0 + o] + (s[1 + o] - s[0 + o]) * Ix + (s[4 + o] - s[0 + o]) * Iy + (s[16 + o] - s[0 + o]) * Iz +
-                          (s[5 + o] - s[1 + o] - s[4 + o] + s[0 + o]) * Ix * Iy +
-                          (s[17 + o] - s[1 + o] - s[16 + o] + s[0 + o]) * Ix * Iz +
-                          (s[20 + o] - s[4 + o] - s[16 + o] + s[0 + o]) * Iy * Iz +
-                          (s[21 + o] + s[1 + o] + s[4 + o] + s[16 + o] - s[5 + o] - s[17 + o] - s[20 + o] - s[0 + o]) * Ix * Iy * Iz;
-                }
-                Fc[(long long)p * ncm + (long long)(fc + c) * CELLS + t] = val;
-            }
+        const T *s = sg + p * PPW * CELLS;
+        T val;
+        if (DIM == 2) {
+            const int o = 2 * ci + 2 * 4 * cj;
+            val = s[0 + o] + (s[1 + o] - s[0 + o]) * Ix + (s[4 + o] - s[0 + o]) * Iy +
+                  (s[5 + o] - s[4 + o] - s[1 + o] + s[0 + o]) * Ix * Iy;
+        } else {
+            const int o = 2 * ci + 2 * 4 * cj + 2 * 4 * 4 * ck;
+            val = s[0 + o] + (s[1 + o] - s[0 + o]) * Ix + (s[4 + o] - s[0 + o]) * Iy + (s[16 + o] - s[0 + o]) * Iz +
+                  (s[5 + o] - s[1 + o] - s[4 + o] + s[0 + o]) * Ix * Iy +
+                  (s[17 + o] - s[1 + o] - s[16 + o] + s[0 + o]) * Ix * Iz +
+                  (s[20 + o] - s[4 + o] - s[16 + o] + s[0 + o]) * Iy * Iz +
+                  (s[21 + o] + s[1 + o] + s[4 + o] + s[16 + o] - s[5 + o] - s[17 + o] - s[20 + o] - s[0 + o]) * Ix * Iy * Iz;
         }
+        Fc[(long long)p * ncm + ccell] = val;
     }
 }
 
 // ---------------------------------------------------------------------------------------------
-// Fine -> coarse: Cu_Average (solver_lbm_kernels_comm.cu:174-347).  One lane per parent cell
-// gathers the 2x2(x2) child cells of its quadrant's child block, rescales their populations and
-// sums them in the reference's order (no LDS, no barriers).
+// Fine -> coarse: Cu_Average (solver_lbm_kernels_comm.cu:174-347).  NG consecutive lanes (NG = 8
+// in 3D, 4 in 2D) serve one parent cell, each lane loading and rescaling one child cell of the
+// cell's 2x2(x2) group; the group sum is formed in the reference's order from the NG lanes by
+// shuffles and each lane stores the directions p = lane (mod NG).  No LDS, no barriers.
 // ---------------------------------------------------------------------------------------------
 template <typename T, int Q>
 __global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int n, T *__restrict__ Fp,
@@ -335,26 +332,35 @@ __global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int
 {
     using VS = VelSet<Q>;
     constexpr int CELLS = VS::CELLS, DIM = VS::DIM;
-    constexpr int NG = (DIM == 3) ? 8 : 4;  // child cells per parent cell
-    const int t = threadIdx.x % CELLS;
-    const int bi = blockIdx.x * (WG / CELLS) + threadIdx.x / CELLS;
-    if (bi >= n) return;
-    const int b = ids[bi];
-    const int fc = nbr_child[b];
-    if (!((fc > -1) && (ave_type == 2 || blk_mask[b] == BLK_INTERFACE))) return;
+    constexpr int NG = (DIM == 3) ? 8 : 4;     // child cells per parent cell
+    constexpr int PER_BLOCK = CELLS * NG;       // lanes per parent block
+    const int k = threadIdx.x % NG;             // which child cell of the group
+    const int lane = threadIdx.x % 64;
+    const int gl = lane - k;                    // first lane of the group
+    const long long gid = (long long)blockIdx.x * WG + threadIdx.x;
+    const long long bi = gid / PER_BLOCK;
+    const int t = (int)((gid % PER_BLOCK) / NG);  // parent cell
+    bool work = bi < n;
+    int b = 0, fc = -1;
+    if (work) {
+        b = ids[bi];
+        fc = nbr_child[b];
+        work = (fc > -1) && (ave_type == 2 || blk_mask[b] == BLK_INTERFACE);
+    }
     const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
     // Child block of this cell's quadrant and the first cell of its 2x2(x2) group.
     const int c = (I / 2) + 2 * (J / 2) + 4 * (K / 2);
     int c0 = 2 * (I % 2) + 4 * (2 * (J % 2));
     if (DIM == 3) c0 += 4 * 4 * (2 * (K % 2));
     const long long cb = (long long)(fc + c) * CELLS;
-    int m = cells_mask[cb + c0];
-    if (ave_type > 0 && m < 2) m = 1;
-    if (m != 1) return;
-    T acc[Q];
-#pragma unroll
-    for (int g = 0; g < NG; g++) {
-        const int off = (g & 1) + 4 * ((g >> 1) & 1) + 16 * ((g >> 2) & 1);
+    if (work) {
+        int m = cells_mask[cb + c0];
+        if (ave_type > 0 && m < 2) m = 1;
+        work = (m == 1);  // uniform over the NG lanes of the group
+    }
+    const int off = (k & 1) + 4 * ((k >> 1) & 1) + 16 * ((k >> 2) & 1);
+    T s[Q];
+    if (work) {
         const long long ccell = cb + c0 + off;
         T f[Q];
 #pragma unroll
@@ -367,13 +373,22 @@ __global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int
             T cdotu = (DIM == 3) ? (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w
                                  : (T)VS::cx(p) * u + (T)VS::cy(p) * v;
             T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
-            T s = fe + (f[p] - fe) * tau_ratio;
-            acc[p] = (g == 0) ? s : acc[p] + s;
+            s[p] = fe + (f[p] - fe) * tau_ratio;
         }
+    } else {
+#pragma unroll
+        for (int p = 0; p < Q; p++) s[p] = (T)0;
     }
     const T scale = (DIM == 2) ? (T)(0.25) : (T)(0.125);
+    const long long pcell = (long long)b * CELLS + t;
 #pragma unroll
-    for (int p = 0; p < Q; p++) Fp[(long long)p * ncm + (long long)b * CELLS + t] = scale * acc[p];
+    for (int p = 0; p < Q; p++) {
+        // s of child cell offsets 0,1,4,5,16,17,20,21 summed left to right (comm.cu:330-339)
+        T acc = __shfl(s[p], gl, 64);
+#pragma unroll
+        for (int j = 1; j < NG; j++) acc = acc + __shfl(s[p], gl + j, 64);
+        if (work && (p % NG) == k) Fp[(long long)p * ncm + pcell] = scale * acc;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -383,6 +398,13 @@ template <typename T>
 static T *Fbuf(Ctx *c, int k) { return static_cast<T *>(c->F[k]); }
 
 static inline int nwg(long long nblocks, int cells) { return (int)((nblocks * cells + WG - 1) / WG); }
+
+template <int Q>
+static inline int interp_grid(long long nparents)
+{
+    constexpr int ppw = IWG / (VelSet<Q>::CHILDREN * VelSet<Q>::CELLS);
+    return (int)((nparents + ppw - 1) / ppw);
+}
 
 template <typename T, int Q>
 static int set_ic_t(Ctx *c, int L)
@@ -428,31 +450,35 @@ static int collide_stream_t(Ctx *c, int L)
     a.omegp = (T)(1.0) - a.omeg;
     a.U = (T)c->P.bc_velocity;
     a.bc_preset = c->P.bc_preset;
-    a.exc_idx = c->exc_idx;
-    a.exc_val = static_cast<T *>(c->exc_val);
-    a.exc_count = c->exc_count;
+    // Each level has its own exception list so that fused steps of different levels may run
+    // concurrently.
+    a.exc_idx = c->exc_idx + (size_t)3 * c->exc_cap * L;
+    a.exc_val = static_cast<T *>(c->exc_val) + (size_t)c->exc_cap * L;
+    a.exc_count = c->exc_count + 2 * L;
     a.exc_cap = c->exc_cap;
+    const unsigned lv = 1u << L;
+    hipStream_t s = sched_begin(c, L, lv, lv);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing.enabled) {
         AGAL_HIP(c, hipEventCreate(&e0));
         AGAL_HIP(c, hipEventCreate(&e1));
-        AGAL_HIP(c, hipEventRecord(e0, c->stream));
+        AGAL_HIP(c, hipEventRecord(e0, s));
     }
     const int grid = (nwg(n, VelSet<Q>::CELLS) + 7) / 8 * 8;  // a multiple of the XCD count
-    hipLaunchKernelGGL((k_collide_stream<T, Q>), dim3(grid), dim3(WG), 0, c->stream, a);
+    hipLaunchKernelGGL((k_collide_stream<T, Q>), dim3(grid), dim3(WG), 0, s, a);
     AGAL_HIP(c, hipGetLastError());
     if (c->timing.enabled) {
-        AGAL_HIP(c, hipEventRecord(e1, c->stream));
+        AGAL_HIP(c, hipEventRecord(e1, s));
         c->timing.ev_pairs.push_back(e0);
         c->timing.ev_pairs.push_back(e1);
         c->timing.fused_launches++;
         c->timing.fused_cells += (double)n * VelSet<Q>::CELLS;
     }
-    hipLaunchKernelGGL((k_fixup<T, Q>), dim3(1), dim3(WG), 0, c->stream, a.B, a.ncm, c->exc_idx, a.exc_val, c->exc_count,
-                       c->exc_cap);
+    hipLaunchKernelGGL((k_fixup<T, Q>), dim3(1), dim3(WG), 0, s, a.B, a.ncm, a.exc_idx, a.exc_val, a.exc_count,
+                       a.exc_cap);
     AGAL_HIP(c, hipGetLastError());
     c->cur[L] ^= 1;
-    return AGAL_OK;
+    return sched_done(c, s, L, lv, lv);
 }
 
 template <typename T, int Q>
@@ -470,16 +496,18 @@ static int interpolate_t(Ctx *c, int L, int var)
         n = c->n_ids[L];
     }
     if (n <= 0) return AGAL_OK;
+    const unsigned rl = 1u << L, wl = 1u << (L + 1);
+    hipStream_t s = sched_begin(c, L + 1, rl, wl);
     if (var == AGAL_INTERP_ADDED)
-        hipLaunchKernelGGL((k_interp<T, Q, true>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, ids, n,
+        hipLaunchKernelGGL((k_interp<T, Q, true>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
                            Fbuf<T>(c, c->cur[L]), Fbuf<T>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
                            c->ref, c->cells_mask, ratio);
     else
-        hipLaunchKernelGGL((k_interp<T, Q, false>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, ids, n,
+        hipLaunchKernelGGL((k_interp<T, Q, false>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
                            Fbuf<T>(c, c->cur[L]), Fbuf<T>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
                            c->ref, c->cells_mask, ratio);
     AGAL_HIP(c, hipGetLastError());
-    return AGAL_OK;
+    return sched_done(c, s, L + 1, rl, wl);
 }
 
 template <typename T, int Q>
@@ -497,11 +525,13 @@ static int average_t(Ctx *c, int L, int var)
         n = c->n_ids[L];
     }
     if (n <= 0) return AGAL_OK;
-    hipLaunchKernelGGL((k_average<T, Q>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, ids, n,
-                       Fbuf<T>(c, c->cur[L]), Fbuf<T>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
-                       c->cells_mask, var, ratio);
+    const unsigned rl = 1u << (L + 1), wl = 1u << L;
+    hipStream_t s = sched_begin(c, L, rl, wl);
+    hipLaunchKernelGGL((k_average<T, Q>), dim3(nwg(n, VelSet<Q>::CELLS * (VelSet<Q>::DIM == 3 ? 8 : 4))), dim3(WG), 0,
+                       s, ids, n, Fbuf<T>(c, c->cur[L]), Fbuf<T>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child,
+                       c->blk_mask, c->cells_mask, var, ratio);
     AGAL_HIP(c, hipGetLastError());
-    return AGAL_OK;
+    return sched_done(c, s, L, rl, wl);
 }
 
 #define AGAL_DISPATCH(c, FN, ...)                                                          \
