@@ -89,20 +89,24 @@ struct Ctx {
     unsigned mesh_epoch = 1;
     bool use_graphs = true;
 
-    // Level scheduler used while a step is being recorded: each kernel goes to the stream of the
-    // level it writes and waits only for the last writer / readers of the levels it touches, so
-    // the recorded graph lets coarse-level work overlap the finer levels' sub-cycles.
-    struct LevelDeps {
-        int w_ev = -1, w_stream = -1;            // last writer
-        std::vector<std::pair<int, int>> readers; // (event, stream) since the last write
+    // Scheduler used while a step is being recorded.  Resources are (level, buffer parity) pairs,
+    // bit 2L+b; each kernel waits only for the last writer and the readers since of the resources
+    // it touches.  Stream slot 0 is the context stream (the chain of the finest level's sub-cycle,
+    // interpolation and averaging); slot 1+L hosts the fused step of a coarser level L, which can
+    // then overlap the finer levels' sub-cycles in the recorded graph.
+    struct ResDeps {
+        int w_ev = -1, w_slot = -1;                // last writer (event, stream slot)
+        std::vector<std::pair<int, int>> readers; // (event, stream slot) since the last write
     };
+    static constexpr int NSLOT = 9, NRES = 16;
     bool sched_on = false;
     hipStream_t lstream[8] = {};
-    bool lstream_joined[8] = {};
+    bool slot_joined[NSLOT] = {};
+    int seen[NSLOT][NSLOT] = {};  // latest event of slot t already waited for by slot s
     hipEvent_t sched_start = nullptr;
     std::vector<hipEvent_t> evpool;
     int ev_next = 0;
-    LevelDeps deps[8];
+    ResDeps deps[NRES];
 
     int *id_set_L(int L) const { return id_set + (size_t)L * n_maxcblocks; }
     int *iface_L(int L) const { return iface + (size_t)L * n_maxcblocks; }
@@ -116,11 +120,12 @@ int hip_fail(Ctx *c, hipError_t e, const char *what);
         if (_e != hipSuccess) return ::agal::hip_fail((c), _e, #call); \
     } while (0)
 
-// Stream for a kernel that reads levels `rmask` and writes levels `wmask` (bit L = level L),
-// placed on the stream of level `home`; inserts the waits it needs while recording.  Without a
+// Stream for a kernel that reads resources `rmask` and writes `wmask` (bit 2L+b = buffer b of
+// level L), placed on stream slot `slot`; inserts the waits it needs while recording.  Without a
 // recording in progress this is the context stream.  sched_done() records the kernel's event.
-hipStream_t sched_begin(Ctx *c, int home, unsigned rmask, unsigned wmask);
-int sched_done(Ctx *c, hipStream_t s, int home, unsigned rmask, unsigned wmask);
+inline unsigned res_bit(int L, int b) { return 1u << (2 * L + (b & 1)); }
+hipStream_t sched_begin(Ctx *c, int slot, unsigned rmask, unsigned wmask);
+int sched_done(Ctx *c, hipStream_t s, int slot, unsigned rmask, unsigned wmask);
 
 // Kernel launchers (lbm_kernels.hip).
 int launch_set_ic(Ctx *c, int L);

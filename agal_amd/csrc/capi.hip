@@ -39,40 +39,45 @@ static hipError_t sched_event(Ctx *c, int &idx)
     return hipSuccess;
 }
 
-hipStream_t sched_begin(Ctx *c, int home, unsigned rmask, unsigned wmask)
+static hipStream_t slot_stream(Ctx *c, int slot) { return slot == 0 ? c->stream : c->lstream[slot - 1]; }
+
+hipStream_t sched_begin(Ctx *c, int slot, unsigned rmask, unsigned wmask)
 {
     if (!c->sched_on) return c->stream;
-    hipStream_t s = c->lstream[home];
-    if (!c->lstream_joined[home]) {
+    hipStream_t s = slot_stream(c, slot);
+    if (!c->slot_joined[slot]) {
         (void)hipStreamWaitEvent(s, c->sched_start, 0);
-        c->lstream_joined[home] = true;
+        c->slot_joined[slot] = true;
     }
-    const unsigned all = rmask | wmask;
-    for (int L = 0; L < 8; L++) {
-        if (!(all >> L & 1)) continue;
-        const auto &d = c->deps[L];
-        if (d.w_ev >= 0 && d.w_stream != home) (void)hipStreamWaitEvent(s, c->evpool[d.w_ev], 0);
-        if (wmask >> L & 1)
-            for (auto &r : d.readers)
-                if (r.second != home) (void)hipStreamWaitEvent(s, c->evpool[r.first], 0);
+    auto wait = [&](int ev, int from) {
+        if (from == slot || ev <= c->seen[slot][from]) return;
+        (void)hipStreamWaitEvent(s, c->evpool[ev], 0);
+        c->seen[slot][from] = ev;
+    };
+    for (int r = 0; r < Ctx::NRES; r++) {
+        if (!((rmask | wmask) >> r & 1)) continue;
+        const auto &d = c->deps[r];
+        if (d.w_ev >= 0) wait(d.w_ev, d.w_slot);
+        if (wmask >> r & 1)
+            for (auto &rd : d.readers) wait(rd.first, rd.second);
     }
     return s;
 }
 
-int sched_done(Ctx *c, hipStream_t s, int home, unsigned rmask, unsigned wmask)
+int sched_done(Ctx *c, hipStream_t s, int slot, unsigned rmask, unsigned wmask)
 {
     if (!c->sched_on) return AGAL_OK;
     int e;
     AGAL_HIP(c, sched_event(c, e));
     AGAL_HIP(c, hipEventRecord(c->evpool[e], s));
-    for (int L = 0; L < 8; L++) {
-        auto &d = c->deps[L];
-        if (wmask >> L & 1) {
+    for (int r = 0; r < Ctx::NRES; r++) {
+        auto &d = c->deps[r];
+        if (wmask >> r & 1) {
             d.w_ev = e;
-            d.w_stream = home;
+            d.w_slot = slot;
             d.readers.clear();
-        } else if (rmask >> L & 1) {
-            d.readers.push_back({e, home});
+        } else if (rmask >> r & 1) {
+            d.readers.push_back({e, slot});
         }
     }
     return AGAL_OK;
@@ -348,9 +353,10 @@ int step_once(Ctx *c, int L0)
     // Record with the level scheduler: fork the level streams from the capture stream, join
     // every used one back before the capture ends.
     c->ev_next = 0;
-    for (int L = 0; L < 8; L++) {
-        c->deps[L] = Ctx::LevelDeps();
-        c->lstream_joined[L] = false;
+    for (auto &d : c->deps) d = Ctx::ResDeps();
+    for (int k = 0; k < Ctx::NSLOT; k++) {
+        c->slot_joined[k] = false;
+        for (int t = 0; t < Ctx::NSLOT; t++) c->seen[k][t] = -1;
     }
     int st = AGAL_OK;
     hipError_t e = hipEventRecord(c->sched_start, c->stream);
@@ -358,11 +364,11 @@ int step_once(Ctx *c, int L0)
     c->sched_on = (st == AGAL_OK);
     if (st == AGAL_OK) st = advance_rec(c, L0);
     c->sched_on = false;
-    for (int L = 0; L < 8; L++) {  // joined also after a failure so that the capture can end
-        if (!c->lstream_joined[L]) continue;
+    for (int k = 1; k < Ctx::NSLOT; k++) {  // joined also after a failure so that the capture can end
+        if (!c->slot_joined[k]) continue;
         int ev;
         e = sched_event(c, ev);
-        if (e == hipSuccess) e = hipEventRecord(c->evpool[ev], c->lstream[L]);
+        if (e == hipSuccess) e = hipEventRecord(c->evpool[ev], slot_stream(c, k));
         if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->evpool[ev], 0);
         if (e != hipSuccess && st == AGAL_OK) st = hip_fail(c, e, "join level stream");
     }

@@ -18,6 +18,7 @@
 //     The partner knows its value, the initiator does not: the initiator always pushes, and a
 //     partner whose value fails the guard records the link in an exception list that a one-
 //     workgroup fix-up kernel resolves after the step (normally empty).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "agal_common.hpp"
@@ -456,29 +457,32 @@ static int collide_stream_t(Ctx *c, int L)
     a.exc_val = static_cast<T *>(c->exc_val) + (size_t)c->exc_cap * L;
     a.exc_count = c->exc_count + 2 * L;
     a.exc_cap = c->exc_cap;
-    const unsigned lv = 1u << L;
-    hipStream_t s = sched_begin(c, L, lv, lv);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // Reads buffer cur of level L, writes buffer cur^1.  The finest level's steps stay on the
+    // context stream; a coarser level's run beside the finer levels' sub-cycle.
+    const unsigned rl = res_bit(L, c->cur[L]), wl = res_bit(L, c->cur[L] ^ 1);
+    const int slot = L == c->MAX_LEVELS - 1 ? 0 : 1 + L;
+    hipStream_t s = sched_begin(c, slot, rl, wl);
+    const int grid = (nwg(n, VelSet<Q>::CELLS) + 7) / 8 * 8;  // a multiple of the XCD count
     if (c->timing.enabled) {
+        // Events recorded by the dispatch itself: they bracket the kernel's execution.
+        hipEvent_t e0 = nullptr, e1 = nullptr;
         AGAL_HIP(c, hipEventCreate(&e0));
         AGAL_HIP(c, hipEventCreate(&e1));
-        AGAL_HIP(c, hipEventRecord(e0, s));
-    }
-    const int grid = (nwg(n, VelSet<Q>::CELLS) + 7) / 8 * 8;  // a multiple of the XCD count
-    hipLaunchKernelGGL((k_collide_stream<T, Q>), dim3(grid), dim3(WG), 0, s, a);
-    AGAL_HIP(c, hipGetLastError());
-    if (c->timing.enabled) {
-        AGAL_HIP(c, hipEventRecord(e1, s));
+        hipExtLaunchKernelGGL((k_collide_stream<T, Q>), dim3(grid), dim3(WG), 0, s, e0, e1, 0, a);
+        AGAL_HIP(c, hipGetLastError());
         c->timing.ev_pairs.push_back(e0);
         c->timing.ev_pairs.push_back(e1);
         c->timing.fused_launches++;
         c->timing.fused_cells += (double)n * VelSet<Q>::CELLS;
+    } else {
+        hipLaunchKernelGGL((k_collide_stream<T, Q>), dim3(grid), dim3(WG), 0, s, a);
+        AGAL_HIP(c, hipGetLastError());
     }
     hipLaunchKernelGGL((k_fixup<T, Q>), dim3(1), dim3(WG), 0, s, a.B, a.ncm, a.exc_idx, a.exc_val, a.exc_count,
                        a.exc_cap);
     AGAL_HIP(c, hipGetLastError());
     c->cur[L] ^= 1;
-    return sched_done(c, s, L, lv, lv);
+    return sched_done(c, s, slot, rl, wl);
 }
 
 template <typename T, int Q>
@@ -496,8 +500,8 @@ static int interpolate_t(Ctx *c, int L, int var)
         n = c->n_ids[L];
     }
     if (n <= 0) return AGAL_OK;
-    const unsigned rl = 1u << L, wl = 1u << (L + 1);
-    hipStream_t s = sched_begin(c, L + 1, rl, wl);
+    const unsigned rl = res_bit(L, c->cur[L]), wl = res_bit(L + 1, c->cur[L + 1]);
+    hipStream_t s = sched_begin(c, 0, rl, wl);
     if (var == AGAL_INTERP_ADDED)
         hipLaunchKernelGGL((k_interp<T, Q, true>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
                            Fbuf<T>(c, c->cur[L]), Fbuf<T>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
@@ -507,7 +511,7 @@ static int interpolate_t(Ctx *c, int L, int var)
                            Fbuf<T>(c, c->cur[L]), Fbuf<T>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
                            c->ref, c->cells_mask, ratio);
     AGAL_HIP(c, hipGetLastError());
-    return sched_done(c, s, L + 1, rl, wl);
+    return sched_done(c, s, 0, rl, wl);
 }
 
 template <typename T, int Q>
@@ -525,13 +529,13 @@ static int average_t(Ctx *c, int L, int var)
         n = c->n_ids[L];
     }
     if (n <= 0) return AGAL_OK;
-    const unsigned rl = 1u << (L + 1), wl = 1u << L;
-    hipStream_t s = sched_begin(c, L, rl, wl);
+    const unsigned rl = res_bit(L + 1, c->cur[L + 1]), wl = res_bit(L, c->cur[L]);
+    hipStream_t s = sched_begin(c, 0, rl, wl);
     hipLaunchKernelGGL((k_average<T, Q>), dim3(nwg(n, VelSet<Q>::CELLS * (VelSet<Q>::DIM == 3 ? 8 : 4))), dim3(WG), 0,
                        s, ids, n, Fbuf<T>(c, c->cur[L]), Fbuf<T>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child,
                        c->blk_mask, c->cells_mask, var, ratio);
     AGAL_HIP(c, hipGetLastError());
-    return sched_done(c, s, L, rl, wl);
+    return sched_done(c, s, 0, rl, wl);
 }
 
 #define AGAL_DISPATCH(c, FN, ...)                                                          \

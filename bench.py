@@ -57,8 +57,8 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}))
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
-              "described in DESIGN.md but not built.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU partitioning is "
+              "a design, not built.", file=sys.stderr)
         sys.exit(2)
     import numpy as np
 
@@ -124,7 +124,7 @@ def main():
     rebuild_one_ms = (time.time() - t1) * 1e3
 
     traffic = None
-    tp = os.path.join(ROOT, "profiles", "fused_step_cfg2_traffic.json")
+    tp = os.path.join(ROOT, "profiles", "k_collide_stream_d3q19_f32_ldc64_3lev_traffic.json")
     if os.path.exists(tp):
         try:
             traffic = json.load(open(tp)).get("bytes_per_launch")

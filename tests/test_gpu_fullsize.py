@@ -1,4 +1,4 @@
-"""Parity at the benchmark configurations' own sizes.
+"""Parity at the benchmark workloads' own sizes.
 
 * D3Q19 lid-driven cavity, 64^3 root, 3 levels, Re 1000, rebuild every 100 coarse steps: the HIP
   library and the CPU restatement run the same driver through two solution-based rebuilds; the
@@ -22,13 +22,13 @@ from parity import compare_states, ldc_params
 pytestmark = pytest.mark.gpu
 
 
-def cfg2_params(**kw):
+def ldc64_params(**kw):
     return ldc_params(vs=capi.D3Q19, N=64, Re=1000.0, levels=3, P_REFINE=100, **kw)
 
 
-def test_cfg2_full_size_two_rebuilds():
+def test_ldc64_three_levels_two_rebuilds():
     set_threads(16)
-    p = cfg2_params()
+    p = ldc64_params()
     cap = 120000
     g, o = LbmMesh(p, cap), OracleMesh(p, cap)
     for m in (g, o):
@@ -47,7 +47,7 @@ def test_cfg2_full_size_two_rebuilds():
 
 
 @pytest.mark.parametrize("prec", [capi.FP32, capi.FP64])
-def test_cfg1_d2q9_128(prec):
+def test_d2q9_ldc128(prec):
     set_threads(16)
     p = ldc_params(vs=capi.D2Q9, N=128, Re=100.0, levels=1, prec=prec)
     g, o = LbmMesh(p, 1100), OracleMesh(p, 1100)
