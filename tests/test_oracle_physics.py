@@ -307,3 +307,75 @@ def test_criterion_q_invariant():
     W = np.floor(np.log2(abs(a * b)))  # Q-like = 0.5*(2 u_y v_x)
     ref = m.block_ints(capi.BLK_REF)
     assert np.all(ref == expected_marks(np.full(ref.size * 64, W), np.zeros(ref.size)))
+
+
+# ---- moments (Cu_RefreshVariables, solver_lbm_kernels.cu:110-159) --------------------------------
+@pytest.mark.parametrize("Q", [9, 19, 27])
+def test_refresh_returns_the_moments_of_equilibria(Q):
+    """Equilibria built from random (rho, u) have exactly those zeroth and first moments."""
+    dim = 2 if Q == 9 else 3
+    p = ldc_params(vs=Q, N=8, prec=capi.FP64, bc_preset=capi.BC_NONE)
+    m = OracleMesh(p, 64)
+    m.S_SetIC(0)
+    nc = m.populations().shape[1]
+    rng = np.random.default_rng(100 + Q)
+    rho = 1 + 0.2 * rng.uniform(-1, 1, nc)
+    u = np.zeros((3, nc))
+    u[:dim] = 0.1 * rng.uniform(-1, 1, (dim, nc))
+    m.set_populations(feq_np(Q, rho, u))
+    m.S_RefreshVariables(0)
+    mac = m.macros()
+    assert np.max(np.abs(mac[0] - rho)) < 1e-14
+    assert np.max(np.abs(mac[1:1 + dim] - u[:dim])) < 1e-14
+
+
+# ---- inflow / outflow presets of the boundary hook (inc/custom.h:87-97,107-117) ------------------
+@pytest.mark.parametrize("Q,preset", [(9, capi.BC_FPSC2D), (19, capi.BC_FPSC3D), (27, capi.BC_FPSC3D)])
+def test_inflow_outflow_presets_one_step_closed_form(Q, preset):
+    """One step from a uniform flow u' = (0.03, 0, 0) with inflow velocity U = 0.05, rho = 1.
+
+    Collision keeps the equilibrium; links inside the domain swap equal values; a link leaving the
+    domain keeps its own post-collision value in slot p after the hook: the moving wall (faces
+    -1, -3, -4 and in 3D -5, -6) gives feq_p - 6 w_p c_px U, the anti-bounce-back outlet (face -2)
+    gives -feq_p + 2 w_p (1 + 4.5 (c_p.u')^2 - 1.5 u'.u'), which equals feq_pb(u').  The face of a
+    link leaving through an edge or corner is its x face, else its y face (x before y before z).
+    Expected values come from this rule applied cell by cell to the positions of the cells."""
+    dim = 2 if Q == 9 else 3
+    N, U, Up = (16, 0.05, 0.03) if dim == 2 else (8, 0.05, 0.03)
+    p = ldc_params(vs=Q, N=N, prec=capi.FP64, bc_preset=preset, bc_velocity=U, ic_u=Up, Re=50.0)
+    m = OracleMesh(p, 256)
+    m.S_SetIC(0)
+    f0 = m.populations().copy()
+    m.S_Advance(0)
+    f1 = m.populations()
+    c, w, pb, _ = O.velset(Q)
+    pos = np.floor(cell_positions(m)).astype(int)[:, : f0.shape[1]]
+    ncell = f0.shape[1]
+    uvec = np.zeros((3, ncell))
+    uvec[0] = Up
+    fe = feq_np(Q, np.ones(ncell), uvec)
+    expect = f0.copy()
+    n_touched = 0
+    for q in range(1, Q):
+        tgt = pos + c[:, q][:, None]
+        face = np.zeros(ncell, dtype=int)
+        for a in reversed(range(dim)):  # x has precedence: assigned last
+            lo, hi = tgt[a] < 0, tgt[a] >= N
+            face[lo] = -(2 * a + 1)
+            face[hi] = -(2 * a + 2)
+        out = face < 0
+        if not out.any():
+            continue
+        n_touched += out.sum()
+        wall = out & (face != -2)
+        outlet = face == -2
+        val = np.empty(ncell)
+        val[wall] = fe[q][wall] - 6.0 * w[q] * c[0, q] * U
+        cu = c[0, q] * Up
+        val[outlet] = -fe[q][outlet] + 2.0 * w[q] * (1 + 4.5 * cu * cu - 1.5 * Up * Up)
+        # slot q holds direction pb(q): the download shows it as row pb(q)
+        expect[pb[q]][out] = val[out]
+    assert n_touched > 0
+    assert np.max(np.abs(f1 - expect)) < 1e-14
+    # the outlet really reproduces the equilibrium of the opposite direction
+    assert np.max(np.abs(f1 - f0)) > 1e-3 * U
