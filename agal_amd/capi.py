@@ -26,7 +26,7 @@ AGAL_ERR_OVERFLOW = 7
 AGAL_ERR_NOT_BUILT = 8
 
 D2Q9, D3Q19, D3Q27 = 9, 19, 27
-FP32, FP64 = 0, 1
+FP32, FP64, FP16, BF16 = 0, 1, 2, 3
 BGK = 0
 BC_NONE, BC_LDC2D, BC_LDC3D, BC_FPSC2D, BC_FPSC3D = 0, 1, 2, 3, 4
 INTERP_INTERFACE, INTERP_ADDED = 0, 1

@@ -131,4 +131,53 @@ __device__ __forceinline__ T bc_hook(int preset, T U, int nbr_id, T f, T u, T v,
     return f;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Population storage.  FP32 / FP64 pools store f itself.  The 16-bit pools store the deviation
+// from the lattice weight, d = f - w_q, rounded to nearest-even, and every kernel computes in
+// FP32 on f = (float)d + w_q.  A value written to a 16-bit pool is therefore always
+// w_q + round16(f - w_q); writing back a decoded value reproduces it exactly.
+// ---------------------------------------------------------------------------------------------
+struct bf16s {
+    uint16_t bits;
+};
+
+__host__ __device__ inline uint16_t bf16_bits_rne(float x)
+{
+    uint32_t u = __builtin_bit_cast(uint32_t, x);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+template <typename T, typename S>
+struct Store {  // S == T: the pool holds f
+    __device__ static T dec(S s, T) { return s; }
+    __device__ static S enc(T f, T) { return f; }
+    static constexpr bool quantized = false;
+};
+
+template <>
+struct Store<float, _Float16> {
+    __device__ static float dec(_Float16 s, float w) { return (float)s + w; }
+    __device__ static _Float16 enc(float f, float w) { return (_Float16)(f - w); }
+    static constexpr bool quantized = true;
+};
+
+template <>
+struct Store<float, bf16s> {
+    __device__ static float dec(bf16s s, float w) { return __builtin_bit_cast(float, (uint32_t)s.bits << 16) + w; }
+    __device__ static bf16s enc(float f, float w) { return bf16s{bf16_bits_rne(f - w)}; }
+    static constexpr bool quantized = true;
+};
+
+// The value a store followed by a load yields.
+template <typename T, typename S>
+__device__ __forceinline__ T quant(T f, T w)
+{
+    if constexpr (Store<T, S>::quantized)
+        return Store<T, S>::dec(Store<T, S>::enc(f, w), w);
+    else
+        return f;
+}
+
 }  // namespace agal

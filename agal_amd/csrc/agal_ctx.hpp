@@ -39,7 +39,9 @@ struct Ctx {
     // Geometry of the pool.
     int Q = 19, DIM = 3, QMAX = 27, CELLS = 64, NCH = 8;
     int precision = AGAL_FP32;
-    size_t tsize = 4;
+    size_t tsize = 4;   // bytes per stored population
+    size_t csize = 4;   // bytes of the compute type (macroscopic fields, transfers)
+    double weight[27] = {};
     long long n_maxcblocks = 0, n_maxcells = 0;
     int Nxi[3] = {0, 0, 0};
     int MAX_LEVELS = 1, MAX_LEVELS_INTERIOR = 1, MAX_LEVELS_WALL = 1, N_LEVEL_START = 0;

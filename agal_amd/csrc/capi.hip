@@ -84,10 +84,14 @@ int sched_done(Ctx *c, hipStream_t s, int slot, unsigned rmask, unsigned wmask)
 }
 
 // Element-wise gather (dir 0) or scatter (dir 1) between a block's current population buffer
-// and a contiguous staging array, Q rows of n_blocks*cells.
-template <typename T>
-__global__ void k_stage_pops(T *F0, T *F1, T *stage, long long ncm, long long n_blocks, int cells, int Q,
-                             const int *level, int4 cur_lo, int4 cur_hi, int dir)
+// and a contiguous staging array of decoded values, Q rows of n_blocks*cells.
+struct Weights {
+    double w[27];
+};
+
+template <typename T, typename S>
+__global__ void k_stage_pops(S *F0, S *F1, T *stage, long long ncm, long long n_blocks, int cells, int Q,
+                             const int *level, int4 cur_lo, int4 cur_hi, int dir, Weights wt)
 {
     long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     long long nc = n_blocks * cells;
@@ -96,12 +100,12 @@ __global__ void k_stage_pops(T *F0, T *F1, T *stage, long long ncm, long long n_
     int L = level[b];
     int cl[8] = {cur_lo.x, cur_lo.y, cur_lo.z, cur_lo.w, cur_hi.x, cur_hi.y, cur_hi.z, cur_hi.w};
     int k = (L >= 0 && L < 8) ? cl[L] : 0;
-    T *F = k ? F1 : F0;
+    S *F = k ? F1 : F0;
     for (int q = 0; q < Q; q++) {
         if (dir == 0)
-            stage[q * nc + i] = F[q * ncm + i];
+            stage[q * nc + i] = Store<T, S>::dec(F[q * ncm + i], (T)wt.w[q]);
         else
-            F[q * ncm + i] = stage[q * nc + i];
+            F[q * ncm + i] = Store<T, S>::enc(stage[q * nc + i], (T)wt.w[q]);
     }
 }
 
@@ -500,7 +504,9 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
 {
     if (!p || !out || block_capacity <= 0) return AGAL_ERR_ARG;
     if (p->velocity_set != 9 && p->velocity_set != 19 && p->velocity_set != 27) return AGAL_ERR_ARG;
-    if (p->precision != AGAL_FP32 && p->precision != AGAL_FP64) return AGAL_ERR_ARG;
+    if (p->precision != AGAL_FP32 && p->precision != AGAL_FP64 && p->precision != AGAL_FP16 &&
+        p->precision != AGAL_BF16)
+        return AGAL_ERR_ARG;
     if (p->collision != AGAL_BGK || p->interp_order != AGAL_INTERP_LINEAR) return AGAL_ERR_NOT_BUILT;
     if (p->Nx <= 0 || p->Nx % 4 != 0) return AGAL_ERR_ARG;
     Ctx *c = new (std::nothrow) Ctx;
@@ -513,7 +519,10 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     c->CELLS = c->DIM == 2 ? 16 : 64;
     c->NCH = c->DIM == 2 ? 4 : 8;
     c->precision = p->precision;
-    c->tsize = p->precision == AGAL_FP64 ? 8 : 4;
+    c->csize = p->precision == AGAL_FP64 ? 8 : 4;
+    c->tsize = (p->precision == AGAL_FP16 || p->precision == AGAL_BF16) ? 2 : c->csize;
+    for (int q = 0; q < c->Q; q++)
+        c->weight[q] = c->Q == 9 ? VelSet<9>::w(q) : (c->Q == 19 ? VelSet<19>::w(q) : VelSet<27>::w(q));
     float Lx = (float)p->L_c;
     float Ly = (float)(p->L_fy * Lx), Lz = (float)(p->L_fz * Lx);
     c->Nxi[0] = p->Nx;
@@ -562,7 +571,7 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
         return st;
     }
     const long long nmax = c->n_maxcblocks, ncm = c->n_maxcells;
-    const size_t ts = c->tsize;
+    const size_t ts = c->tsize, cs = c->csize;
     c->exc_cap = std::min<long long>(ncm, 1LL << 22);
 #define ALLOC(ptr, bytes)                                        \
     if ((st = dalloc(c, &(ptr), (bytes))) != AGAL_OK) {          \
@@ -571,7 +580,7 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     }
     ALLOC(c->F[0], (size_t)c->Q * ncm * ts);
     ALLOC(c->F[1], (size_t)c->Q * ncm * ts);
-    ALLOC(c->M, (size_t)4 * ncm * ts);
+    ALLOC(c->M, (size_t)4 * ncm * cs);
     ALLOC(c->cells_mask, sizeof(int) * ncm);
     ALLOC(c->X, sizeof(float) * c->DIM * nmax);
     ALLOC(c->nbr, sizeof(int) * c->QMAX * nmax);
@@ -593,7 +602,7 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     ALLOC(c->partials, sizeof(int) * ((nmax + 2047) / 2048 + 64));
     ALLOC(c->dscal, sizeof(int) * 64);
     ALLOC(c->exc_idx, sizeof(int) * 3 * c->exc_cap * c->MAX_LEVELS);
-    ALLOC(c->exc_val, ts * c->exc_cap * c->MAX_LEVELS);
+    ALLOC(c->exc_val, cs * c->exc_cap * c->MAX_LEVELS);
     ALLOC(c->exc_count, sizeof(int) * 2 * c->MAX_LEVELS);
 #undef ALLOC
     e = hipHostMalloc((void **)&c->hscal, sizeof(int) * 64, hipHostMallocDefault);
@@ -843,29 +852,47 @@ static int stage_pops(Ctx *c, void *host, long long n_blocks, long long stride, 
     const long long nc = n_blocks * c->CELLS;
     if (stride < nc) return AGAL_ERR_ARG;
     void *stage = nullptr;
-    AGAL_HIP(c, hipMalloc(&stage, (size_t)c->Q * nc * c->tsize));
+    const size_t cs = c->csize;
+    AGAL_HIP(c, hipMalloc(&stage, (size_t)c->Q * nc * cs));
     int cl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int L = 0; L < c->MAX_LEVELS; L++) cl[L] = c->cur[L];
     int4 lo = make_int4(cl[0], cl[1], cl[2], cl[3]), hi = make_int4(cl[4], cl[5], cl[6], cl[7]);
+    Weights wt{};
+    for (int q = 0; q < c->Q; q++) wt.w[q] = c->weight[q];
     int rc = AGAL_OK;
     auto body = [&]() -> int {
         const char *h = static_cast<const char *>(host);
         if (dir == 1)
             for (int q = 0; q < c->Q; q++)
-                AGAL_HIP(c, hipMemcpyAsync((char *)stage + (size_t)q * nc * c->tsize, h + (size_t)q * stride * c->tsize,
-                                           nc * c->tsize, hipMemcpyHostToDevice, c->stream));
-        unsigned grid = (unsigned)((nc + 255) / 256);
-        if (c->precision == AGAL_FP64)
-            hipLaunchKernelGGL(k_stage_pops<double>, dim3(grid), dim3(256), 0, c->stream, (double *)c->F[0], (double *)c->F[1],
-                               (double *)stage, c->n_maxcells, n_blocks, c->CELLS, c->Q, c->level, lo, hi, dir);
-        else
-            hipLaunchKernelGGL(k_stage_pops<float>, dim3(grid), dim3(256), 0, c->stream, (float *)c->F[0], (float *)c->F[1],
-                               (float *)stage, c->n_maxcells, n_blocks, c->CELLS, c->Q, c->level, lo, hi, dir);
+                AGAL_HIP(c, hipMemcpyAsync((char *)stage + (size_t)q * nc * cs, h + (size_t)q * stride * cs, nc * cs,
+                                           hipMemcpyHostToDevice, c->stream));
+        const dim3 grid((unsigned)((nc + 255) / 256)), blk(256);
+        switch (c->precision) {
+            case AGAL_FP64:
+                hipLaunchKernelGGL((k_stage_pops<double, double>), grid, blk, 0, c->stream, (double *)c->F[0],
+                                   (double *)c->F[1], (double *)stage, c->n_maxcells, n_blocks, c->CELLS, c->Q, c->level,
+                                   lo, hi, dir, wt);
+                break;
+            case AGAL_FP16:
+                hipLaunchKernelGGL((k_stage_pops<float, _Float16>), grid, blk, 0, c->stream, (_Float16 *)c->F[0],
+                                   (_Float16 *)c->F[1], (float *)stage, c->n_maxcells, n_blocks, c->CELLS, c->Q,
+                                   c->level, lo, hi, dir, wt);
+                break;
+            case AGAL_BF16:
+                hipLaunchKernelGGL((k_stage_pops<float, bf16s>), grid, blk, 0, c->stream, (bf16s *)c->F[0],
+                                   (bf16s *)c->F[1], (float *)stage, c->n_maxcells, n_blocks, c->CELLS, c->Q, c->level,
+                                   lo, hi, dir, wt);
+                break;
+            default:
+                hipLaunchKernelGGL((k_stage_pops<float, float>), grid, blk, 0, c->stream, (float *)c->F[0],
+                                   (float *)c->F[1], (float *)stage, c->n_maxcells, n_blocks, c->CELLS, c->Q, c->level,
+                                   lo, hi, dir, wt);
+        }
         AGAL_HIP(c, hipGetLastError());
         if (dir == 0)
             for (int q = 0; q < c->Q; q++)
-                AGAL_HIP(c, hipMemcpyAsync((char *)host + (size_t)q * stride * c->tsize, (char *)stage + (size_t)q * nc * c->tsize,
-                                           nc * c->tsize, hipMemcpyDeviceToHost, c->stream));
+                AGAL_HIP(c, hipMemcpyAsync((char *)host + (size_t)q * stride * cs, (char *)stage + (size_t)q * nc * cs,
+                                           nc * cs, hipMemcpyDeviceToHost, c->stream));
         AGAL_HIP(c, hipStreamSynchronize(c->stream));
         return AGAL_OK;
     };
@@ -898,8 +925,8 @@ int agal_download_macros(agal_ctx *ctx, void *dst, long long n_blocks, long long
     const long long nc = n_blocks * c->CELLS;
     if (dst_stride < nc) return AGAL_ERR_ARG;
     for (int k = 0; k < 4; k++)
-        AGAL_HIP(c, hipMemcpyAsync((char *)dst + (size_t)k * dst_stride * c->tsize,
-                                   (char *)c->M + (size_t)k * c->n_maxcells * c->tsize, nc * c->tsize,
+        AGAL_HIP(c, hipMemcpyAsync((char *)dst + (size_t)k * dst_stride * c->csize,
+                                   (char *)c->M + (size_t)k * c->n_maxcells * c->csize, nc * c->csize,
                                    hipMemcpyDeviceToHost, c->stream));
     AGAL_HIP(c, hipStreamSynchronize(c->stream));
     return AGAL_OK;

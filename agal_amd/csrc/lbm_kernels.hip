@@ -31,8 +31,8 @@ constexpr int WG = 256;
 // ---------------------------------------------------------------------------------------------
 // Initial conditions: Cu_SetInitialConditions (solver_lbm_kernels.cu:13-77), uniform state hook.
 // ---------------------------------------------------------------------------------------------
-template <typename T, int Q>
-__global__ __launch_bounds__(WG) void k_set_ic(const int *__restrict__ ids, int n, T *__restrict__ F, long long ncm,
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(WG) void k_set_ic(const int *__restrict__ ids, int n, S *__restrict__ F, long long ncm,
                                                const int *__restrict__ cells_mask, T rho, T u, T v, T w)
 {
     using VS = VelSet<Q>;
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(WG) void k_set_ic(const int *__restrict__ ids, int 
         T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
         T f = (T)(-1.0);
         if (m != CELL_SOLID) f = rho * (T)VS::w(p) * ((T)1.0 + (T)3.0 * cdotu + (T)4.5 * cdotu * cdotu - (T)1.5 * udotu);
-        F[(long long)p * ncm + cell] = f;
+        F[(long long)p * ncm + cell] = Store<T, S>::enc(f, (T)VS::w(p));
     }
 }
 
@@ -73,8 +73,8 @@ __device__ __forceinline__ void moments(const T *f, T &rho, T &u, T &v, T &w)
     if (VS::DIM == 3) w /= rho;
 }
 
-template <typename T, int Q>
-__global__ __launch_bounds__(WG) void k_refresh(const int *__restrict__ ids, int n, const T *__restrict__ F,
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(WG) void k_refresh(const int *__restrict__ ids, int n, const S *__restrict__ F,
                                                 T *__restrict__ M, long long ncm)
 {
     using VS = VelSet<Q>;
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(WG) void k_refresh(const int *__restrict__ ids, int
     const long long cell = (long long)ids[bi] * CELLS + t;
     T f[Q];
 #pragma unroll
-    for (int p = 0; p < Q; p++) f[p] = F[(long long)p * ncm + cell];
+    for (int p = 0; p < Q; p++) f[p] = Store<T, S>::dec(F[(long long)p * ncm + cell], (T)VS::w(p));
     T rho, u, v, w;
     moments<T, Q>(f, rho, u, v, w);
     M[0 * ncm + cell] = rho;
@@ -97,12 +97,12 @@ __global__ __launch_bounds__(WG) void k_refresh(const int *__restrict__ ids, int
 // ---------------------------------------------------------------------------------------------
 // Fused collide + boundary hook + stream of one level sub-step.
 // ---------------------------------------------------------------------------------------------
-template <typename T>
+template <typename T, typename S>
 struct StepArgs {
     const int *ids;
     int n;
-    const T *A;
-    T *B;
+    const S *A;
+    S *B;
     long long ncm;
     const int *nbr_slot;   // per block, Q_max neighbour IDs in 3x3(x3) slot order
     const int *part;       // block collides and streams (not solid-with-children)
@@ -126,10 +126,11 @@ __device__ __forceinline__ int xcd_local_wg(int nwg_padded)
     return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
 }
 
-template <typename T, int Q>
-__global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T> a)
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T, S> a)
 {
     using VS = VelSet<Q>;
+    using ST = Store<T, S>;
     constexpr int CELLS = VS::CELLS, DIM = VS::DIM, QMAX = VS::QMAX;
     const int t = threadIdx.x % CELLS;
     const int lane = threadIdx.x % 64;
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T> a)
         inert = !part || mask == CELL_SOLID;
         onb = a.onb[b] == 1 && mask != CELL_SOLID;
 #pragma unroll
-        for (int q = 0; q < Q; q++) f[q] = a.A[(long long)q * a.ncm + cell];
+        for (int q = 0; q < Q; q++) f[q] = ST::dec(a.A[(long long)q * a.ncm + cell], (T)VS::w(q));
     } else {
 #pragma unroll
         for (int q = 0; q < Q; q++) f[q] = (T)1;
@@ -174,18 +175,20 @@ __global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T> a)
         for (int p = 0; p < Q; p++) fs[p] = f[VS::pb(p)];
     }
 
-    if (active) a.B[cell] = fs[0];
+    if (active) a.B[cell] = ST::enc(fs[0], (T)VS::w(0));
 #pragma unroll
     for (int q = 1; q < Q; q++) {
         const int Ip = I + VS::cx(q), Jp = J + VS::cy(q), Kp = K + VS::cz(q);
-        const int S = (Ip < 0 ? 0 : (Ip >= 4 ? 2 : 1)) + 3 * (Jp < 0 ? 0 : (Jp >= 4 ? 2 : 1)) +
-                      (DIM == 3 ? 9 * (Kp < 0 ? 0 : (Kp >= 4 ? 2 : 1)) : 0);
-        const int nbk = __shfl(my_nb, gbase + S, 64);
+        const int slot = (Ip < 0 ? 0 : (Ip >= 4 ? 2 : 1)) + 3 * (Jp < 0 ? 0 : (Jp >= 4 ? 2 : 1)) +
+                         (DIM == 3 ? 9 * (Kp < 0 ? 0 : (Kp >= 4 ? 2 : 1)) : 0);
+        const int nbk = __shfl(my_nb, gbase + slot, 64);
         if (!active) continue;
         const long long tgt = (long long)nbk * CELLS + ((Ip & 3) + 4 * (Jp & 3) + (DIM == 3 ? 16 * (Kp & 3) : 0));
         // Domain-face hook on links that leave the domain (Cu_ImposeBC, onb blocks, non-solid cells).
         if (onb && nbk < 0 && nbk != N_SKIPID)
             fs[q] = bc_hook<T>(a.bc_preset, a.U, nbk, fs[q], u, v, w, (T)VS::w(q), (T)VS::cx(q), (T)VS::cy(q), (T)VS::cz(q));
+        // In a 16-bit pool the guard below sees the stored value.
+        fs[q] = quant<T, S>(fs[q], (T)VS::w(q));
         bool push;
         bool exc = false;
         if (VS::half(q)) {
@@ -193,7 +196,7 @@ __global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T> a)
             push = !inert && nbk >= 0;
             if (push && !simple) {
                 const bool p_inert = !a.part[nbk] || a.cells_mask[tgt] == CELL_SOLID;
-                if (p_inert) push = a.A[(long long)q * a.ncm + tgt] > (T)0;
+                if (p_inert) push = ST::dec(a.A[(long long)q * a.ncm + tgt], (T)VS::w(q)) > (T)0;
             }
         } else {
             // Partner of the link initiated by the cell x + c_q in direction pb(q).
@@ -205,7 +208,7 @@ __global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T> a)
             }
         }
         if (push) {
-            a.B[(long long)q * a.ncm + tgt] = fs[q];
+            a.B[(long long)q * a.ncm + tgt] = ST::enc(fs[q], (T)VS::w(q));
         } else if (exc) {
             int k = atomicAdd(&a.exc_count[0], 1);
             if (k < a.exc_cap) {
@@ -217,15 +220,15 @@ __global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T> a)
                 a.exc_count[1] = 1;
             }
         } else {
-            a.B[(long long)VS::pb(q) * a.ncm + cell] = fs[q];
+            a.B[(long long)VS::pb(q) * a.ncm + cell] = ST::enc(fs[q], (T)VS::w(q));
         }
     }
 }
 
 // Resolves guard exceptions: the initiator's pushed value returns to it (bounce-back) and the
 // partner keeps its own post-collision value.  One workgroup; clears the counter when done.
-template <typename T, int Q>
-__global__ __launch_bounds__(WG) void k_fixup(T *__restrict__ B, long long ncm, const int *__restrict__ exc_idx,
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(WG) void k_fixup(S *__restrict__ B, long long ncm, const int *__restrict__ exc_idx,
                                               const T *__restrict__ exc_val, int *exc_count, long long exc_cap)
 {
     using VS = VelSet<Q>;
@@ -234,11 +237,16 @@ __global__ __launch_bounds__(WG) void k_fixup(T *__restrict__ B, long long ncm, 
         const long long i = exc_idx[3 * k + 0], c = exc_idx[3 * k + 1];
         const int q = exc_idx[3 * k + 2];
         int qb = 0;
+        T wq = 0;
 #pragma unroll
         for (int p = 0; p < Q; p++)
-            if (p == q) qb = VS::pb(p);
+            if (p == q) {
+                qb = VS::pb(p);
+                wq = (T)VS::w(p);
+            }
+        // Rows q and pb(q) share one weight: a stored value moves between them unchanged.
         B[(long long)q * ncm + i] = B[(long long)qb * ncm + c];
-        B[(long long)qb * ncm + c] = exc_val[k];
+        B[(long long)qb * ncm + c] = Store<T, S>::enc(exc_val[k], wq);
     }
     __syncthreads();
     if (threadIdx.x == 0) exc_count[0] = 0;
@@ -252,9 +260,9 @@ __global__ __launch_bounds__(WG) void k_fixup(T *__restrict__ B, long long ncm, 
 // ---------------------------------------------------------------------------------------------
 constexpr int IWG = 512;
 
-template <typename T, int Q, bool ADDED>
-__global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int n, const T *__restrict__ Fp,
-                                                T *__restrict__ Fc, long long ncm, const int *__restrict__ nbr_child,
+template <typename T, typename S, int Q, bool ADDED>
+__global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int n, const S *__restrict__ Fp,
+                                                S *__restrict__ Fc, long long ncm, const int *__restrict__ nbr_child,
                                                 const int *__restrict__ blk_mask, const int *__restrict__ ref,
                                                 const int *__restrict__ cells_mask, T tau_ratio)
 {
@@ -279,7 +287,7 @@ __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int
         const long long cell = (long long)b * CELLS + t;
         T f[Q];
 #pragma unroll
-        for (int p = 0; p < Q; p++) f[p] = Fp[(long long)p * ncm + cell];
+        for (int p = 0; p < Q; p++) f[p] = Store<T, S>::dec(Fp[(long long)p * ncm + cell], (T)VS::w(p));
         T rho, u, v, w;
         moments<T, Q>(f, rho, u, v, w);
         const T udotu = u * u + v * v + w * w;
@@ -315,7 +323,7 @@ __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int
                   (s[20 + o] - s[4 + o] - s[16 + o] + s[0 + o]) * Iy * Iz +
                   (s[21 + o] + s[1 + o] + s[4 + o] + s[16 + o] - s[5 + o] - s[17 + o] - s[20 + o] - s[0 + o]) * Ix * Iy * Iz;
         }
-        Fc[(long long)p * ncm + ccell] = val;
+        Fc[(long long)p * ncm + ccell] = Store<T, S>::enc(val, (T)VS::w(p));
     }
 }
 
@@ -325,9 +333,9 @@ __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int
 // cell's 2x2(x2) group; the group sum is formed in the reference's order from the NG lanes by
 // shuffles and each lane stores the directions p = lane (mod NG).  No LDS, no barriers.
 // ---------------------------------------------------------------------------------------------
-template <typename T, int Q>
-__global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int n, T *__restrict__ Fp,
-                                                const T *__restrict__ Fc, long long ncm, const int *__restrict__ nbr_child,
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int n, S *__restrict__ Fp,
+                                                const S *__restrict__ Fc, long long ncm, const int *__restrict__ nbr_child,
                                                 const int *__restrict__ blk_mask, const int *__restrict__ cells_mask,
                                                 int ave_type, T tau_ratio)
 {
@@ -365,7 +373,7 @@ __global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int
         const long long ccell = cb + c0 + off;
         T f[Q];
 #pragma unroll
-        for (int p = 0; p < Q; p++) f[p] = Fc[(long long)p * ncm + ccell];
+        for (int p = 0; p < Q; p++) f[p] = Store<T, S>::dec(Fc[(long long)p * ncm + ccell], (T)VS::w(p));
         T rho, u, v, w;
         moments<T, Q>(f, rho, u, v, w);
         const T udotu = (DIM == 3) ? u * u + v * v + w * w : u * u + v * v;
@@ -388,7 +396,7 @@ __global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int
         T acc = __shfl(s[p], gl, 64);
 #pragma unroll
         for (int j = 1; j < NG; j++) acc = acc + __shfl(s[p], gl + j, 64);
-        if (work && (p % NG) == k) Fp[(long long)p * ncm + pcell] = scale * acc;
+        if (work && (p % NG) == k) Fp[(long long)p * ncm + pcell] = Store<T, S>::enc(scale * acc, (T)VS::w(p));
     }
 }
 
@@ -407,39 +415,39 @@ static inline int interp_grid(long long nparents)
     return (int)((nparents + ppw - 1) / ppw);
 }
 
-template <typename T, int Q>
+template <typename T, typename S, int Q>
 static int set_ic_t(Ctx *c, int L)
 {
     int n = c->n_ids[L];
     if (n <= 0) return AGAL_OK;
-    hipLaunchKernelGGL((k_set_ic<T, Q>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, c->id_set_L(L), n,
-                       Fbuf<T>(c, c->cur[L]), c->n_maxcells, c->cells_mask, (T)c->P.ic_rho, (T)c->P.ic_u,
+    hipLaunchKernelGGL((k_set_ic<T, S, Q>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, c->id_set_L(L), n,
+                       Fbuf<S>(c, c->cur[L]), c->n_maxcells, c->cells_mask, (T)c->P.ic_rho, (T)c->P.ic_u,
                        (T)c->P.ic_v, (T)c->P.ic_w);
     AGAL_HIP(c, hipGetLastError());
     return AGAL_OK;
 }
 
-template <typename T, int Q>
+template <typename T, typename S, int Q>
 static int refresh_t(Ctx *c, int L)
 {
     int n = c->n_ids[L];
     if (n <= 0) return AGAL_OK;
-    hipLaunchKernelGGL((k_refresh<T, Q>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, c->id_set_L(L), n,
-                       Fbuf<T>(c, c->cur[L]), static_cast<T *>(c->M), c->n_maxcells);
+    hipLaunchKernelGGL((k_refresh<T, S, Q>), dim3(nwg(n, VelSet<Q>::CELLS)), dim3(WG), 0, c->stream, c->id_set_L(L),
+                       n, Fbuf<S>(c, c->cur[L]), static_cast<T *>(c->M), c->n_maxcells);
     AGAL_HIP(c, hipGetLastError());
     return AGAL_OK;
 }
 
-template <typename T, int Q>
+template <typename T, typename S, int Q>
 static int collide_stream_t(Ctx *c, int L)
 {
     int n = c->n_ids[L];
     if (n <= 0) return AGAL_OK;
-    StepArgs<T> a;
+    StepArgs<T, S> a;
     a.ids = c->id_set_L(L);
     a.n = n;
-    a.A = Fbuf<T>(c, c->cur[L]);
-    a.B = Fbuf<T>(c, c->cur[L] ^ 1);
+    a.A = Fbuf<S>(c, c->cur[L]);
+    a.B = Fbuf<S>(c, c->cur[L] ^ 1);
     a.ncm = c->n_maxcells;
     a.nbr_slot = c->nbr_slot;
     a.part = c->part;
@@ -468,24 +476,24 @@ static int collide_stream_t(Ctx *c, int L)
         hipEvent_t e0 = nullptr, e1 = nullptr;
         AGAL_HIP(c, hipEventCreate(&e0));
         AGAL_HIP(c, hipEventCreate(&e1));
-        hipExtLaunchKernelGGL((k_collide_stream<T, Q>), dim3(grid), dim3(WG), 0, s, e0, e1, 0, a);
+        hipExtLaunchKernelGGL((k_collide_stream<T, S, Q>), dim3(grid), dim3(WG), 0, s, e0, e1, 0, a);
         AGAL_HIP(c, hipGetLastError());
         c->timing.ev_pairs.push_back(e0);
         c->timing.ev_pairs.push_back(e1);
         c->timing.fused_launches++;
         c->timing.fused_cells += (double)n * VelSet<Q>::CELLS;
     } else {
-        hipLaunchKernelGGL((k_collide_stream<T, Q>), dim3(grid), dim3(WG), 0, s, a);
+        hipLaunchKernelGGL((k_collide_stream<T, S, Q>), dim3(grid), dim3(WG), 0, s, a);
         AGAL_HIP(c, hipGetLastError());
     }
-    hipLaunchKernelGGL((k_fixup<T, Q>), dim3(1), dim3(WG), 0, s, a.B, a.ncm, a.exc_idx, a.exc_val, a.exc_count,
+    hipLaunchKernelGGL((k_fixup<T, S, Q>), dim3(1), dim3(WG), 0, s, a.B, a.ncm, a.exc_idx, a.exc_val, a.exc_count,
                        a.exc_cap);
     AGAL_HIP(c, hipGetLastError());
     c->cur[L] ^= 1;
     return sched_done(c, s, slot, rl, wl);
 }
 
-template <typename T, int Q>
+template <typename T, typename S, int Q>
 static int interpolate_t(Ctx *c, int L, int var)
 {
     if (L >= c->MAX_LEVELS - 1) return AGAL_OK;
@@ -503,18 +511,18 @@ static int interpolate_t(Ctx *c, int L, int var)
     const unsigned rl = res_bit(L, c->cur[L]), wl = res_bit(L + 1, c->cur[L + 1]);
     hipStream_t s = sched_begin(c, 0, rl, wl);
     if (var == AGAL_INTERP_ADDED)
-        hipLaunchKernelGGL((k_interp<T, Q, true>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
-                           Fbuf<T>(c, c->cur[L]), Fbuf<T>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
+        hipLaunchKernelGGL((k_interp<T, S, Q, true>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
+                           Fbuf<S>(c, c->cur[L]), Fbuf<S>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
                            c->ref, c->cells_mask, ratio);
     else
-        hipLaunchKernelGGL((k_interp<T, Q, false>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
-                           Fbuf<T>(c, c->cur[L]), Fbuf<T>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
+        hipLaunchKernelGGL((k_interp<T, S, Q, false>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
+                           Fbuf<S>(c, c->cur[L]), Fbuf<S>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
                            c->ref, c->cells_mask, ratio);
     AGAL_HIP(c, hipGetLastError());
     return sched_done(c, s, 0, rl, wl);
 }
 
-template <typename T, int Q>
+template <typename T, typename S, int Q>
 static int average_t(Ctx *c, int L, int var)
 {
     if (L >= c->MAX_LEVELS - 1) return AGAL_OK;
@@ -531,23 +539,26 @@ static int average_t(Ctx *c, int L, int var)
     if (n <= 0) return AGAL_OK;
     const unsigned rl = res_bit(L + 1, c->cur[L + 1]), wl = res_bit(L, c->cur[L]);
     hipStream_t s = sched_begin(c, 0, rl, wl);
-    hipLaunchKernelGGL((k_average<T, Q>), dim3(nwg(n, VelSet<Q>::CELLS * (VelSet<Q>::DIM == 3 ? 8 : 4))), dim3(WG), 0,
-                       s, ids, n, Fbuf<T>(c, c->cur[L]), Fbuf<T>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child,
+    hipLaunchKernelGGL((k_average<T, S, Q>), dim3(nwg(n, VelSet<Q>::CELLS * (VelSet<Q>::DIM == 3 ? 8 : 4))), dim3(WG),
+                       0, s, ids, n, Fbuf<S>(c, c->cur[L]), Fbuf<S>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child,
                        c->blk_mask, c->cells_mask, var, ratio);
     AGAL_HIP(c, hipGetLastError());
     return sched_done(c, s, 0, rl, wl);
 }
 
+#define AGAL_DISPATCH_Q(c, FN, T, S, ...)                                                  \
+    do {                                                                                   \
+        if ((c)->Q == 9) return FN<T, S, 9>(__VA_ARGS__);                                  \
+        if ((c)->Q == 19) return FN<T, S, 19>(__VA_ARGS__);                                \
+        if ((c)->Q == 27) return FN<T, S, 27>(__VA_ARGS__);                                \
+    } while (0)
 #define AGAL_DISPATCH(c, FN, ...)                                                          \
     do {                                                                                   \
-        if ((c)->precision == AGAL_FP64) {                                                 \
-            if ((c)->Q == 9) return FN<double, 9>(__VA_ARGS__);                            \
-            if ((c)->Q == 19) return FN<double, 19>(__VA_ARGS__);                          \
-            if ((c)->Q == 27) return FN<double, 27>(__VA_ARGS__);                          \
-        } else {                                                                           \
-            if ((c)->Q == 9) return FN<float, 9>(__VA_ARGS__);                             \
-            if ((c)->Q == 19) return FN<float, 19>(__VA_ARGS__);                           \
-            if ((c)->Q == 27) return FN<float, 27>(__VA_ARGS__);                           \
+        switch ((c)->precision) {                                                          \
+            case AGAL_FP64: AGAL_DISPATCH_Q(c, FN, double, double, __VA_ARGS__); break;    \
+            case AGAL_FP32: AGAL_DISPATCH_Q(c, FN, float, float, __VA_ARGS__); break;      \
+            case AGAL_FP16: AGAL_DISPATCH_Q(c, FN, float, _Float16, __VA_ARGS__); break;   \
+            case AGAL_BF16: AGAL_DISPATCH_Q(c, FN, float, bf16s, __VA_ARGS__); break;      \
         }                                                                                  \
         return AGAL_ERR_ARG;                                                               \
     } while (0)

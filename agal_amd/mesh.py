@@ -160,6 +160,13 @@ class LbmMesh:
         self._call("download_block_ints", int(which), np_ptr(out, C.c_int), nb, nb)
         return out if rows > 1 else out[0]
 
+    def set_block_ints(self, which, a: np.ndarray):
+        """Overwrite a block table for blocks [0, a.shape[-1]) (M_LoadToGPU for one array)."""
+        a = np.ascontiguousarray(np.atleast_2d(a), dtype=np.int32)
+        nb = a.shape[1]
+        self._call("upload_block_ints", int(which), np_ptr(a, C.c_int), nb, nb)
+        self.refresh_sizes()
+
     def coords(self, n_blocks=None) -> np.ndarray:
         nb = self.n_blocks() if n_blocks is None else int(n_blocks)
         out = np.empty((self.sizes.n_dim, nb), dtype=np.float32)

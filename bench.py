@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--capacity", type=int, default=400000, help="block pool size")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-storage-study", action="store_true",
+                    help="skip the FP16/BF16/FP32/FP64 comparison on one mesh sequence")
     ap.add_argument("--roofline-steps", type=int, default=20)
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -134,6 +136,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(m, p, args.capacity, upd_actual, args.cpu_seconds)
+    del m
+    storage = None
+    if not args.no_storage_study:
+        storage = storage_study(p, args)
 
     line = {
         "metric": METRIC,
@@ -175,12 +181,71 @@ def main():
             "bytes_per_launch_algorithmic": round(alg_bytes / max(1, launches)),
         },
         "cpu_baseline": cpu,
+        "storage16": storage,
     }
     txt = json.dumps(line)
     print(txt, flush=True)
     if args.out:
         with open(args.out, "w") as f:
             f.write(txt + "\n")
+
+
+def storage_study(p, args):
+    """FP16 / BF16 pools of f - w_q against FP32 and FP64 on one mesh sequence.
+
+    An FP64 run from rest leads; FP32, FP16 and BF16 runs follow it, taking its refinement marks
+    at every rebuild, so all four share every mesh.  After the warm-up the velocity of the leaf
+    cells is compared with FP64, then each follower times K coarse steps (graph replays) on the
+    shared mesh, and the fused kernel of each is timed per launch."""
+    import ctypes as C
+
+    from agal_amd import capi
+    from agal_amd.mesh import LbmMesh
+    from agal_amd.twins import advance_twins, velocity_error
+
+    precs = {"fp64": capi.FP64, "fp32": capi.FP32, "fp16": capi.FP16, "bf16": capi.BF16}
+    ctx = {}
+    for k, v in precs.items():
+        q = capi.Params.from_buffer_copy(p)
+        q.precision = v
+        ctx[k] = LbmMesh(q, args.capacity)
+        ctx[k].S_SetIC(0)
+        ctx[k].M_FreezeRefinedCells(0)
+    t0 = time.time()
+    advance_twins(ctx["fp64"], [ctx["fp32"], ctx["fp16"], ctx["bf16"]], 0, args.warmup)
+    warm_s = time.time() - t0
+    errs = {k: velocity_error(ctx["fp64"], ctx[k]) for k in ("fp32", "fp16", "bf16")}
+    s = ctx["fp64"].refresh_sizes()
+    levels = s.max_levels
+    n_ids = [int(s.n_ids[L]) for L in range(levels)]
+    upd = sum(n_ids[L] * 64 * (1 << L) for L in range(levels))
+    out = {"warmup_steps": args.warmup, "warmup_seconds": round(warm_s, 2), "blocks_per_level": n_ids,
+           "cell_updates_per_step": upd, "vel_err_vs_fp64": {}, "mlups": {}, "fused_kernel": {}}
+    bpp = {"fp64": 8, "fp32": 4, "fp16": 2, "bf16": 2}
+    for k in ("fp64", "fp32", "fp16", "bf16"):
+        m = ctx[k]
+        m.sync()
+        _, step_ms, _ = m.advance_loop(args.warmup, args.steps)
+        m.sync()
+        out["mlups"][k] = round(upd * args.steps / (step_ms * 1e-3) / 1e6, 1)
+        m._f["timing_reset"](m.h, 1)
+        for _ in range(max(1, args.roofline_steps // 2)):
+            m.S_Advance(0)
+        m.sync()
+        fms, nl, fc = C.c_double(), C.c_longlong(), C.c_double()
+        m._f["timing_get"](m.h, C.byref(fms), C.byref(nl), C.byref(fc))
+        m._f["timing_reset"](m.h, 0)
+        alg = 2 * 19 * bpp[k] * fc.value
+        out["fused_kernel"][k] = {"avg_launch_ms": round(fms.value / max(1, nl.value), 5),
+                                  "achieved_gbs": round(alg / (fms.value * 1e-3) / 1e9, 1) if fms.value > 0 else None,
+                                  "bytes_per_cell_update": 2 * 19 * bpp[k]}
+        if k != "fp64":
+            out["vel_err_vs_fp64"][k] = {"rel_l2": float(f"{errs[k]['rel_l2_velocity']:.4g}"),
+                                         "max_abs": float(f"{errs[k]['max_abs_velocity']:.4g}")}
+    out["leaf_cells_compared"] = errs["fp16"]["leaf_cells"]
+    out["speedup_fp16_vs_fp32"] = round(out["mlups"]["fp16"] / out["mlups"]["fp32"], 3)
+    out["fp16_within_1e-3"] = out["vel_err_vs_fp64"]["fp16"]["rel_l2"] <= 1e-3
+    return out
 
 
 def cpu_baseline(m, p, capacity, upd_per_step, budget_s):

@@ -46,6 +46,11 @@ extern "C" {
 
 #define AGAL_FP32      0    /* populations stored and computed in FP32                         */
 #define AGAL_FP64      1    /* populations stored and computed in FP64                         */
+/* 16-bit pools (no counterpart in the reference): each population is stored as its deviation
+ * from the lattice weight, f - w_q, rounded to nearest-even, and every kernel computes in FP32.
+ * Population and macroscopic-field transfers use FP32 (decoded values) for these. */
+#define AGAL_FP16      2    /* f - w_q stored as IEEE binary16                                   */
+#define AGAL_BF16      3    /* f - w_q stored as bfloat16                                        */
 
 #define AGAL_BGK       0    /* CollisionOperator::BGK (inc/solver_lbm.h:19-24)                  */
 
@@ -93,7 +98,7 @@ extern "C" {
  * fields here.  agal_params_default() fills the reference's parser defaults. */
 typedef struct agal_params {
     int    velocity_set;          /* AGAL_D2Q9 | AGAL_D3Q19 | AGAL_D3Q27 (VSET, src/Makefile:160)   */
-    int    precision;             /* AGAL_FP32 | AGAL_FP64 (REAL_s, src/main.cu:60)                */
+    int    precision;             /* AGAL_FP32 | AGAL_FP64 (REAL_s, src/main.cu:60) | FP16 | BF16  */
     int    collision;             /* AGAL_BGK (S_COLLISION)                                       */
     int    interp_order;          /* AGAL_INTERP_LINEAR (S_INTERP)                                */
     int    Nx;                    /* root-grid cells along x (Nx)                                 */

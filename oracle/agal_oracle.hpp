@@ -102,6 +102,64 @@ inline VelSet make_velset(int Q)
     return V;
 }
 
+/* 16-bit population pools (no reference counterpart): the pool holds d = f - w_q rounded to
+ * nearest-even in binary16 or bfloat16; a stored value reads back as (float)d + w_q.  The
+ * conversions are written out so the restatement does not depend on compiler support for 16-bit
+ * types; tests compare them with numpy's float16 and torch's bfloat16. */
+inline uint32_t f32_bits(float x)
+{
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    return u;
+}
+inline float bits_f32(uint32_t u)
+{
+    float x;
+    std::memcpy(&x, &u, 4);
+    return x;
+}
+
+inline uint16_t f16_from_f32(float x)
+{
+    const uint32_t u = f32_bits(x);
+    const uint16_t sign = (uint16_t)((u >> 16) & 0x8000u);
+    const uint32_t a = u & 0x7FFFFFFFu;
+    if (a > 0x7F800000u) return (uint16_t)(sign | 0x7E00u);  // NaN
+    if (a >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);  // rounds to infinity (>= 65520)
+    if (a < 0x38800000u) {                                     // below 2^-14: subnormal or zero
+        // |x| * 2^24 is exact in double; nearbyint rounds half to even.
+        const double m = std::nearbyint((double)bits_f32(a) * 16777216.0);
+        return (uint16_t)(sign | (uint16_t)m);                 // m == 1024 is the smallest normal
+    }
+    const uint32_t e = (a >> 23) - 127 + 15, mant = a & 0x7FFFFFu;
+    uint32_t h = (e << 10) | (mant >> 13);
+    const uint32_t rem = mant & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;  // carries into the exponent
+    return (uint16_t)(sign | h);
+}
+
+inline float f16_to_f32(uint16_t h)
+{
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+    if (e == 0) {
+        const float v = (float)((double)m / 16777216.0);
+        return (h & 0x8000u) ? -v : v;
+    }
+    if (e == 31) return bits_f32(sign | 0x7F800000u | (m << 13));
+    return bits_f32(sign | ((e - 15 + 127) << 23) | (m << 13));
+}
+
+inline uint16_t bf16_from_f32(float x)
+{
+    uint32_t u = f32_bits(x);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+inline float bf16_to_f32(uint16_t h) { return bits_f32((uint32_t)h << 16); }
+
 /* Half set of directions that initiate a swap in Cu_Stream / pair in Cu_ImposeBC
  * (solver_lbm_kernels.cu:597,769). */
 inline bool in_half_set(int dim, int p)
@@ -201,6 +259,21 @@ struct Oracle {
     std::vector<int> id_set, n_ids, id_max, gap_set;
     int n_gaps = 0;
     int global_Liter = 0;
+    // 0: populations kept in T.  1 / 2: a binary16 / bfloat16 pool of f - w_q; F then holds the
+    // value each stored population reads back as, and every write into the pool goes through
+    // quant().  The pool is written where the build's kernels write it: after the fused
+    // collide + boundary hook (before the swap, whose guard sees stored values), by the initial
+    // condition, by interpolation and averaging, and by uploads.
+    int store_mode = 0;
+
+    T quant(T f, int dir) const
+    {
+        if (store_mode == 0) return f;
+        const float w = (float)V.w[dir];
+        const float d = (float)f - w;
+        const float r = store_mode == 1 ? f16_to_f32(f16_from_f32(d)) : bf16_to_f32(bf16_from_f32(d));
+        return (T)(r + w);
+    }
 
     // ---------------------------------------------------------------------------------------
     // Mesh<..>::M_Init (src/mesh/mesh_init.cu:10-567) and Solver_LBM::S_Init (solver_lbm_init.cu).
@@ -209,6 +282,8 @@ struct Oracle {
     {
         P = p;
         V = make_velset(P.velocity_set);
+        store_mode = (P.precision == AGAL_FP16) ? 1 : (P.precision == AGAL_BF16 ? 2 : 0);
+        if (store_mode && sizeof(T) != sizeof(float)) return AGAL_ERR_ARG;
         Q = V.Q;
         DIM = V.DIM;
         Q_max = V.Q_max;
@@ -403,7 +478,7 @@ struct Oracle {
                     T f_p = (T)(-1.0);
                     if (valid_mask != CELL_SOLID)
                         f_p = rho * (T)V.w[p] * ((T)1.0 + (T)3.0 * cdotu + (T)4.5 * cdotu * cdotu - (T)1.5 * udotu);
-                    Fp(V.pb[p])[cell] = f_p;
+                    Fp(V.pb[p])[cell] = quant(f_p, p);
                 }
                 Fp(Q + 0)[cell] = rho;
                 Fp(Q + 1)[cell] = u;
@@ -636,9 +711,10 @@ struct Oracle {
                                 int s = 2 * ci + 2 * 4 * cj;
                                 long ccell = (long)(i_kap_bc + c) * M_CBLOCK + t;
                                 if ((var == 0 && cells_mask[ccell] == CELL_GHOST) || (var == 1))
-                                    Fp(V.pb[p])[ccell] = s_F[0 + s] + (s_F[1 + s] - s_F[0 + s]) * Ix +
-                                                         (s_F[4 + s] - s_F[0 + s]) * Iy +
-                                                         (s_F[5 + s] - s_F[4 + s] - s_F[1 + s] + s_F[0 + s]) * Ix * Iy;
+                                    Fp(V.pb[p])[ccell] = quant(s_F[0 + s] + (s_F[1 + s] - s_F[0 + s]) * Ix +
+                                                                   (s_F[4 + s] - s_F[0 + s]) * Iy +
+                                                                   (s_F[5 + s] - s_F[4 + s] - s_F[1 + s] + s_F[0 + s]) * Ix * Iy,
+                                                               p);
                             }
                     } else {
                         for (int ck = 0; ck < 2; ck++)
@@ -648,14 +724,15 @@ struct Oracle {
                                     int s = 2 * ci + 2 * 4 * cj + 2 * 4 * 4 * ck;
                                     long ccell = (long)(i_kap_bc + c) * M_CBLOCK + t;
                                     if ((var == 0 && cells_mask[ccell] == CELL_GHOST) || (var == 1))
-                                        Fp(V.pb[p])[ccell] =
+                                        Fp(V.pb[p])[ccell] = quant(
                                             s_F[0 + s] + (s_F[1 + s] - s_F[0 + s]) * Ix + (s_F[4 + s] - s_F[0 + s]) * Iy +
-                                            (s_F[16 + s] - s_F[0 + s]) * Iz +
-                                            (s_F[5 + s] - s_F[1 + s] - s_F[4 + s] + s_F[0 + s]) * Ix * Iy +
-                                            (s_F[17 + s] - s_F[1 + s] - s_F[16 + s] + s_F[0 + s]) * Ix * Iz +
-                                            (s_F[20 + s] - s_F[4 + s] - s_F[16 + s] + s_F[0 + s]) * Iy * Iz +
-                                            (s_F[21 + s] + s_F[1 + s] + s_F[4 + s] + s_F[16 + s] - s_F[5 + s] - s_F[17 + s] -
-                                             s_F[20 + s] - s_F[0 + s]) * Ix * Iy * Iz;
+                                                (s_F[16 + s] - s_F[0 + s]) * Iz +
+                                                (s_F[5 + s] - s_F[1 + s] - s_F[4 + s] + s_F[0 + s]) * Ix * Iy +
+                                                (s_F[17 + s] - s_F[1 + s] - s_F[16 + s] + s_F[0 + s]) * Ix * Iz +
+                                                (s_F[20 + s] - s_F[4 + s] - s_F[16 + s] + s_F[0 + s]) * Iy * Iz +
+                                                (s_F[21 + s] + s_F[1 + s] + s_F[4 + s] + s_F[16 + s] - s_F[5 + s] -
+                                                 s_F[17 + s] - s_F[20 + s] - s_F[0 + s]) * Ix * Iy * Iz,
+                                            p);
                                 }
                     }
                 }
@@ -713,13 +790,15 @@ struct Oracle {
                                 if (s_mask[c0] == 1 && inq) {
                                     long pcell = (long)b * M_CBLOCK + t;
                                     if (DIM == 2)
-                                        Fp(V.pb[p])[pcell] = (T)(0.25) * (s_Fc[c0 + 0 + 4 * 0] + s_Fc[c0 + 1 + 4 * 0] +
-                                                                          s_Fc[c0 + 0 + 4 * 1] + s_Fc[c0 + 1 + 4 * 1]);
+                                        Fp(V.pb[p])[pcell] = quant((T)(0.25) * (s_Fc[c0 + 0 + 4 * 0] + s_Fc[c0 + 1 + 4 * 0] +
+                                                                                s_Fc[c0 + 0 + 4 * 1] + s_Fc[c0 + 1 + 4 * 1]),
+                                                                   p);
                                     else
-                                        Fp(V.pb[p])[pcell] = (T)(0.125) * (s_Fc[c0 + 0 + 4 * 0 + 16 * 0] + s_Fc[c0 + 1 + 4 * 0 + 16 * 0] +
-                                                                           s_Fc[c0 + 0 + 4 * 1 + 16 * 0] + s_Fc[c0 + 1 + 4 * 1 + 16 * 0] +
-                                                                           s_Fc[c0 + 0 + 4 * 0 + 16 * 1] + s_Fc[c0 + 1 + 4 * 0 + 16 * 1] +
-                                                                           s_Fc[c0 + 0 + 4 * 1 + 16 * 1] + s_Fc[c0 + 1 + 4 * 1 + 16 * 1]);
+                                        Fp(V.pb[p])[pcell] = quant((T)(0.125) * (s_Fc[c0 + 0 + 4 * 0 + 16 * 0] + s_Fc[c0 + 1 + 4 * 0 + 16 * 0] +
+                                                                                 s_Fc[c0 + 0 + 4 * 1 + 16 * 0] + s_Fc[c0 + 1 + 4 * 1 + 16 * 0] +
+                                                                                 s_Fc[c0 + 0 + 4 * 0 + 16 * 1] + s_Fc[c0 + 1 + 4 * 0 + 16 * 1] +
+                                                                                 s_Fc[c0 + 0 + 4 * 1 + 16 * 1] + s_Fc[c0 + 1 + 4 * 1 + 16 * 1]),
+                                                                   p);
                                 }
                             }
                         }
@@ -731,7 +810,28 @@ struct Oracle {
     {
         collide(L);
         impose_bc(L);
+        if (store_mode) store_level(L);
         stream(L);
+    }
+
+    // Writes every population of level L into the 16-bit pool (values already stored read back
+    // unchanged).
+    void store_level(int L)
+    {
+        const int nL = n_ids[L];
+        const int *ids = idset(L);
+#pragma omp parallel for schedule(static)
+        for (int bi = 0; bi < nL; bi++) {
+            int b = ids[bi];
+            if (b < 0) continue;
+            for (int s = 0; s < Q; s++) {
+                T *F = Fp(s);
+                for (int t = 0; t < M_CBLOCK; t++) {
+                    long cell = (long)b * M_CBLOCK + t;
+                    F[cell] = quant(F[cell], V.pb[s]);
+                }
+            }
+        }
     }
 
     // Solver_LBM::S_Advance (solver_lbm_advance.cu:30-174), forces excluded (they do not alter

@@ -185,8 +185,10 @@ int oracle_upload_populations(void *h, const void *src, long long n_blocks, long
         if (n_blocks < 0 || n_blocks > O.n_maxcblocks) return AGAL_ERR_ARG;
         const T *in = static_cast<const T *>(src);
         long long nc = n_blocks * O.M_CBLOCK;
-        for (int q = 0; q < O.Q; q++)
-            std::memcpy(O.Fp(O.V.pb[q]), in + q * src_stride, sizeof(T) * nc);
+        for (int q = 0; q < O.Q; q++) {
+            T *F = O.Fp(O.V.pb[q]);
+            for (long long i = 0; i < nc; i++) F[i] = O.quant(in[q * src_stride + i], q);
+        }
         return AGAL_OK;
     });
 }
@@ -341,6 +343,21 @@ int oracle_cell_mask_rule(int Q, const int *skip, int t)
 }
 
 int oracle_in_half_set(int dim, int p) { return in_half_set(dim, p) ? 1 : 0; }
+
+/* The 16-bit conversions of the restatement: mode 1 binary16, mode 2 bfloat16. */
+int oracle_to16(int mode, const float *in, uint16_t *out, long long n)
+{
+    if (mode != 1 && mode != 2) return AGAL_ERR_ARG;
+    for (long long i = 0; i < n; i++) out[i] = mode == 1 ? f16_from_f32(in[i]) : bf16_from_f32(in[i]);
+    return AGAL_OK;
+}
+
+int oracle_from16(int mode, const uint16_t *in, float *out, long long n)
+{
+    if (mode != 1 && mode != 2) return AGAL_ERR_ARG;
+    for (long long i = 0; i < n; i++) out[i] = mode == 1 ? f16_to_f32(in[i]) : bf16_to_f32(in[i]);
+    return AGAL_OK;
+}
 
 /* Overwrite the stored rho,u,v,w fields (the reference's fields N_Q..N_Q+3). */
 int oracle_upload_macros(void *h, const void *src, long long n_blocks, long long src_stride)

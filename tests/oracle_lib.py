@@ -37,6 +37,8 @@ def oracle_library():
             "cell_mask_rule": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.c_int]),
             "in_half_set": (C.c_int, [C.c_int, C.c_int]),
             "upload_macros": (C.c_int, [C.c_void_p, C.c_void_p, C.c_longlong, C.c_longlong]),
+            "to16": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_longlong]),
+            "from16": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_longlong]),
         }.items():
             f = getattr(_lib, "oracle_" + extra)
             f.restype, f.argtypes = res, args
@@ -151,3 +153,22 @@ def cell_mask_rule(Q: int, skip, t: int) -> int:
 
 def in_half_set(dim: int, p: int) -> bool:
     return bool(oracle_library().oracle_in_half_set(dim, p))
+
+
+def to16(mode: int, x):
+    """The restatement's float32 -> 16-bit conversion (1 binary16, 2 bfloat16) as raw bits."""
+    import numpy as np
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(x.shape, dtype=np.uint16)
+    st = oracle_library().oracle_to16(mode, x.ctypes.data, out.ctypes.data, x.size)
+    assert st == capi.AGAL_OK
+    return out
+
+
+def from16(mode: int, bits):
+    import numpy as np
+    bits = np.ascontiguousarray(bits, dtype=np.uint16)
+    out = np.empty(bits.shape, dtype=np.float32)
+    st = oracle_library().oracle_from16(mode, bits.ctypes.data, out.ctypes.data, bits.size)
+    assert st == capi.AGAL_OK
+    return out
