@@ -565,6 +565,8 @@ __global__ __launch_bounds__(AWG) void k_criterion(int id_max, long long ncm, co
             W = floor_log2_exact<T>(sqrt((wY - vZ) * (wY - vZ) + (uZ - wX) * (uZ - wX) + (vX - uY) * (vX - uY)));
         if (crit == 1)
             W = floor_log2_exact<T>(fabs((T)0.5 * (uX * uX + vY * vY + wZ * wZ + (T)2.0 * (uY * vX + uZ * wX + wY * vZ))));
+        if (crit == 2)  // |grad u| (Frobenius norm); no counterpart in the reference
+            W = floor_log2_exact<T>(sqrt(uX * uX + uY * uY + uZ * uZ + vX * vX + vY * vY + vZ * vZ + wX * wX + wY * wY + wZ * wZ));
         if (cells_mask[(long long)b * CELLS + t] == CELL_GHOST) W = (T)0.0;
     }
     // Block maximum (BlockwiseMaximum, inc/util.h:786-799): fmax ignores NaN operands.

@@ -126,8 +126,132 @@ __device__ __forceinline__ int xcd_local_wg(int nwg_padded)
     return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
 }
 
+// One link of the fused step for the cell `cell`, direction q >= 1, whose neighbour block in that
+// direction is nbk (target cell tgt): boundary hook, swap guard, and the single store of f*_q
+// either into the neighbour (the link streams) or into the cell's own opposite slot (bounce-back).
+template <typename T, typename S, int Q>
+__device__ __forceinline__ void push_link(const StepArgs<T, S> &a, int q, T fq, int nbk, long long tgt, long long cell,
+                                          bool inert, bool simple, bool onb, T u, T v, T w)
+{
+    using VS = VelSet<Q>;
+    using ST = Store<T, S>;
+    // Domain-face hook on links that leave the domain (Cu_ImposeBC, onb blocks, non-solid cells).
+    if (onb && nbk < 0 && nbk != N_SKIPID)
+        fq = bc_hook<T>(a.bc_preset, a.U, nbk, fq, u, v, w, (T)VS::w(q), (T)VS::cx(q), (T)VS::cy(q), (T)VS::cz(q));
+    // In a 16-bit pool the guard below sees the stored value.
+    fq = quant<T, S>(fq, (T)VS::w(q));
+    bool push;
+    bool exc = false;
+    if (VS::half(q)) {
+        // Initiator of the link x -> x + c_q.
+        push = !inert && nbk >= 0;
+        if (!simple && push) {
+            const bool p_inert = !a.part[nbk] || a.cells_mask[tgt] == CELL_SOLID;
+            if (p_inert) push = ST::dec(a.A[(long long)q * a.ncm + tgt], (T)VS::w(q)) > (T)0;
+        }
+    } else {
+        // Partner of the link initiated by the cell x + c_q in direction pb(q).
+        push = nbk >= 0;
+        if (!simple && push) push = a.part[nbk] && a.cells_mask[tgt] != CELL_SOLID;
+        if (push && !(fq > (T)0)) {
+            push = false;
+            exc = !inert;  // a colliding partner cannot tell the initiator in time
+        }
+    }
+    if (exc) {
+        int k = atomicAdd(&a.exc_count[0], 1);
+        if (k < a.exc_cap) {
+            a.exc_idx[3 * (long long)k + 0] = (int)tgt;  // cell ids fit in int for pools < 2^31 cells
+            a.exc_idx[3 * (long long)k + 1] = (int)cell;
+            a.exc_idx[3 * (long long)k + 2] = q;
+            a.exc_val[k] = fq;
+        } else {
+            a.exc_count[1] = 1;
+        }
+        return;
+    }
+    const long long at = push ? (long long)q * a.ncm + tgt : (long long)VS::pb(q) * a.ncm + cell;
+    a.B[at] = ST::enc(fq, (T)VS::w(q));
+}
+
+// 3D: one wavefront per block, so the block, its flags and its 27 neighbour IDs are wave-uniform
+// (scalar registers); each lane picks its link's neighbour among at most eight of them with
+// selects on its own cell position -- no LDS traffic and no per-direction waits.
+template <typename T, typename S, int Q>
+__device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a)
+{
+    using VS = VelSet<Q>;
+    using ST = Store<T, S>;
+    const int t = threadIdx.x & 63;
+    const int bi = __builtin_amdgcn_readfirstlane(xcd_local_wg(gridDim.x) * (WG / 64) + (int)(threadIdx.x >> 6));
+    if (bi >= a.n) return;  // whole wavefronts only; the kernel has no barrier
+    const int b = __builtin_amdgcn_readfirstlane(a.ids[bi]);
+    const long long cell = (long long)b * 64 + t;
+    const int my_nb = t < 27 ? a.nbr_slot[(long long)b * 27 + t] : 0;
+    const int simple_u = __builtin_amdgcn_readfirstlane(a.simple[b]);
+    const int part_u = __builtin_amdgcn_readfirstlane(a.part[b]);
+    const int onb_u = __builtin_amdgcn_readfirstlane(a.onb[b]);
+    T f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = ST::dec(a.A[(long long)q * a.ncm + cell], (T)VS::w(q));
+    const bool simple = simple_u != 0;
+    int mask = CELL_INTERIOR;
+    if (!simple) mask = a.cells_mask[cell];
+    const bool inert = part_u == 0 || mask == CELL_SOLID;
+    const bool onb = onb_u == 1 && mask != CELL_SOLID;
+    int nbs[27];
+#pragma unroll
+    for (int s = 0; s < 27; s++) nbs[s] = __builtin_amdgcn_readlane(my_nb, s);
+
+    T fs[Q];
+    T rho, u, v, w;
+    moments<T, Q>(f, rho, u, v, w);
+    if (!inert) {
+        const T udotu = u * u + v * v + w * w;
+#pragma unroll
+        for (int p = 0; p < Q; p++) {
+            T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+            fs[p] = f[p] * a.omegp + ((T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu)) * a.omeg;
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < Q; p++) fs[p] = f[VS::pb(p)];
+    }
+
+    a.B[cell] = ST::enc(fs[0], (T)VS::w(0));
+    const int I = t & 3, J = (t >> 2) & 3, K = t >> 4;
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+        const int cx = VS::cx(q), cy = VS::cy(q), cz = VS::cz(q);
+        // Does the link leave the block along x / y / z?
+        const bool ex = cx > 0 ? I == 3 : (cx < 0 ? I == 0 : false);
+        const bool ey = cy > 0 ? J == 3 : (cy < 0 ? J == 0 : false);
+        const bool ez = cz > 0 ? K == 3 : (cz < 0 ? K == 0 : false);
+        // Slot (dx+1) + 3(dy+1) + 9(dz+1) of the neighbour block; 13 is the block itself.
+        const int sxyz = nbs[13 + cx + 3 * cy + 9 * cz], sxy = nbs[13 + cx + 3 * cy], sxz = nbs[13 + cx + 9 * cz],
+                  syz = nbs[13 + 3 * cy + 9 * cz], sx = nbs[13 + cx], sy = nbs[13 + 3 * cy], sz = nbs[13 + 9 * cz];
+        const int nbk = ex ? (ey ? (ez ? sxyz : sxy) : (ez ? sxz : sx)) : (ey ? (ez ? syz : sy) : (ez ? sz : b));
+        const long long tgt = (long long)nbk * 64 + (((I + cx) & 3) + 4 * ((J + cy) & 3) + 16 * ((K + cz) & 3));
+        push_link<T, S, Q>(a, q, fs[q], nbk, tgt, cell, inert, simple, onb, u, v, w);
+    }
+}
+
+template <typename T, typename S, int Q>
+__device__ __forceinline__ void collide_stream_2d(const StepArgs<T, S> &a);
+
 template <typename T, typename S, int Q>
 __global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T, S> a)
+{
+    using VS = VelSet<Q>;
+    if constexpr (VS::DIM == 3)
+        collide_stream_3d<T, S, Q>(a);
+    else
+        collide_stream_2d<T, S, Q>(a);
+}
+
+// 2D: four 4x4 blocks per wavefront; neighbour IDs are distributed with ds_bpermute.
+template <typename T, typename S, int Q>
+__device__ __forceinline__ void collide_stream_2d(const StepArgs<T, S> &a)
 {
     using VS = VelSet<Q>;
     using ST = Store<T, S>;
@@ -184,44 +308,7 @@ __global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T, S> a)
         const int nbk = __shfl(my_nb, gbase + slot, 64);
         if (!active) continue;
         const long long tgt = (long long)nbk * CELLS + ((Ip & 3) + 4 * (Jp & 3) + (DIM == 3 ? 16 * (Kp & 3) : 0));
-        // Domain-face hook on links that leave the domain (Cu_ImposeBC, onb blocks, non-solid cells).
-        if (onb && nbk < 0 && nbk != N_SKIPID)
-            fs[q] = bc_hook<T>(a.bc_preset, a.U, nbk, fs[q], u, v, w, (T)VS::w(q), (T)VS::cx(q), (T)VS::cy(q), (T)VS::cz(q));
-        // In a 16-bit pool the guard below sees the stored value.
-        fs[q] = quant<T, S>(fs[q], (T)VS::w(q));
-        bool push;
-        bool exc = false;
-        if (VS::half(q)) {
-            // Initiator of the link x -> x + c_q.
-            push = !inert && nbk >= 0;
-            if (push && !simple) {
-                const bool p_inert = !a.part[nbk] || a.cells_mask[tgt] == CELL_SOLID;
-                if (p_inert) push = ST::dec(a.A[(long long)q * a.ncm + tgt], (T)VS::w(q)) > (T)0;
-            }
-        } else {
-            // Partner of the link initiated by the cell x + c_q in direction pb(q).
-            push = nbk >= 0;
-            if (push && !simple) push = a.part[nbk] && a.cells_mask[tgt] != CELL_SOLID;
-            if (push && !(fs[q] > (T)0)) {
-                push = false;
-                exc = !inert;  // a colliding partner cannot tell the initiator in time
-            }
-        }
-        if (push) {
-            a.B[(long long)q * a.ncm + tgt] = ST::enc(fs[q], (T)VS::w(q));
-        } else if (exc) {
-            int k = atomicAdd(&a.exc_count[0], 1);
-            if (k < a.exc_cap) {
-                a.exc_idx[3 * (long long)k + 0] = (int)tgt;  // cell ids fit in int for pools < 2^31 cells
-                a.exc_idx[3 * (long long)k + 1] = (int)cell;
-                a.exc_idx[3 * (long long)k + 2] = q;
-                a.exc_val[k] = fs[q];
-            } else {
-                a.exc_count[1] = 1;
-            }
-        } else {
-            a.B[(long long)VS::pb(q) * a.ncm + cell] = ST::enc(fs[q], (T)VS::w(q));
-        }
+        push_link<T, S, Q>(a, q, fs[q], nbk, tgt, cell, inert, simple, onb, u, v, w);
     }
 }
 

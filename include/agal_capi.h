@@ -106,7 +106,8 @@ typedef struct agal_params {
     int    PERIODIC_X, PERIODIC_Y, PERIODIC_Z;
     int    MAX_LEVELS_WALL, MAX_LEVELS_INTERIOR, N_LEVEL_START;
     double v0;                    /* kinematic viscosity in the reference's units (tau = 3 v0 + dx/2) */
-    int    S_CRITERION;           /* 0: |vorticity|, 1: Q-like invariant                          */
+    int    S_CRITERION;           /* 0: |vorticity|, 1: Q-like invariant (solver_lbm_criterion.cu:
+                                     544-547); 2: |grad u| Frobenius norm (no reference counterpart) */
     double N_REFINE_START, N_REFINE_INC, N_REFINE_MAX;
     int    P_REFINE;              /* coarse steps between solution-based rebuilds                 */
     int    bc_preset;             /* AGAL_BC_* : the Cu_ImposeBC hook                             */

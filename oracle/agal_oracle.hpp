@@ -944,6 +944,10 @@ struct Oracle {
                     W = floor_log2_exact<T>(std::sqrt((wY - vZ) * (wY - vZ) + (uZ - wX) * (uZ - wX) + (vX - uY) * (vX - uY)));
                 if (P.S_CRITERION == 1)
                     W = floor_log2_exact<T>(std::fabs((T)0.5 * (uX * uX + vY * vY + wZ * wZ + (T)2.0 * (uY * vX + uZ * wX + wY * vZ))));
+                // S_CRITERION 2: |grad u|, Frobenius norm (no reference counterpart; the build defines it).
+                if (P.S_CRITERION == 2)
+                    W = floor_log2_exact<T>(std::sqrt(uX * uX + uY * uY + uZ * uZ + vX * vX + vY * vY + vZ * vZ + wX * wX +
+                                                      wY * wY + wZ * wZ));
                 if (cells_mask[(long)b * M_CBLOCK + t] == CELL_GHOST) W = (T)0.0;
                 // fmax semantics: a NaN operand is ignored.
                 if (first) { Wmax = W; first = false; }

@@ -82,6 +82,21 @@ def test_solution_adaptive_three_levels(vs):
     assert g.sizes.n_ids[2] > 0
 
 
+@pytest.mark.parametrize("crit", [1, 2])
+def test_solution_adaptive_other_criteria(crit):
+    """The Q-like invariant and the |grad u| criterion drive the same rebuild bit for bit."""
+    p = ldc_params(vs=capi.D3Q19, N=16, Re=40.0, levels=3, P_REFINE=15, S_CRITERION=crit,
+                   N_REFINE_START=-4.0 if crit == 1 else -2.0)
+    g, o = pair(p, 8192)
+    both((g, o), lambda m: m.S_SetIC(0))
+    for k in range(3):
+        rg = g.advance_loop(15 * k, 15)
+        ro = o.advance_loop(15 * k, 15)
+        assert rg[0] == ro[0]
+        assert_same(g, o)
+    assert g.sizes.n_ids[1] > 0
+
+
 def test_d2q9_solution_adaptive():
     p = ldc_params(vs=capi.D2Q9, N=32, Re=100.0, levels=3, P_REFINE=25)
     g, o = pair(p, 2048)

@@ -301,6 +301,26 @@ def test_criterion_cubic_field_extrapolated_halo():
     assert (ref == capi.REF_MARK_REFINE).any() and (ref == capi.REF_UNREFINED).any()
 
 
+@pytest.mark.parametrize("A", [((0.3, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)),
+                               ((0.01, -0.025, 0.005), (0.035, 0.0, 0.015), (-0.02, 0.01, -0.01)),
+                               ((0.0, 0.15, 0.0), (-0.15, 0.0, 0.0), (0.0, 0.0, 0.0))])
+def test_criterion_gradient_norm_of_linear_fields(A):
+    """S_CRITERION 2 on u = A x: central differences and the halo extrapolation are exact for
+    linear fields, so every cell sees |grad u| = ||A||_F; the vorticity criterion on the same field
+    sees |curl| instead (a pure strain has none), which tells the two apart."""
+    A = np.array(A)
+    field = lambda x, y, z: tuple(A[i, 0] * x + A[i, 1] * y + A[i, 2] * z for i in range(3))
+    m, _ = criterion_case(field, crit=2)
+    ref = m.block_ints(capi.BLK_REF)
+    W = np.floor(np.log2(np.sqrt(np.sum(A ** 2))))
+    assert np.all(ref == expected_marks(np.full(ref.size * 64, W), np.zeros(ref.size)))
+    m0, _ = criterion_case(field, crit=0)
+    curl = np.array([A[2, 1] - A[1, 2], A[0, 2] - A[2, 0], A[1, 0] - A[0, 1]])
+    nc = np.sqrt(np.sum(curl ** 2))
+    W0 = np.floor(np.log2(nc)) if nc > 0 else -np.inf
+    assert np.all(m0.block_ints(capi.BLK_REF) == expected_marks(np.full(ref.size * 64, W0), np.zeros(ref.size)))
+
+
 def test_criterion_q_invariant():
     a, b = 0.5, 0.25
     m, _ = criterion_case(lambda x, y, z: (a * y, b * x, 0 * x), crit=1)
