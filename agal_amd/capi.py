@@ -29,7 +29,8 @@ D2Q9, D3Q19, D3Q27 = 9, 19, 27
 FP32, FP64, FP16, BF16 = 0, 1, 2, 3
 BGK = 0
 BC_NONE, BC_LDC2D, BC_LDC3D, BC_FPSC2D, BC_FPSC3D = 0, 1, 2, 3, 4
-INTERP_INTERFACE, INTERP_ADDED = 0, 1
+INTERP_LINEAR, INTERP_CUBIC = 0, 1          # interpolation order (agal_params.interp_order)
+INTERP_INTERFACE, INTERP_ADDED = 0, 1       # interpolation mode (agal_interpolate var)
 AVERAGE_INTERFACE, AVERAGE_BLOCK, AVERAGE_GRID = 0, 1, 2
 REF_UNREFINED, REF_REFINED, REF_REFINED_WCHILD, REF_REFINED_PERM = 0, 1, 2, 3
 REF_MARK_REFINE, REF_MARK_COARSEN, REF_NEW, REF_REMOVE, REF_INACTIVE = 4, 5, 6, 7, 8

@@ -281,6 +281,13 @@ int build_root(Ctx *c)
     return AGAL_OK;
 }
 
+// V_AVERAGE_ADVANCE (solver_lbm_init.cu:94-97): INTERFACE after linear interpolation, BLOCK
+// after any other order.
+static int average_advance(const Ctx *c)
+{
+    return c->P.interp_order == AGAL_INTERP_CUBIC ? AGAL_AVERAGE_BLOCK : AGAL_AVERAGE_INTERFACE;
+}
+
 int advance_rec(Ctx *c, int L)
 {
     int st;
@@ -291,7 +298,7 @@ int advance_rec(Ctx *c, int L)
             if ((st = launch_interpolate(c, L, AGAL_INTERP_INTERFACE))) return st;
             if ((st = launch_collide_stream(c, L))) return st;
             if ((st = advance_rec(c, L + 1))) return st;
-            if ((st = launch_average(c, L, AGAL_AVERAGE_INTERFACE))) return st;
+            if ((st = launch_average(c, L, average_advance(c)))) return st;
         } else {
             if (L < ML - 1)
                 if ((st = launch_interpolate(c, L, AGAL_INTERP_INTERFACE))) return st;
@@ -299,14 +306,14 @@ int advance_rec(Ctx *c, int L)
             if (L == ML - 1) c->global_Liter++;
             if (L < ML - 1) {
                 if ((st = advance_rec(c, L + 1))) return st;
-                if ((st = launch_average(c, L, AGAL_AVERAGE_INTERFACE))) return st;
+                if ((st = launch_average(c, L, average_advance(c)))) return st;
                 if ((st = launch_interpolate(c, L, AGAL_INTERP_INTERFACE))) return st;
             }
             if ((st = launch_collide_stream(c, L))) return st;
             if (L == ML - 1) c->global_Liter++;
             if (L < ML - 1) {
                 if ((st = advance_rec(c, L + 1))) return st;
-                if ((st = launch_average(c, L, AGAL_AVERAGE_INTERFACE))) return st;
+                if ((st = launch_average(c, L, average_advance(c)))) return st;
             }
         }
     } else {
@@ -507,7 +514,8 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     if (p->precision != AGAL_FP32 && p->precision != AGAL_FP64 && p->precision != AGAL_FP16 &&
         p->precision != AGAL_BF16)
         return AGAL_ERR_ARG;
-    if (p->collision != AGAL_BGK || p->interp_order != AGAL_INTERP_LINEAR) return AGAL_ERR_NOT_BUILT;
+    if (p->collision != AGAL_BGK) return AGAL_ERR_NOT_BUILT;
+    if (p->interp_order != AGAL_INTERP_LINEAR && p->interp_order != AGAL_INTERP_CUBIC) return AGAL_ERR_ARG;
     if (p->Nx <= 0 || p->Nx % 4 != 0) return AGAL_ERR_ARG;
     Ctx *c = new (std::nothrow) Ctx;
     if (!c) return AGAL_ERR_ALLOC;

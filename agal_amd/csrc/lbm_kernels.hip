@@ -415,6 +415,102 @@ __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int
 }
 
 // ---------------------------------------------------------------------------------------------
+// Coarse -> fine, cubic: Cu_Interpolate_Cubic (solver_lbm_kernels_comm.cu:406-638).  Per parent
+// and direction the 4^d monomial coefficients alpha_i = sum_q C[i][q] s[q] of the tensor-product
+// Lagrange interpolant through the parent's cells are formed once (the reference's coefficient
+// tables, summed term by term in its order), spread over all lanes of the parent's group, and
+// kept in LDS; each child cell then evaluates the polynomial by Horner's rule at its own centre.
+// Same workgroup shape as the linear kernel: one lane per child cell.
+// ---------------------------------------------------------------------------------------------
+#define AGAL_CUBIC_TABLE __constant__ static const
+#include "cubic_table.inc"
+#undef AGAL_CUBIC_TABLE
+
+template <typename T, typename S, int Q, bool ADDED>
+__global__ __launch_bounds__(IWG) void k_interp_cubic(const int *__restrict__ ids, int n, const S *__restrict__ Fp,
+                                                      S *__restrict__ Fc, long long ncm, const int *__restrict__ nbr_child,
+                                                      const int *__restrict__ blk_mask, const int *__restrict__ ref,
+                                                      const int *__restrict__ cells_mask, T tau_ratio)
+{
+    using VS = VelSet<Q>;
+    constexpr int CELLS = VS::CELLS, DIM = VS::DIM, NCH = VS::CHILDREN;
+    constexpr int GROUP = NCH * CELLS;
+    constexpr int PPW = IWG / GROUP;
+    __shared__ T s_all[Q * PPW * CELLS];  // rescaled parent populations, [p][parent][cell]
+    __shared__ T a_all[Q * PPW * CELLS];  // monomial coefficients, [p][parent][i]
+    const int g = threadIdx.x / GROUP, r = threadIdx.x % GROUP;
+    const int c = r / CELLS, t = r % CELLS;
+    const int bi = blockIdx.x * PPW + g;
+    int b = -1;
+    if (bi < n) b = ids[bi];
+    bool work = false;
+    int fc = 0;
+    if (b >= 0) {
+        fc = nbr_child[b];
+        work = fc >= 0 && (ADDED ? (ref[b] == REF_MARK_REFINE) : (blk_mask[b] == BLK_INTERFACE));
+    }
+    if (work && c == 0) {
+        const long long cell = (long long)b * CELLS + t;
+        T f[Q];
+#pragma unroll
+        for (int p = 0; p < Q; p++) f[p] = Store<T, S>::dec(Fp[(long long)p * ncm + cell], (T)VS::w(p));
+        T rho, u, v, w;
+        moments<T, Q>(f, rho, u, v, w);
+        const T udotu = u * u + v * v + w * w;
+#pragma unroll
+        for (int p = 0; p < Q; p++) {
+            T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+            T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
+            s_all[(p * PPW + g) * CELLS + t] = fe + (f[p] - fe) * tau_ratio;
+        }
+    }
+    __syncthreads();
+    if (work) {
+        const int *start = (DIM == 3) ? CUBIC3_START : CUBIC2_START;
+        const unsigned char *qs = (DIM == 3) ? CUBIC3_Q : CUBIC2_Q;
+        const double *cs = (DIM == 3) ? CUBIC3_C : CUBIC2_C;
+        for (int k = r; k < Q * CELLS; k += GROUP) {
+            const int p = k / CELLS, i = k % CELLS;
+            const T *s = s_all + (p * PPW + g) * CELLS;
+            T acc = (T)cs[start[i]] * s[qs[start[i]]];
+            for (int m = start[i] + 1; m < start[i + 1]; m++) acc = acc + (T)cs[m] * s[qs[m]];
+            a_all[(p * PPW + g) * CELLS + i] = acc;
+        }
+    }
+    __syncthreads();
+    if (!work) return;
+    const long long ccell = (long long)(fc + c) * CELLS + t;
+    if (!ADDED && cells_mask[ccell] != CELL_GHOST) return;
+    const T M_ONE_TWELFTH = (T)(-0.083333333333333), ONE_SIXTH = (T)(0.166666666666667);
+    const T TWO_THIRDS = (T)(0.666666666666667), ZERO_THIRDS = (T)(0.0);
+    const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
+    const T x = M_ONE_TWELFTH + ONE_SIXTH * (T)I;
+    const T y = M_ONE_TWELFTH + ONE_SIXTH * (T)J;
+    const T z = (DIM == 3) ? M_ONE_TWELFTH + ONE_SIXTH * (T)K : (T)0;
+    const T ox = (c & 1) ? TWO_THIRDS : ZERO_THIRDS, oy = (c & 2) ? TWO_THIRDS : ZERO_THIRDS;
+    const T oz = (c & 4) ? TWO_THIRDS : ZERO_THIRDS;
+#pragma unroll 1
+    for (int p = 0; p < Q; p++) {
+        const T *al = a_all + (p * PPW + g) * CELLS;
+        T res = 0;
+#pragma unroll
+        for (int mk = 0; mk < (DIM == 3 ? 4 : 1); mk++) {
+            T gam = 0;
+#pragma unroll
+            for (int mj = 0; mj < 4; mj++) {
+                T beta = 0;
+#pragma unroll
+                for (int mi = 0; mi < 4; mi++)
+                    beta = al[(3 - mi) + 4 * (3 - mj) + (DIM == 3 ? 16 * (3 - mk) : 0)] + (x + ox) * beta;
+                gam = beta + (y + oy) * gam;
+            }
+            res = (DIM == 3) ? gam + (z + oz) * res : gam;
+        }
+        Fc[(long long)p * ncm + ccell] = Store<T, S>::enc(res, (T)VS::w(p));
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Fine -> coarse: Cu_Average (solver_lbm_kernels_comm.cu:174-347).  NG consecutive lanes (NG = 8
 // in 3D, 4 in 2D) serve one parent cell, each lane loading and rescaling one child cell of the
 // cell's 2x2(x2) group; the group sum is formed in the reference's order from the NG lanes by
@@ -597,7 +693,16 @@ static int interpolate_t(Ctx *c, int L, int var)
     if (n <= 0) return AGAL_OK;
     const unsigned rl = res_bit(L, c->cur[L]), wl = res_bit(L + 1, c->cur[L + 1]);
     hipStream_t s = sched_begin(c, 0, rl, wl);
-    if (var == AGAL_INTERP_ADDED)
+    if (c->P.interp_order == AGAL_INTERP_CUBIC) {
+        if (var == AGAL_INTERP_ADDED)
+            hipLaunchKernelGGL((k_interp_cubic<T, S, Q, true>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
+                               Fbuf<S>(c, c->cur[L]), Fbuf<S>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child,
+                               c->blk_mask, c->ref, c->cells_mask, ratio);
+        else
+            hipLaunchKernelGGL((k_interp_cubic<T, S, Q, false>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
+                               Fbuf<S>(c, c->cur[L]), Fbuf<S>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child,
+                               c->blk_mask, c->ref, c->cells_mask, ratio);
+    } else if (var == AGAL_INTERP_ADDED)
         hipLaunchKernelGGL((k_interp<T, S, Q, true>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
                            Fbuf<S>(c, c->cur[L]), Fbuf<S>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
                            c->ref, c->cells_mask, ratio);
@@ -616,7 +721,7 @@ static int average_t(Ctx *c, int L, int var)
     const T ratio = (T)c->tau_vec[L] / (T)c->tau_vec[L + 1];
     const int *ids;
     int n;
-    if (var == AGAL_AVERAGE_INTERFACE) {
+    if (var == AGAL_AVERAGE_INTERFACE || var == AGAL_AVERAGE_BLOCK) {  // both act on INTERFACE blocks only
         ids = c->iface_L(L);
         n = c->n_iface[L];
     } else {

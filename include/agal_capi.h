@@ -55,6 +55,8 @@ extern "C" {
 #define AGAL_BGK       0    /* CollisionOperator::BGK (inc/solver_lbm.h:19-24)                  */
 
 #define AGAL_INTERP_LINEAR 0 /* InterpOrder::Linear (inc/solver_lbm.h:25-30)                    */
+#define AGAL_INTERP_CUBIC  1 /* InterpOrder::Cubic: Cu_Interpolate_Cubic (solver_lbm_kernels_comm.cu:
+                                406-638); the advance then averages in BLOCK mode (solver_lbm_init.cu:94-97) */
 
 /* Boundary-condition presets of the reference's device hook Cu_ImposeBC (inc/custom.h:66-123). */
 #define AGAL_BC_NONE    0   /* hook returns f unchanged on every face: plain bounce-back        */
@@ -100,7 +102,7 @@ typedef struct agal_params {
     int    velocity_set;          /* AGAL_D2Q9 | AGAL_D3Q19 | AGAL_D3Q27 (VSET, src/Makefile:160)   */
     int    precision;             /* AGAL_FP32 | AGAL_FP64 (REAL_s, src/main.cu:60) | FP16 | BF16  */
     int    collision;             /* AGAL_BGK (S_COLLISION)                                       */
-    int    interp_order;          /* AGAL_INTERP_LINEAR (S_INTERP)                                */
+    int    interp_order;          /* AGAL_INTERP_LINEAR | AGAL_INTERP_CUBIC (S_INTERP)            */
     int    Nx;                    /* root-grid cells along x (Nx)                                 */
     double L_c, L_fy, L_fz;       /* domain length and aspect ratios                              */
     int    PERIODIC_X, PERIODIC_Y, PERIODIC_Z;

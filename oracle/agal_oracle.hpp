@@ -38,6 +38,12 @@
 
 namespace agal_oracle {
 
+// Coefficient tables of the cubic interpolation (the reference's Cu_Interp2D/3D terms, pinned
+// against its text by tests/test_oracle_tables.py).
+#define AGAL_CUBIC_TABLE static const
+#include "../agal_amd/csrc/cubic_table.inc"
+#undef AGAL_CUBIC_TABLE
+
 constexpr int N_SKIPID = -999;
 
 constexpr int REF_UNREFINED = 0, REF_UNREFINED_VIO = 9, REF_REFINED = 1, REF_REFINED_WCHILD = 2,
@@ -660,10 +666,104 @@ struct Oracle {
         }
     }
 
+    // Solver_LBM::S_Interpolate (inc/solver_lbm.h:186-200): linear or cubic by InterpOrder.
+    void interpolate(int L, int var)
+    {
+        if (P.interp_order == AGAL_INTERP_CUBIC)
+            interpolate_cubic(L, var);
+        else
+            interpolate_linear(L, var);
+    }
+
+    // Averaging mode of the advance (solver_lbm_init.cu:94-97): INTERFACE after linear
+    // interpolation, BLOCK after any other order.
+    int average_advance() const { return P.interp_order == AGAL_INTERP_CUBIC ? AVERAGE_BLOCK : AVERAGE_INTERFACE; }
+
+    // ---------------------------------------------------------------------------------------
+    // Cu_Interpolate_Cubic (solver_lbm_kernels_comm.cu:406-638) + S_Interpolate_Cubic (:643-659).
+    // alpha_idx = sum of the terms C * s_F[q] that Cu_Interp2D/3D (inc/util_interp.h) list for case
+    // idx, left to right: the monomial coefficients of the tensor-product Lagrange interpolant
+    // through the parent's cell centres at nodes {0, 1/3, 2/3, 1} per axis (plus, in 3D, the
+    // reference's ~1e-13 remainders of a numerical inversion, which its sums include).  Children
+    // are evaluated by Horner's rule at x = -1/12 + I/6 (+ 2/3 for the upper child on the axis).
+    // ---------------------------------------------------------------------------------------
+    T cubic_alpha(int idx, const T *s) const
+    {
+        const int *start = (DIM == 3) ? CUBIC3_START : CUBIC2_START;
+        const unsigned char *qs = (DIM == 3) ? CUBIC3_Q : CUBIC2_Q;
+        const double *cs = (DIM == 3) ? CUBIC3_C : CUBIC2_C;
+        T acc = (T)cs[start[idx]] * s[qs[start[idx]]];
+        for (int k = start[idx] + 1; k < start[idx + 1]; k++) acc = acc + (T)cs[k] * s[qs[k]];
+        return acc;
+    }
+
+    void interpolate_cubic(int L, int var)
+    {
+        if (L >= MAX_LEVELS - 1) return;
+        refresh(L);
+        if (n_ids[L] <= 0) return;
+        const T tau_ratio = tau_vec[L + 1] / tau_vec[L];
+        const T M_ONE_TWELFTH = (T)(-0.083333333333333), ONE_SIXTH = (T)(0.166666666666667);
+        const T TWO_THIRDS = (T)(0.666666666666667), ZERO_THIRDS = (T)(0.0);
+        const int NCH = N_CHILDREN;
+        const int nL = n_ids[L];
+        const int *ids = idset(L);
+#pragma omp parallel for schedule(static)
+        for (int bi = 0; bi < nL; bi++) {
+            int b = ids[bi];
+            if (b < 0) continue;
+            int i_kap_bc = nbr_child[b];
+            bool b1 = (var == INTERP_INTERFACE) && (blk_mask[b] == BLK_INTERFACE);
+            bool b2 = (var == INTERP_ADDED) && (ref[b] == REF_MARK_REFINE);
+            if (!(b1 || b2) || i_kap_bc < 0) continue;
+            T s_F[64], alpha[64];
+            for (int p = 0; p < Q; p++) {
+                for (int t = 0; t < M_CBLOCK; t++) {
+                    long cell = (long)b * M_CBLOCK + t;
+                    T rho = Fp(Q + 0)[cell], u = Fp(Q + 1)[cell], v = Fp(Q + 2)[cell];
+                    T w = (DIM == 3) ? Fp(Q + 3)[cell] : (T)0;
+                    T udotu = u * u + v * v + w * w;
+                    T f_p = Fp(V.pb[p])[cell];
+                    T fe = feq(p, rho, u, v, w, udotu);
+                    s_F[t] = fe + (f_p - fe) * tau_ratio;
+                }
+                for (int idx = 0; idx < M_CBLOCK; idx++) alpha[idx] = cubic_alpha(idx, s_F);
+                for (int t = 0; t < M_CBLOCK; t++) {
+                    int I, J, K;
+                    cell_ijk(t, I, J, K);
+                    const T x = M_ONE_TWELFTH + ONE_SIXTH * (T)I;
+                    const T y = M_ONE_TWELFTH + ONE_SIXTH * (T)J;
+                    const T z = (DIM == 3) ? M_ONE_TWELFTH + ONE_SIXTH * (T)K : (T)0;
+                    for (int c = 0; c < NCH; c++) {
+                        const T ox = (c & 1) ? TWO_THIRDS : ZERO_THIRDS, oy = (c & 2) ? TWO_THIRDS : ZERO_THIRDS;
+                        const T oz = (c & 4) ? TWO_THIRDS : ZERO_THIRDS;
+                        T res = 0;
+                        for (int mk = 0; mk < (DIM == 3 ? 4 : 1); mk++) {
+                            T gam = 0;
+                            for (int mj = 0; mj < 4; mj++) {
+                                T beta = 0;
+                                for (int mi = 0; mi < 4; mi++)
+                                    beta = alpha[(3 - mi) + 4 * (3 - mj) + (DIM == 3 ? 16 * (3 - mk) : 0)] + (x + ox) * beta;
+                                gam = beta + (y + oy) * gam;
+                            }
+                            if (DIM == 3)
+                                res = gam + (z + oz) * res;
+                            else
+                                res = gam;
+                        }
+                        long ccell = (long)(i_kap_bc + c) * M_CBLOCK + t;
+                        if ((var == INTERP_INTERFACE && cells_mask[ccell] == CELL_GHOST) || var == INTERP_ADDED)
+                            Fp(V.pb[p])[ccell] = quant(res, p);
+                    }
+                }
+            }
+        }
+    }
+
     // ---------------------------------------------------------------------------------------
     // Cu_Interpolate_Linear (solver_lbm_kernels_comm.cu:13-138) + S_Interpolate_Linear (:141-155).
     // ---------------------------------------------------------------------------------------
-    void interpolate(int L, int var)
+    void interpolate_linear(int L, int var)
     {
         if (L >= MAX_LEVELS - 1) return;  // M_Interpolate guard (src/mesh/mesh_comm.cu:9)
         refresh(L);
@@ -844,21 +944,21 @@ struct Oracle {
                 interpolate(L, INTERP_INTERFACE);
                 collide_stream(L);
                 advance(L + 1);
-                average(L, AVERAGE_INTERFACE);
+                average(L, average_advance());
             } else {
                 if (L < MAX_LEVELS - 1) interpolate(L, INTERP_INTERFACE);
                 collide_stream(L);
                 if (L == MAX_LEVELS - 1) global_Liter++;
                 if (L < MAX_LEVELS - 1) {
                     advance(L + 1);
-                    average(L, AVERAGE_INTERFACE);
+                    average(L, average_advance());
                     interpolate(L, INTERP_INTERFACE);
                 }
                 collide_stream(L);
                 if (L == MAX_LEVELS - 1) global_Liter++;
                 if (L < MAX_LEVELS - 1) {
                     advance(L + 1);
-                    average(L, AVERAGE_INTERFACE);
+                    average(L, average_advance());
                 }
             }
         } else {

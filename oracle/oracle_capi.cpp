@@ -344,6 +344,22 @@ int oracle_cell_mask_rule(int Q, const int *skip, int t)
 
 int oracle_in_half_set(int dim, int p) { return in_half_set(dim, p) ? 1 : 0; }
 
+/* Terms of the cubic interpolation's alpha_idx (dim 2 or 3): fills q[] and coef[] (room for 64)
+ * in summation order and returns their number. */
+int oracle_cubic_terms(int dim, int idx, int *q, double *coef)
+{
+    const int *start = (dim == 3) ? CUBIC3_START : CUBIC2_START;
+    const unsigned char *qs = (dim == 3) ? CUBIC3_Q : CUBIC2_Q;
+    const double *cs = (dim == 3) ? CUBIC3_C : CUBIC2_C;
+    if (idx < 0 || idx >= (dim == 3 ? 64 : 16)) return -1;
+    int n = 0;
+    for (int k = start[idx]; k < start[idx + 1]; k++, n++) {
+        q[n] = qs[k];
+        coef[n] = cs[k];
+    }
+    return n;
+}
+
 /* The 16-bit conversions of the restatement: mode 1 binary16, mode 2 bfloat16. */
 int oracle_to16(int mode, const float *in, uint16_t *out, long long n)
 {

@@ -6,6 +6,7 @@ Reads (as text, nothing is compiled or run):
   src/mesh/mesh_amr.cu:154-944               child-neighbour table of Cu_UpdateConnectivity
   src/mesh/mesh_amr.cu:981-1297              ghost-cell conditions of Cu_UpdateMasks_2
   inc/parser.h:62-144                        input.txt keys and defaults
+  inc/util_interp.h:13-191                   coefficient tables of the cubic interpolation
 
 Run from the repository root: python tests/golden/gen_reference_tables.py [/root/reference]
 """
@@ -86,13 +87,30 @@ def parser_defaults(src):
     return {"int": ival, "double": {k: float(v) for k, v in dbls.items()}}
 
 
+def cubic_tables(src):
+    """Cu_Interp2D / Cu_Interp3D (inc/util_interp.h): case i -> [[q, coefficient], ...] in the
+    order of the source expression."""
+    out = {}
+    for name, key in (("Cu_Interp2D", "2"), ("Cu_Interp3D", "3")):
+        start = src.index(name)
+        nxt = src.find("template", start)
+        body = src[start:nxt if nxt > 0 else len(src)]
+        table = {}
+        for m in re.finditer(r"case\s+(\d+):\s*return\s+([^;]*);", body):
+            terms = re.findall(r"static_cast<ufloat_t>\(([-0-9.eE+]+)\)\*s_F\[(\d+)\]", m.group(2))
+            table[int(m.group(1))] = [[int(q), float(c)] for c, q in terms]
+        out[key] = table
+    return out
+
+
 def main():
     util = read("inc/util.h")
+    interp = read("inc/util_interp.h")
     init = read("src/solver_lbm/solver_lbm_init.cu")
     amr = read("src/mesh/mesh_amr.cu")
     parser = read("inc/parser.h")
     out = {
-        "source": "KhodrJ/AGAL (text of inc/util.h, solver_lbm_init.cu, mesh_amr.cu, parser.h)",
+        "source": "KhodrJ/AGAL (text of inc/util.h, util_interp.h, solver_lbm_init.cu, mesh_amr.cu, parser.h)",
         "V_CONN_ID_2D": int_array(util, "V_CONN_ID_2D"),
         "V_CONN_ID_3D": int_array(util, "V_CONN_ID_3D"),
         "V_CONN_ID_PB_2D": int_array(util, "V_CONN_ID_PB_2D"),
@@ -108,7 +126,9 @@ def main():
         "child_connectivity": connectivity(amr),
         "ghost_conditions": ghost_conditions(amr),
         "parser_defaults": parser_defaults(parser),
+        "interp_cubic": cubic_tables(interp),
     }
+    assert len(out["interp_cubic"]["2"]) == 16 and len(out["interp_cubic"]["3"]) == 64
     n2 = sum(len(v) for v in out["child_connectivity"]["2"].values())
     n3 = sum(len(v) for v in out["child_connectivity"]["3"].values())
     assert n2 == 9 * 4 and n3 == 27 * 8, (n2, n3)

@@ -82,6 +82,23 @@ def test_solution_adaptive_three_levels(vs):
     assert g.sizes.n_ids[2] > 0
 
 
+@pytest.mark.parametrize("vs,N,prec", [(capi.D3Q19, 16, capi.FP32), (capi.D3Q19, 16, capi.FP64),
+                                       (capi.D2Q9, 32, capi.FP32), (capi.D3Q27, 16, capi.FP16)])
+def test_cubic_interpolation_adaptive(vs, N, prec):
+    """Cubic coarse-to-fine interpolation with BLOCK averaging in the advance, through solution-
+    based rebuilds (ADDED children are interpolated cubically too)."""
+    p = ldc_params(vs=vs, N=N, Re=40.0, levels=3, P_REFINE=15, prec=prec, interp_order=capi.INTERP_CUBIC)
+    g, o = pair(p, 8192)
+    both((g, o), lambda m: m.S_SetIC(0))
+    both((g, o), lambda m: m.M_FreezeRefinedCells(0))
+    for k in range(3):
+        rg = g.advance_loop(15 * k, 15)
+        ro = o.advance_loop(15 * k, 15)
+        assert rg[0] == ro[0]
+        assert_same(g, o)
+    assert g.sizes.n_ids[1] > 0
+
+
 @pytest.mark.parametrize("crit", [1, 2])
 def test_solution_adaptive_other_criteria(crit):
     """The Q-like invariant and the |grad u| criterion drive the same rebuild bit for bit."""

@@ -210,6 +210,62 @@ def test_interpolation_stencil_is_trilinear_in_parent_sub_cube():
         assert np.max(np.abs(g[:, (fc + c) * 64:(fc + c + 1) * 64] - val)) < 1e-13, c
 
 
+@pytest.mark.parametrize("Q", [9, 19])
+def test_cubic_interpolation_reproduces_tricubic_fields(Q):
+    """Cu_Interpolate_Cubic (solver_lbm_kernels_comm.cu:406-638): with a uniform macroscopic state
+    and non-equilibrium parts that are cubic per axis in the cell position, the rescaled parent
+    values are a tensor-product cubic, which the Lagrange interpolant through the 4^d parent cells
+    reproduces exactly; so every child equals feq + tau ratio * d(child centre)."""
+    dim = 2 if Q == 9 else 3
+    p = ldc_params(vs=Q, N=16 if dim == 2 else 8, levels=2, prec=capi.FP64, interp_order=capi.INTERP_CUBIC)
+    m = OracleMesh(p, 256)
+    m.S_SetIC(0)
+    m.mark_blocks([3], capi.REF_MARK_REFINE)
+    m.M_RefineAndCoarsenBlocks()
+    nc = m.populations().shape[1]
+    rng = np.random.default_rng(11)
+    u0 = np.array([[0.02], [0.01], [-0.03 if dim == 3 else 0.0]])
+    fe = feq_np(Q, np.full(nc, 0.99), np.repeat(u0, nc, axis=1))
+    pos = cell_positions(m) / 4.0  # block units keep the monomials O(1)
+
+    def cubic_field(x):
+        # sum over a few random monomials x^i y^j z^k with i, j, k <= 3, each with a zero-moment
+        # direction vector
+        out = np.zeros((Q, x.shape[1]))
+        for _ in range(6):
+            e = rng.integers(0, 4, 3)
+            if dim == 2:
+                e[2] = 0
+            vec = zero_moment_perturbation(Q, 1, rng, scale=2e-3)
+            out += vec * (x[0] ** e[0] * x[1] ** e[1] * x[2] ** e[2])[None, :]
+        return out, None
+
+    state = rng.bit_generator.state
+    d, _ = cubic_field(pos)
+    m.set_populations(fe + d)
+    m.mark_blocks([3], capi.REF_MARK_REFINE)
+    m.S_Interpolate(0, capi.INTERP_ADDED)
+    g = m.populations()
+    dxf, tau = m.level_params()
+    ratio = tau[1] / tau[0]
+    rng.bit_generator.state = state
+    fc = m.block_ints(capi.BLK_NBR_CHILD)[0, 3]
+    cpb = 64 if dim == 3 else 16
+    kids = slice(fc * cpb, (fc + (8 if dim == 3 else 4)) * cpb)
+    dk, _ = cubic_field(cell_positions(m)[:, kids] / 4.0)
+    assert np.max(np.abs(g[:, kids] - (fe[:, kids] + ratio * dk))) < 1e-12
+    # a linear interpolant cannot do this: the same field through the linear kernel misses
+    p_lin = ldc_params(vs=Q, N=16 if dim == 2 else 8, levels=2, prec=capi.FP64)
+    m2 = OracleMesh(p_lin, 256)
+    m2.S_SetIC(0)
+    m2.mark_blocks([3], capi.REF_MARK_REFINE)
+    m2.M_RefineAndCoarsenBlocks()
+    m2.set_populations(fe + d)
+    m2.mark_blocks([3], capi.REF_MARK_REFINE)
+    m2.S_Interpolate(0, capi.INTERP_ADDED)
+    assert np.max(np.abs(m2.populations()[:, kids] - (fe[:, kids] + ratio * dk))) > 1e-6
+
+
 # ---- averaging (solver_lbm_kernels_comm.cu:174-347) --------------------------------------------
 def test_interpolate_then_average_restores_parent():
     """Uniform macroscopic state with linearly varying non-equilibrium parts: interpolation to new

@@ -94,3 +94,59 @@ def test_parameter_defaults_match_reference_parser():
         assert getattr(p, k) == ints[k], k
     for k in ("L_c", "L_fy", "L_fz", "v0", "N_REFINE_START", "N_REFINE_INC", "N_REFINE_MAX"):
         assert getattr(p, k) == dbls[k], k
+
+
+def _lagrange_monomial_matrix():
+    """M[i][a]: coefficient of x^i in the cubic Lagrange basis polynomial of node a/3, exactly."""
+    nodes = [Fraction(a, 3) for a in range(4)]
+    M = [[Fraction(0)] * 4 for _ in range(4)]
+    for a in range(4):
+        poly = [Fraction(1)]  # coefficients, lowest degree first
+        den = Fraction(1)
+        for b in range(4):
+            if b == a:
+                continue
+            poly = [Fraction(0)] + poly  # times x
+            for k in range(len(poly) - 1):
+                poly[k] -= nodes[b] * poly[k + 1]
+            den *= nodes[a] - nodes[b]
+        for i in range(4):
+            M[i][a] = poly[i] / den
+    return M
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_cubic_interpolation_tables_match_reference_text(dim):
+    """Cu_Interp2D/3D (inc/util_interp.h): every listed term is the exact tensor product of the
+    Lagrange monomial matrix for nodes {0,1/3,2/3,1}, except (3D only) terms whose exact value is
+    zero, which the reference lists with |C| < 1e-10; every exactly nonzero product is listed;
+    terms run in ascending s_F index.  The restatement sums the same terms in the same order."""
+    M = _lagrange_monomial_matrix()
+    lib = O.oracle_library()
+    lib.oracle_cubic_terms.restype = O.C.c_int
+    lib.oracle_cubic_terms.argtypes = [O.C.c_int, O.C.c_int, O.C.POINTER(O.C.c_int), O.C.POINTER(O.C.c_double)]
+    n = 4 ** dim
+    gold = GOLD["interp_cubic"][str(dim)]
+    assert len(gold) == n
+    n_tiny = 0
+    for idx in range(n):
+        i, j, k = idx % 4, (idx // 4) % 4, idx // 16
+        exact = {}
+        for q in range(n):
+            a, b, c = q % 4, (q // 4) % 4, q // 16
+            exact[q] = M[i][a] * M[j][b] * (M[k][c] if dim == 3 else 1)
+        terms = gold[str(idx)]
+        qs = [q for q, _ in terms]
+        assert qs == sorted(qs) and len(set(qs)) == len(qs)
+        for q, coef in terms:
+            if exact[q] == 0:
+                assert dim == 3 and abs(coef) < 1e-10
+                n_tiny += 1
+            else:
+                assert coef == float(exact[q]), (idx, q)
+        assert {q for q in range(n) if exact[q] != 0} <= set(qs)
+        qq = (O.C.c_int * 64)()
+        cc = (O.C.c_double * 64)()
+        m = lib.oracle_cubic_terms(dim, idx, qq, cc)
+        assert [[qq[t], cc[t]] for t in range(m)] == terms
+    assert n_tiny == (432 if dim == 3 else 0)
