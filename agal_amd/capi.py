@@ -27,9 +27,9 @@ AGAL_ERR_NOT_BUILT = 8
 
 D2Q9, D3Q19, D3Q27 = 9, 19, 27
 FP32, FP64, FP16, BF16 = 0, 1, 2, 3
-BGK = 0
+BGK, MRT = 0, 2
 BC_NONE, BC_LDC2D, BC_LDC3D, BC_FPSC2D, BC_FPSC3D = 0, 1, 2, 3, 4
-INTERP_LINEAR, INTERP_CUBIC = 0, 1          # interpolation order (agal_params.interp_order)
+INTERP_LINEAR, INTERP_CUBIC = 0, 2          # interpolation order (agal_params.interp_order)
 INTERP_INTERFACE, INTERP_ADDED = 0, 1       # interpolation mode (agal_interpolate var)
 AVERAGE_INTERFACE, AVERAGE_BLOCK, AVERAGE_GRID = 0, 1, 2
 REF_UNREFINED, REF_REFINED, REF_REFINED_WCHILD, REF_REFINED_PERM = 0, 1, 2, 3
@@ -69,6 +69,12 @@ class Params(C.Structure):
         ("ic_u", C.c_double),
         ("ic_v", C.c_double),
         ("ic_w", C.c_double),
+        ("s1", C.c_double),
+        ("s2", C.c_double),
+        ("s3", C.c_double),
+        ("s4", C.c_double),
+        ("s5", C.c_double),
+        ("s6", C.c_double),
     ]
 
 
@@ -109,6 +115,7 @@ def default_params(**kw) -> Params:
     p.bc_preset = BC_LDC3D
     p.bc_velocity = 0.05
     p.ic_rho, p.ic_u, p.ic_v, p.ic_w = 1.0, 0.0, 0.0, 0.0
+    p.s1, p.s2, p.s3, p.s4, p.s5, p.s6 = 1.19, 1.4, 1.2, 1.2, 1.2, 1.98
     for k, v in kw.items():
         if not hasattr(p, k):
             raise KeyError(k)

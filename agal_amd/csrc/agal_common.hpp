@@ -102,6 +102,134 @@ __host__ __device__ constexpr int dir2(int dx, int dy)
     return m[(dx + 1) + 3 * (dy + 1)];
 }
 
+// ---------------------------------------------------------------------------------------------
+// MRT moment bases: integer rows m(k, p) over the velocity set, mutually orthogonal, so that
+// M^-1 = M^T D^-1 with D_k = sum_p m(k,p)^2.  rate(k): 0 conserved (not relaxed), 1 the viscous
+// rate omega_L, 2..7 the parameters s1..s6.
+//   D3Q19: d'Humieres et al. (2002), the rows of the reference's legacy transform
+//          (src/solver_old/solver_lbm_vel2mom_mrt_d3q19.cu:118-136) and its rate assignment
+//          (solver_lbm_collide_mrt_d3q19.cu:132-146).
+//   D2Q9 / D3Q27: products of the 1D orthogonal polynomials {1, c, 3c^2 - 2} per axis, with the
+//          second-order block {P2x, P2y, P2z} rotated into its trace (bulk) and deviators.
+// ---------------------------------------------------------------------------------------------
+__host__ __device__ constexpr int mrt_p2(int c) { return 3 * c * c - 2; }
+
+template <int Q>
+struct Mrt;
+
+template <>
+struct Mrt<19> {
+    __host__ __device__ static constexpr int m(int k, int p)
+    {
+        const int x = c3x(p), y = c3y(p), z = c3z(p), c2 = x * x + y * y + z * z;
+        switch (k) {
+            case 0: return 1;
+            case 1: return 19 * c2 - 30;
+            case 2: return (21 * c2 * c2 - 53 * c2 + 24) / 2;
+            case 3: return x;
+            case 4: return (5 * c2 - 9) * x;
+            case 5: return y;
+            case 6: return (5 * c2 - 9) * y;
+            case 7: return z;
+            case 8: return (5 * c2 - 9) * z;
+            case 9: return 3 * x * x - c2;
+            case 10: return (3 * c2 - 5) * (3 * x * x - c2);
+            case 11: return y * y - z * z;
+            case 12: return (3 * c2 - 5) * (y * y - z * z);
+            case 13: return x * y;
+            case 14: return y * z;
+            case 15: return x * z;
+            case 16: return (y * y - z * z) * x;
+            case 17: return (z * z - x * x) * y;
+            case 18: return (x * x - y * y) * z;
+        }
+        return 0;
+    }
+    __host__ __device__ static constexpr int rate(int k)
+    {
+        constexpr int r[19] = {0, 2, 3, 0, 4, 0, 4, 0, 4, 1, 6, 1, 6, 1, 1, 1, 7, 7, 7};
+        return r[k];
+    }
+};
+
+template <>
+struct Mrt<27> {
+    __host__ __device__ static constexpr int m(int k, int p)
+    {
+        const int x = c3x(p), y = c3y(p), z = c3z(p);
+        const int px = mrt_p2(x), py = mrt_p2(y), pz = mrt_p2(z);
+        switch (k) {
+            case 0: return 1;
+            case 1: return x;
+            case 2: return y;
+            case 3: return z;
+            case 4: return x * y;
+            case 5: return y * z;
+            case 6: return x * z;
+            case 7: return 2 * px - py - pz;
+            case 8: return py - pz;
+            case 9: return px + py + pz;
+            case 10: return px * y;
+            case 11: return px * z;
+            case 12: return py * x;
+            case 13: return py * z;
+            case 14: return pz * x;
+            case 15: return pz * y;
+            case 16: return x * y * z;
+            case 17: return px * py;
+            case 18: return py * pz;
+            case 19: return px * pz;
+            case 20: return px * y * z;
+            case 21: return py * x * z;
+            case 22: return pz * x * y;
+            case 23: return px * py * z;
+            case 24: return py * pz * x;
+            case 25: return px * pz * y;
+            case 26: return px * py * pz;
+        }
+        return 0;
+    }
+    __host__ __device__ static constexpr int rate(int k)
+    {
+        constexpr int r[27] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 5, 5, 5, 6, 6, 6, 7};
+        return r[k];
+    }
+};
+
+template <>
+struct Mrt<9> {
+    __host__ __device__ static constexpr int m(int k, int p)
+    {
+        const int x = c2x(p), y = c2y(p);
+        const int px = mrt_p2(x), py = mrt_p2(y);
+        switch (k) {
+            case 0: return 1;
+            case 1: return x;
+            case 2: return y;
+            case 3: return x * y;
+            case 4: return px - py;
+            case 5: return px + py;
+            case 6: return px * y;
+            case 7: return py * x;
+            case 8: return px * py;
+        }
+        return 0;
+    }
+    __host__ __device__ static constexpr int rate(int k)
+    {
+        constexpr int r[9] = {0, 0, 0, 1, 1, 2, 4, 4, 3};
+        return r[k];
+    }
+};
+
+template <int Q>
+__host__ __device__ constexpr int mrt_norm(int k)
+{
+    int s = 0;
+    for (int p = 0; p < Q; p++) s += Mrt<Q>::m(k, p) * Mrt<Q>::m(k, p);
+    return s;
+}
+
 // Boundary-condition presets of the reference hook Cu_ImposeBC (inc/custom.h:66-123).
 enum BcPreset { BC_NONE = 0, BC_LDC2D = 1, BC_LDC3D = 2, BC_FPSC2D = 3, BC_FPSC3D = 4 };
 

@@ -483,6 +483,12 @@ void agal_params_default(agal_params *p)
     p->bc_preset = AGAL_BC_LDC3D;
     p->bc_velocity = 0.05;
     p->ic_rho = 1.0;
+    p->s1 = 1.19;
+    p->s2 = 1.4;
+    p->s3 = 1.2;
+    p->s4 = 1.2;
+    p->s5 = 1.2;
+    p->s6 = 1.98;
 }
 
 const char *agal_status_string(int s)
@@ -514,7 +520,7 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     if (p->precision != AGAL_FP32 && p->precision != AGAL_FP64 && p->precision != AGAL_FP16 &&
         p->precision != AGAL_BF16)
         return AGAL_ERR_ARG;
-    if (p->collision != AGAL_BGK) return AGAL_ERR_NOT_BUILT;
+    if (p->collision != AGAL_BGK && p->collision != AGAL_MRT) return AGAL_ERR_NOT_BUILT;
     if (p->interp_order != AGAL_INTERP_LINEAR && p->interp_order != AGAL_INTERP_CUBIC) return AGAL_ERR_ARG;
     if (p->Nx <= 0 || p->Nx % 4 != 0) return AGAL_ERR_ARG;
     Ctx *c = new (std::nothrow) Ctx;

@@ -53,9 +53,15 @@ extern "C" {
 #define AGAL_BF16      3    /* f - w_q stored as bfloat16                                        */
 
 #define AGAL_BGK       0    /* CollisionOperator::BGK (inc/solver_lbm.h:19-24)                  */
+/* CollisionOperator::MRT.  The reference's MRT exists only as dead legacy code for D3Q19
+ * (src/solver_old/solver_lbm_collide_mrt_d3q19.cu:14-1460); here f* = f - M^T D^-1 S M (f - feq)
+ * with integer orthogonal bases: d'Humieres et al. (2002) for D3Q19 (the legacy rows), tensor
+ * products of {1, c, 3c^2-2} with the second-order block split into trace and deviator for D2Q9
+ * and D3Q27.  Stress moments relax with omega = dx_L/tau_L, the others with s1..s6. */
+#define AGAL_MRT       2
 
 #define AGAL_INTERP_LINEAR 0 /* InterpOrder::Linear (inc/solver_lbm.h:25-30)                    */
-#define AGAL_INTERP_CUBIC  1 /* InterpOrder::Cubic: Cu_Interpolate_Cubic (solver_lbm_kernels_comm.cu:
+#define AGAL_INTERP_CUBIC  2 /* InterpOrder::Cubic: Cu_Interpolate_Cubic (solver_lbm_kernels_comm.cu:
                                 406-638); the advance then averages in BLOCK mode (solver_lbm_init.cu:94-97) */
 
 /* Boundary-condition presets of the reference's device hook Cu_ImposeBC (inc/custom.h:66-123). */
@@ -101,7 +107,7 @@ extern "C" {
 typedef struct agal_params {
     int    velocity_set;          /* AGAL_D2Q9 | AGAL_D3Q19 | AGAL_D3Q27 (VSET, src/Makefile:160)   */
     int    precision;             /* AGAL_FP32 | AGAL_FP64 (REAL_s, src/main.cu:60) | FP16 | BF16  */
-    int    collision;             /* AGAL_BGK (S_COLLISION)                                       */
+    int    collision;             /* AGAL_BGK | AGAL_MRT (S_COLLISION)                            */
     int    interp_order;          /* AGAL_INTERP_LINEAR | AGAL_INTERP_CUBIC (S_INTERP)            */
     int    Nx;                    /* root-grid cells along x (Nx)                                 */
     double L_c, L_fy, L_fz;       /* domain length and aspect ratios                              */
@@ -115,6 +121,8 @@ typedef struct agal_params {
     int    bc_preset;             /* AGAL_BC_* : the Cu_ImposeBC hook                             */
     double bc_velocity;           /* wall / inflow speed used by the hook (0.05 in custom.h)      */
     double ic_rho, ic_u, ic_v, ic_w; /* Cu_ComputeIC hook: uniform state (custom.h:48-60)         */
+    double s1, s2, s3, s4, s5, s6;   /* MRT relaxation rates of the non-hydrodynamic moments
+                                        (input.txt keys s1..s6, solver_lbm_init.cu:55-61)       */
 } agal_params;
 
 typedef struct agal_ctx agal_ctx;

@@ -572,6 +572,13 @@ struct Oracle {
                 if (DIM == 3) Fp(Q + 3)[cell] = w;
                 T omeg = dx_L / tau_L;
                 T omegp = (T)(1.0) - omeg;
+                if (P.collision == AGAL_MRT) {
+                    T fs[27];
+                    mrt_collide(f_p, fs, rho, u, v, w, udotu, omeg);
+                    for (int p = 0; p < Q; p++)
+                        if (valid_mask != CELL_SOLID) Fp(p)[cell] = fs[p];
+                    continue;
+                }
                 for (int p = 0; p < Q; p++) {
                     T cdotu = (T)V.c[0][p] * u + (T)V.c[1][p] * v + (T)V.c[2][p] * w;
                     T f_pi = f_p[p] * omegp +
@@ -579,6 +586,86 @@ struct Oracle {
                     if (valid_mask != CELL_SOLID) Fp(p)[cell] = f_pi;
                 }
             }
+        }
+    }
+
+    // ---------------------------------------------------------------------------------------
+    // MRT (CollisionOperator::MRT; the reference has it only as legacy D3Q19 code,
+    // src/solver_old/solver_lbm_collide_mrt_d3q19.cu:14-1460).  f* = f - M^T D^-1 S M (f - feq)
+    // with an integer orthogonal basis M: for D3Q19 the d'Humieres et al. (2002) rows of the
+    // legacy transform (solver_lbm_vel2mom_mrt_d3q19.cu:118-136) and its rates (legacy collide
+    // :132-146: s1 e, s2 eps, s3 q, s5 pi, s6 m, omega for the stresses); for D2Q9 / D3Q27 the
+    // products of {1, c, 3c^2-2} per axis with the second-order block split into trace (s1) and
+    // deviators (omega).  M (f - feq) equals m - m_eq of the legacy form, whose equilibrium
+    // moments are those of feq.
+    // ---------------------------------------------------------------------------------------
+    int mrt_m(int k, int p) const
+    {
+        const int x = V.c[0][p], y = V.c[1][p], z = V.c[2][p];
+        auto P2 = [](int c) { return 3 * c * c - 2; };
+        if (Q == 19) {
+            const int c2 = x * x + y * y + z * z;
+            const int row[19] = {1, 19 * c2 - 30, (21 * c2 * c2 - 53 * c2 + 24) / 2, x, (5 * c2 - 9) * x, y,
+                                 (5 * c2 - 9) * y, z, (5 * c2 - 9) * z, 3 * x * x - c2, (3 * c2 - 5) * (3 * x * x - c2),
+                                 y * y - z * z, (3 * c2 - 5) * (y * y - z * z), x * y, y * z, x * z,
+                                 (y * y - z * z) * x, (z * z - x * x) * y, (x * x - y * y) * z};
+            return row[k];
+        }
+        const int px = P2(x), py = P2(y), pz = P2(z);
+        if (Q == 9) {
+            const int row[9] = {1, x, y, x * y, px - py, px + py, px * y, py * x, px * py};
+            return row[k];
+        }
+        const int row[27] = {1, x, y, z, x * y, y * z, x * z, 2 * px - py - pz, py - pz, px + py + pz,
+                             px * y, px * z, py * x, py * z, pz * x, pz * y, x * y * z, px * py, py * pz, px * pz,
+                             px * y * z, py * x * z, pz * x * y, px * py * z, py * pz * x, px * pz * y, px * py * pz};
+        return row[k];
+    }
+
+    // 0: conserved, 1: omega, 2..7: s1..s6.
+    int mrt_rate(int k) const
+    {
+        static const int r19[19] = {0, 2, 3, 0, 4, 0, 4, 0, 4, 1, 6, 1, 6, 1, 1, 1, 7, 7, 7};
+        static const int r27[27] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 5, 5, 5, 6, 6, 6, 7};
+        static const int r9[9] = {0, 0, 0, 1, 1, 2, 4, 4, 3};
+        return Q == 19 ? r19[k] : (Q == 27 ? r27[k] : r9[k]);
+    }
+
+    void mrt_collide(const T *f, T *fs, T rho, T u, T v, T w, T udotu, T omeg) const
+    {
+        const double s[6] = {P.s1, P.s2, P.s3, P.s4, P.s5, P.s6};
+        T d[27], r[27];
+        for (int p = 0; p < Q; p++) d[p] = f[p] - feq(p, rho, u, v, w, udotu);
+        for (int k = 0; k < Q; k++) {
+            r[k] = 0;
+            const int rc = mrt_rate(k);
+            if (rc == 0) continue;
+            int norm = 0;
+            for (int p = 0; p < Q; p++) norm += mrt_m(k, p) * mrt_m(k, p);
+            const T rate = rc == 1 ? omeg : (T)s[rc - 2];
+            const T ck = (T)((double)rate / (double)norm);
+            T acc = 0;
+            bool first = true;
+            for (int p = 0; p < Q; p++) {
+                const int m = mrt_m(k, p);
+                if (m == 0) continue;
+                const T term = (T)m * d[p];
+                acc = first ? term : acc + term;
+                first = false;
+            }
+            r[k] = acc * ck;
+        }
+        for (int p = 0; p < Q; p++) {
+            T acc = 0;
+            bool first = true;
+            for (int k = 0; k < Q; k++) {
+                const int m = mrt_m(k, p);
+                if (mrt_rate(k) == 0 || m == 0) continue;
+                const T term = (T)m * r[k];
+                acc = first ? term : acc + term;
+                first = false;
+            }
+            fs[p] = f[p] - acc;
         }
     }
 

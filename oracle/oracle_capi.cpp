@@ -360,6 +360,20 @@ int oracle_cubic_terms(int dim, int idx, int *q, double *coef)
     return n;
 }
 
+/* The restatement's MRT basis: fills m[k*Q + p] and rate[k] (0 conserved, 1 omega, 2..7 s1..s6). */
+int oracle_mrt_basis(int Q, int *m, int *rate)
+{
+    if (Q != 9 && Q != 19 && Q != 27) return AGAL_ERR_ARG;
+    Oracle<double> O;
+    O.V = make_velset(Q);
+    O.Q = Q;
+    for (int k = 0; k < Q; k++) {
+        rate[k] = O.mrt_rate(k);
+        for (int p = 0; p < Q; p++) m[k * Q + p] = O.mrt_m(k, p);
+    }
+    return AGAL_OK;
+}
+
 /* The 16-bit conversions of the restatement: mode 1 binary16, mode 2 bfloat16. */
 int oracle_to16(int mode, const float *in, uint16_t *out, long long n)
 {

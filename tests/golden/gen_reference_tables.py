@@ -7,6 +7,8 @@ Reads (as text, nothing is compiled or run):
   src/mesh/mesh_amr.cu:981-1297              ghost-cell conditions of Cu_UpdateMasks_2
   inc/parser.h:62-144                        input.txt keys and defaults
   inc/util_interp.h:13-191                   coefficient tables of the cubic interpolation
+  src/solver_old/solver_lbm_vel2mom_mrt_d3q19.cu:118-136, solver_lbm_collide_mrt_d3q19.cu:132-146
+                                             legacy D3Q19 MRT moment rows and rate assignment
 
 Run from the repository root: python tests/golden/gen_reference_tables.py [/root/reference]
 """
@@ -103,8 +105,26 @@ def cubic_tables(src):
     return out
 
 
+def mrt_d3q19(vel2mom, collide):
+    """Legacy D3Q19 MRT: the moment rows m_k = sum_j C f_j (solver_lbm_vel2mom_mrt_d3q19.cu:118-136)
+    and, per moment, the rate symbol its collision uses (solver_lbm_collide_mrt_d3q19.cu:132-146)."""
+    rows = []
+    for k in range(19):
+        m = re.search(r"\tm_%d = (.*);" % k, vel2mom)
+        row = [0.0] * 19
+        for c, j in re.findall(r"N_Pf\(([-0-9.]+)\)\*f_(\d+)", m.group(1)):
+            row[int(j)] = float(c)
+        rows.append(row)
+    rates = {}
+    for k, rate in re.findall(r"\tm_(\d+) = m_\d+ - \(?(s_\d|omeg)\)?\*", collide):
+        rates[int(k)] = rate
+    return {"rows": rows, "rates": rates}
+
+
 def main():
     util = read("inc/util.h")
+    mrt = mrt_d3q19(read("src/solver_old/solver_lbm_vel2mom_mrt_d3q19.cu"),
+                    read("src/solver_old/solver_lbm_collide_mrt_d3q19.cu"))
     interp = read("inc/util_interp.h")
     init = read("src/solver_lbm/solver_lbm_init.cu")
     amr = read("src/mesh/mesh_amr.cu")
@@ -127,7 +147,9 @@ def main():
         "ghost_conditions": ghost_conditions(amr),
         "parser_defaults": parser_defaults(parser),
         "interp_cubic": cubic_tables(interp),
+        "mrt_d3q19_legacy": mrt,
     }
+    assert len(mrt["rates"]) == 15, mrt["rates"]
     assert len(out["interp_cubic"]["2"]) == 16 and len(out["interp_cubic"]["3"]) == 64
     n2 = sum(len(v) for v in out["child_connectivity"]["2"].values())
     n3 = sum(len(v) for v in out["child_connectivity"]["3"].values())

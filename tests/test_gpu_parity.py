@@ -99,6 +99,30 @@ def test_cubic_interpolation_adaptive(vs, N, prec):
     assert g.sizes.n_ids[1] > 0
 
 
+@pytest.mark.parametrize("vs,N", [(capi.D2Q9, 32), (capi.D3Q19, 16), (capi.D3Q27, 8)])
+@pytest.mark.parametrize("prec", [capi.FP32, capi.FP64])
+def test_mrt_uniform_ldc(vs, N, prec):
+    p = ldc_params(vs=vs, N=N, Re=50.0, levels=1, prec=prec, collision=capi.MRT)
+    g, o = pair(p, 512)
+    both((g, o), lambda m: m.S_SetIC(0))
+    for _ in range(60):
+        both((g, o), lambda m: m.S_Advance(0))
+    assert_same(g, o)
+
+
+@pytest.mark.parametrize("vs,prec", [(capi.D3Q19, capi.FP32), (capi.D3Q27, capi.FP16)])
+def test_mrt_adaptive(vs, prec):
+    p = ldc_params(vs=vs, N=16, Re=40.0, levels=3, P_REFINE=15, prec=prec, collision=capi.MRT)
+    g, o = pair(p, 8192)
+    both((g, o), lambda m: m.S_SetIC(0))
+    for k in range(3):
+        rg = g.advance_loop(15 * k, 15)
+        ro = o.advance_loop(15 * k, 15)
+        assert rg[0] == ro[0]
+        assert_same(g, o)
+    assert g.sizes.n_ids[1] > 0
+
+
 @pytest.mark.parametrize("crit", [1, 2])
 def test_solution_adaptive_other_criteria(crit):
     """The Q-like invariant and the |grad u| criterion drive the same rebuild bit for bit."""

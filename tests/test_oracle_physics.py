@@ -10,6 +10,11 @@ import oracle_lib as O
 from oracle_lib import OracleMesh
 from parity import ldc_params
 
+import json as _json
+import os as _os
+
+GOLD_MRT = _json.load(open(_os.path.join(_os.path.dirname(__file__), "golden", "reference_tables.json")))["mrt_d3q19_legacy"]
+
 
 def feq_np(Q, rho, u):
     """Second-order equilibrium, f_q = w_q rho (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u)."""
@@ -455,3 +460,103 @@ def test_inflow_outflow_presets_one_step_closed_form(Q, preset):
     assert np.max(np.abs(f1 - expect)) < 1e-14
     # the outlet really reproduces the equilibrium of the opposite direction
     assert np.max(np.abs(f1 - f0)) > 1e-3 * U
+
+
+# ---- MRT (legacy D3Q19 definition: src/solver_old/solver_lbm_collide_mrt_d3q19.cu:14-1460) -------
+def mrt_basis(Q):
+    m = (O.C.c_int * (Q * Q))()
+    r = (O.C.c_int * Q)()
+    assert O.oracle_library().oracle_mrt_basis(Q, m, r) == 0
+    return np.array(list(m)).reshape(Q, Q), np.array(list(r))
+
+
+@pytest.mark.parametrize("Q", [9, 19, 27])
+def test_mrt_basis_orthogonal_integer_complete(Q):
+    M, rate = mrt_basis(Q)
+    G = M @ M.T
+    assert np.array_equal(G, np.diag(np.diag(G)))
+    assert np.linalg.matrix_rank(M) == Q
+    c, _, _, _ = O.velset(Q)
+    dim = 2 if Q == 9 else 3
+    cons = M[rate == 0]
+    # the conserved rows span exactly {1, c_x, c_y(, c_z)}
+    basis = np.vstack([np.ones(Q), c[:dim, :Q]])
+    assert cons.shape[0] == dim + 1
+    assert np.linalg.matrix_rank(np.vstack([cons, basis])) == dim + 1
+    # the viscous (omega) rows span the traceless second-order tensors
+    sec = [c[a, :Q] * c[b, :Q] for a in range(dim) for b in range(a, dim)]
+    trace = sum(c[a, :Q] ** 2 for a in range(dim))
+    dev = [s for s in sec]
+    visc = M[rate == 1]
+    assert visc.shape[0] == (2 if dim == 2 else 5)
+    # every viscous row is a second-order polynomial orthogonal to the trace
+    span = np.vstack([np.ones(Q)] + dev)
+    assert np.linalg.matrix_rank(np.vstack([span, visc])) == span.shape[0]
+    assert np.allclose(visc @ (trace - trace.mean()), 0)
+
+
+def test_mrt_d3q19_is_the_legacy_definition():
+    M, rate = mrt_basis(19)
+    leg = GOLD_MRT["rows"]
+    assert np.array_equal(M, np.array(leg))
+    sym = {"s_1": 2, "s_2": 3, "s_3": 4, "s_4": 5, "s_5": 6, "s_6": 7, "omeg": 1}
+    for k in range(19):
+        expect = sym[GOLD_MRT["rates"][str(k)]] if str(k) in GOLD_MRT["rates"] else 0
+        assert rate[k] == expect, k
+
+
+@pytest.mark.parametrize("Q", [9, 19, 27])
+def test_mrt_relaxes_each_moment_at_its_rate_and_equals_bgk_at_one_rate(Q):
+    dim = 2 if Q == 9 else 3
+    rates = dict(s1=1.19, s2=1.4, s3=1.2, s4=1.1, s5=1.3, s6=1.98)
+    p = ldc_params(vs=Q, N=8 if dim == 3 else 16, Re=20.0, prec=capi.FP64, bc_preset=capi.BC_NONE,
+                   collision=capi.MRT, **rates)
+    m = OracleMesh(p, 64)
+    m.S_SetIC(0)
+    nc = m.populations().shape[1]
+    rng = np.random.default_rng(40 + Q)
+    rho = 1 + 0.1 * rng.uniform(-1, 1, nc)
+    u = np.zeros((3, nc))
+    u[:dim] = 0.08 * rng.uniform(-1, 1, (dim, nc))
+    fe = feq_np(Q, rho, u)
+    f = fe * (1 + 0.2 * rng.uniform(-1, 1, fe.shape))
+    _, _, pb, _ = O.velset(Q)
+    m.set_populations(f)
+    m.collide(0)
+    post = m.populations()[pb[:Q]]
+    r0, u0 = moments_np(Q, f)
+    fe0 = feq_np(Q, r0, u0)
+    M, rate = mrt_basis(Q)
+    dxf, tau = m.level_params()
+    omega = dxf[0] / tau[0]
+    S = np.array([0.0, omega] + [rates[f"s{i}"] for i in range(1, 7)])[rate]
+    lhs = M @ (post - fe0)
+    rhs = (1 - S)[:, None] * (M @ (f - fe0))
+    assert np.max(np.abs(lhs - rhs)) < 1e-13
+    # all rates equal to omega: BGK
+    p1 = ldc_params(vs=Q, N=8 if dim == 3 else 16, Re=20.0, prec=capi.FP64, bc_preset=capi.BC_NONE,
+                    collision=capi.MRT, **{f"s{i}": omega for i in range(1, 7)})
+    p2 = ldc_params(vs=Q, N=8 if dim == 3 else 16, Re=20.0, prec=capi.FP64, bc_preset=capi.BC_NONE)
+    a, b = OracleMesh(p1, 64), OracleMesh(p2, 64)
+    for mm in (a, b):
+        mm.S_SetIC(0)
+        mm.set_populations(f)
+        mm.collide(0)
+    assert np.max(np.abs(a.populations() - b.populations())) < 1e-14
+
+
+def test_mrt_couette_closed_form():
+    """Steady Couette flow depends only on the shear viscosity, which MRT sets through omega."""
+    N, U = 8, 0.01
+    p = ldc_params(vs=19, N=N, prec=capi.FP64, PERIODIC_X=1, PERIODIC_Y=1, v0=(1.0 / 6.0) / N, bc_velocity=U,
+                   collision=capi.MRT)
+    m = OracleMesh(p, 64)
+    m.S_SetIC(0)
+    for _ in range(4000):
+        m.S_Advance(0)
+    m.S_RefreshVariables(0)
+    mac = m.macros()
+    h = cell_positions(m)[2]
+    # the steady profile is linear to within O(U^2) effects of the relaxed non-hydrodynamic
+    # moments (~3e-9 here, against U h / N ~ 1e-2)
+    assert np.max(np.abs(mac[1] - U * h / N)) < 1e-8

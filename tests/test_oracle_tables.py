@@ -92,7 +92,8 @@ def test_parameter_defaults_match_reference_parser():
     for k in ("MAX_LEVELS_WALL", "MAX_LEVELS_INTERIOR", "N_LEVEL_START", "Nx", "PERIODIC_X", "PERIODIC_Y",
               "PERIODIC_Z", "S_CRITERION", "P_REFINE"):
         assert getattr(p, k) == ints[k], k
-    for k in ("L_c", "L_fy", "L_fz", "v0", "N_REFINE_START", "N_REFINE_INC", "N_REFINE_MAX"):
+    for k in ("L_c", "L_fy", "L_fz", "v0", "N_REFINE_START", "N_REFINE_INC", "N_REFINE_MAX",
+              "s1", "s2", "s3", "s4", "s5", "s6"):
         assert getattr(p, k) == dbls[k], k
 
 
