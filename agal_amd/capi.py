@@ -36,7 +36,8 @@ REF_UNREFINED, REF_REFINED, REF_REFINED_WCHILD, REF_REFINED_PERM = 0, 1, 2, 3
 REF_MARK_REFINE, REF_MARK_COARSEN, REF_NEW, REF_REMOVE, REF_INACTIVE = 4, 5, 6, 7, 8
 REF_UNREFINED_VIO, REF_REFINED_WCHILD_PERM = 9, 10
 N_SKIPID = -999
-CELL_INTERIOR, CELL_INTERFACE, CELL_GHOST, CELL_SOLID = 0, 1, 2, -1
+CELL_INTERIOR, CELL_INTERFACE, CELL_GHOST, CELL_SOLID, CELL_BOUNDARY = 0, 1, 2, -1, -6
+BLOCKMASK_REGULAR, BLOCKMASK_INTERFACE, BLOCKMASK_SOLIDB, BLOCKMASK_SOLIDA, BLOCKMASK_SOLID = 0, 1, -1, -2, -3
 BLK_NBR, BLK_NBR_CHILD, BLK_REF, BLK_LEVEL, BLK_ONB, BLK_MASK = 0, 1, 2, 3, 4, 5
 OPT_GRAPHS = 1
 
@@ -75,6 +76,7 @@ class Params(C.Structure):
         ("s4", C.c_double),
         ("s5", C.c_double),
         ("s6", C.c_double),
+        ("G_NEAR_WALL_DISTANCE", C.c_double),
     ]
 
 
@@ -116,6 +118,7 @@ def default_params(**kw) -> Params:
     p.bc_velocity = 0.05
     p.ic_rho, p.ic_u, p.ic_v, p.ic_w = 1.0, 0.0, 0.0, 0.0
     p.s1, p.s2, p.s3, p.s4, p.s5, p.s6 = 1.19, 1.4, 1.2, 1.2, 1.2, 1.98
+    p.G_NEAR_WALL_DISTANCE = 0.05
     for k, v in kw.items():
         if not hasattr(p, k):
             raise KeyError(k)
@@ -165,11 +168,23 @@ PROTOTYPES = {
     "set_option": (_I, [_P, _I, _LL]),
     "timing_reset": (_I, [_P, _I]),
     "timing_get": (_I, [_P, C.POINTER(C.c_double), C.POINTER(_LL), C.POINTER(C.c_double)]),
+    "refine_near_wall": (_I, [_P]),
+    "set_geometry": (_I, [_P, C.POINTER(C.c_float), _LL]),
+    "voxelize": (_I, [_P, _I]),
+    "check_masks": (_I, [_P, _I]),
+    "forces": (_I, [_P, C.POINTER(C.c_double)]),
+}
+
+# Host helpers without a context.
+HOST_PROTOTYPES = {
+    "icosphere": (_LL, [C.c_double, C.c_double, C.c_double, C.c_double, _I, C.c_double, C.c_ulonglong,
+                        C.POINTER(C.c_float), _LL]),
+    "load_stl": (_LL, [C.c_char_p, C.POINTER(C.c_float), _LL]),
 }
 
 # Every symbol include/agal_capi.h declares (checked by tests/test_capi_symbols.py).
 EXPORTED = ["agal_params_default", "agal_status_string", "agal_last_error"] + [
-    "agal_" + k for k in PROTOTYPES
+    "agal_" + k for k in list(PROTOTYPES) + list(HOST_PROTOTYPES)
 ]
 
 
@@ -202,6 +217,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.agal_last_error.argtypes = [_P]
     lib.agal_params_default.restype = None
     lib.agal_params_default.argtypes = [C.POINTER(Params)]
+    for name, (res, args) in HOST_PROTOTYPES.items():
+        f = getattr(lib, "agal_" + name)
+        f.restype, f.argtypes = res, args
     if path == LIB_PATH:
         _LIB = lib
     return lib
@@ -211,3 +229,29 @@ def np_ptr(a: np.ndarray, ctype=None):
     if ctype is None:
         return a.ctypes.data_as(C.c_void_p)
     return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def icosphere(center, radius: float, subdivisions: int, noise: float = 0.0, seed: int = 0) -> np.ndarray:
+    """Triangles (n, 3, 3) float32 of the seeded noisy icosphere built by agal_icosphere."""
+    lib = load_library()
+    n = lib.agal_icosphere(*map(float, center), float(radius), int(subdivisions), float(noise), int(seed), None, 0)
+    if n < 0:
+        raise AgalError(AGAL_ERR_ARG, "agal_icosphere")
+    out = np.empty((n, 3, 3), dtype=np.float32)
+    m = lib.agal_icosphere(*map(float, center), float(radius), int(subdivisions), float(noise), int(seed),
+                           np_ptr(out, C.c_float), n)
+    if m != n:
+        raise AgalError(AGAL_ERR_ARG, "agal_icosphere")
+    return out
+
+
+def load_stl(path: str) -> np.ndarray:
+    """Triangles (n, 3, 3) float32 of an ASCII STL file (agal_load_stl)."""
+    lib = load_library()
+    n = lib.agal_load_stl(path.encode(), None, 0)
+    if n < 0:
+        raise AgalError(AGAL_ERR_ARG, f"agal_load_stl({path})", f"code {n}")
+    out = np.empty((n, 3, 3), dtype=np.float32)
+    if lib.agal_load_stl(path.encode(), np_ptr(out, C.c_float), n) != n:
+        raise AgalError(AGAL_ERR_ARG, f"agal_load_stl({path})")
+    return out

@@ -13,9 +13,10 @@ namespace agal {
 constexpr int N_SKIPID = -999;
 constexpr int REF_UNREFINED = 0, REF_UNREFINED_VIO = 9, REF_REFINED = 1, REF_REFINED_WCHILD = 2,
               REF_REFINED_PERM = 3, REF_REFINED_WCHILD_PERM = 10, REF_MARK_REFINE = 4,
-              REF_MARK_COARSEN = 5, REF_NEW = 6, REF_REMOVE = 7, REF_INACTIVE = 8;
-constexpr int CELL_INTERIOR = 0, CELL_INTERFACE = 1, CELL_GHOST = 2, CELL_SOLID = -1;
-constexpr int BLK_REGULAR = 0, BLK_INTERFACE = 1, BLK_SOLID = -3, BLK_SOLIDA = -2;
+              REF_MARK_COARSEN = 5, REF_NEW = 6, REF_REMOVE = 7, REF_INACTIVE = 8, REF_INDET_E = 11,
+              REF_INDET_O = 12;
+constexpr int CELL_INTERIOR = 0, CELL_INTERFACE = 1, CELL_GHOST = 2, CELL_SOLID = -1, CELL_BOUNDARY = -6;
+constexpr int BLK_REGULAR = 0, BLK_INTERFACE = 1, BLK_SOLID = -3, BLK_SOLIDA = -2, BLK_SOLIDB = -1;
 
 // D3Q27 order; D3Q19 is its first 19 entries (inc/util.h:95).
 __host__ __device__ constexpr int c3x(int p)

@@ -78,6 +78,16 @@ struct Ctx {
     void *exc_val = nullptr;  // one value per entry
     int *exc_count = nullptr; // per level: [2L] count, [2L+1] overflow flag
 
+    // Triangle geometry: n_geo_tri triangles (9 floats each) binned on a G x G grid over the (y, z)
+    // bounding box; bin (iy, iz) lists geo_bin_tri[geo_bin_start[iy + G iz] ...].
+    float *geo_tri = nullptr;
+    int *geo_bin_start = nullptr, *geo_bin_tri = nullptr;
+    long long geo_n_tri = 0;
+    int geo_G = 0;
+    double geo_y0 = 0, geo_z0 = 0, geo_inv_dy = 0, geo_inv_dz = 0;
+    double *geo_part = nullptr;   // 3 doubles per finest-level block: force partials
+    double *geo_force = nullptr;  // 3 doubles, result of the force reduction
+
     LevelTiming timing;
 
     // One coarse step recorded as a HIP graph, per starting buffer parity of the levels.  A graph
@@ -141,5 +151,17 @@ int launch_ref_criteria(Ctx *c);
 int refine_coarsen(Ctx *c, int out_counts[4]);
 int rebuild_level_lists(Ctx *c, long long range_hint);
 int launch_fill(Ctx *c, int *p, long long n, int v);
+
+// Geometry (geometry.hip).
+int set_geometry(Ctx *c, const float *tri, long long n_tri);
+void free_geometry(Ctx *c);
+int voxelize_level(Ctx *c, int L);
+int near_wall_passes(double near_wall_distance, int L, double dx_L);
+int mark_near_wall(Ctx *c, int L);
+int check_masks_level(Ctx *c, int L);
+int compute_forces(Ctx *c, double out[3]);
+long long icosphere(double cx, double cy, double cz, double r, int subdiv, double noise, unsigned long long seed,
+                    float *out, long long cap);
+long long load_stl_ascii(const char *path, float *out, long long cap);
 
 }  // namespace agal

@@ -173,6 +173,7 @@ void free_ctx(Ctx *c)
                     c->exc_count};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    free_geometry(c);
     if (c->hscal) (void)hipHostFree(c->hscal);
     for (auto &g : c->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -425,6 +426,58 @@ int refine_with_solution_impl(Ctx *c, int out_counts[4])
     return refine_coarsen(c, out_counts);
 }
 
+// Mesh::M_Advance_RefineNearWall (src/mesh/mesh_advance.cu:53-121).  The benchmark-case hook
+// Cu_RefineRegion marks nothing as shipped (inc/custom.h:17-43).  With a geometry, each wall level
+// is voxelized and marked before its rebuild (M_Geometry_Voxelize_S1) and the boundary cells are
+// found once the hierarchy is complete (M_Geometry_Voxelize_S2).  A cancelled refinement (pool
+// exhausted) does not stop the sequence; its status is returned at the end.
+int refine_near_wall_impl(Ctx *c)
+{
+    int st, status = AGAL_OK;
+    const bool geom = c->geo_tri != nullptr;
+    const int ML = c->MAX_LEVELS;
+    auto keep = [&](int s) {
+        if (s == AGAL_ERR_POOL_EXHAUSTED) {
+            if (status == AGAL_OK) status = s;
+            return AGAL_OK;
+        }
+        return s;
+    };
+    for (int L = 0; L < c->N_LEVEL_START; L++) {
+        const int n = c->n_ids[L];
+        if (n > 0) {
+            hipLaunchKernelGGL(k_mark_uniform, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->id_set_L(L), n, c->ref);
+            AGAL_HIP(c, hipGetLastError());
+        }
+        if ((st = keep(refine_coarsen(c, nullptr)))) return st;
+    }
+    for (int L = c->N_LEVEL_START; L < c->MAX_LEVELS_WALL; L++) {
+        if (geom) {
+            if ((st = voxelize_level(c, L))) return st;
+            if ((st = mark_near_wall(c, L))) return st;
+        }
+        if (ML > 1 && L < c->MAX_LEVELS_WALL - 1)
+            if ((st = keep(refine_coarsen(c, nullptr)))) return st;
+        if ((st = launch_set_ic(c, L))) return st;
+        c->ic_done = true;
+    }
+    if (geom) {
+        for (int L = c->N_LEVEL_START; L < c->MAX_LEVELS_WALL; L++)
+            if ((st = check_masks_level(c, L))) return st;
+        // Solid cells and masks changed: participation and fast-path flags of the blocks.
+        if ((st = rebuild_level_lists(c, 0))) return st;
+    }
+    if (ML > 1) {
+        const int n = c->id_max[ML];
+        if (n > 0) {
+            hipLaunchKernelGGL(k_freeze, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->ref, n, 0);
+            AGAL_HIP(c, hipGetLastError());
+        }
+    }
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    return status;
+}
+
 int check_exceptions(Ctx *c)
 {
     int v[16] = {};
@@ -489,6 +542,7 @@ void agal_params_default(agal_params *p)
     p->s4 = 1.2;
     p->s5 = 1.2;
     p->s6 = 1.98;
+    p->G_NEAR_WALL_DISTANCE = 0.05;
 }
 
 const char *agal_status_string(int s)
@@ -769,6 +823,60 @@ int agal_refine_with_solution(agal_ctx *ctx, int out_counts[4])
 {
     CTX_OR_ARG(ctx);
     return refine_with_solution_impl(reinterpret_cast<Ctx *>(ctx), out_counts);
+}
+
+int agal_refine_near_wall(agal_ctx *ctx)
+{
+    CTX_OR_ARG(ctx);
+    return refine_near_wall_impl(reinterpret_cast<Ctx *>(ctx));
+}
+
+int agal_set_geometry(agal_ctx *ctx, const float *tri, long long n_tri)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    AGAL_HIP(c, hipSetDevice(c->device));
+    return set_geometry(c, tri, n_tri);
+}
+
+int agal_voxelize(agal_ctx *ctx, int L)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (L < 0 || L >= c->MAX_LEVELS) return AGAL_ERR_ARG;
+    int st;
+    if ((st = voxelize_level(c, L)) || (st = mark_near_wall(c, L))) return st;
+    return AGAL_OK;
+}
+
+int agal_check_masks(agal_ctx *ctx, int L)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (L < 0 || L >= c->MAX_LEVELS) return AGAL_ERR_ARG;
+    if (!c->geo_tri) return AGAL_ERR_STATE;
+    return check_masks_level(c, L);
+}
+
+int agal_forces(agal_ctx *ctx, double out[3])
+{
+    CTX_OR_ARG(ctx);
+    if (!out) return AGAL_ERR_ARG;
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c->ic_done) return AGAL_ERR_STATE;
+    return compute_forces(c, out);
+}
+
+long long agal_icosphere(double cx, double cy, double cz, double radius, int subdivisions, double noise,
+                         unsigned long long seed, float *out, long long capacity)
+{
+    return icosphere(cx, cy, cz, radius, subdivisions, noise, seed, out, capacity);
+}
+
+long long agal_load_stl(const char *path, float *out, long long capacity)
+{
+    if (!path) return -1;
+    return load_stl_ascii(path, out, capacity);
 }
 
 int agal_mark_uniform(agal_ctx *ctx, int L)

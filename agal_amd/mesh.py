@@ -111,6 +111,29 @@ class LbmMesh:
         a = np.ascontiguousarray(np.asarray(ids, dtype=np.int32))
         self._call("mark_blocks", np_ptr(a, C.c_int), int(a.size), int(mark))
 
+    def M_Advance_RefineNearWall(self):
+        """Returns the status (AGAL_OK or AGAL_ERR_POOL_EXHAUSTED)."""
+        st = self._call("refine_near_wall", ok=(capi.AGAL_OK, capi.AGAL_ERR_POOL_EXHAUSTED))
+        self.refresh_sizes()
+        return st
+
+    def set_geometry(self, tri: np.ndarray):
+        """Closed triangle surface, (n, 3, 3) or (n, 9) float32 in domain coordinates."""
+        t = np.ascontiguousarray(tri, dtype=np.float32).reshape(-1, 9)
+        self._call("set_geometry", np_ptr(t, C.c_float), int(t.shape[0]))
+
+    def M_Geometry_Voxelize_S1(self, L):
+        self._call("voxelize", int(L))
+
+    def M_Geometry_Voxelize_S2(self, L):
+        self._call("check_masks", int(L))
+
+    def S_ComputeForces(self) -> np.ndarray:
+        """Momentum-exchange force on the geometry after the last coarse step (x, y, z)."""
+        out = (C.c_double * 3)()
+        self._call("forces", out)
+        return np.array(list(out))
+
     def M_FreezeRefinedCells(self, var=0):
         self._call("freeze_refined", int(var))
 

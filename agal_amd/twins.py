@@ -54,7 +54,8 @@ def advance_twins(lead: LbmMesh, followers: list, i0: int, n: int) -> list:
 
 
 def velocity_error(ref: LbmMesh, other: LbmMesh) -> dict:
-    """Relative L2 velocity difference over the active leaf cells of two contexts on one mesh."""
+    """Relative L2 velocity difference over the active, non-solid leaf cells of two contexts on one
+    mesh."""
     s = ref.refresh_sizes()
     ML = s.max_levels
     for m in (ref, other):
@@ -67,7 +68,7 @@ def velocity_error(ref: LbmMesh, other: LbmMesh) -> dict:
         raise RuntimeError("the contexts do not share one mesh")
     first_child = ref.block_ints(capi.BLK_NBR_CHILD, nb)[0]
     leaf = (ref_ids != capi.REF_INACTIVE) & (first_child < 0)
-    cells = np.repeat(leaf, cpb)
+    cells = np.repeat(leaf, cpb) & (ref.cell_masks(nb) != capi.CELL_SOLID)
     a = ref.macros(nb)[1:1 + s.n_dim][:, cells].astype(np.float64)
     b = other.macros(nb)[1:1 + s.n_dim][:, cells].astype(np.float64)
     err = float(np.sqrt(np.sum((a - b) ** 2) / np.sum(a ** 2)))

@@ -123,6 +123,8 @@ typedef struct agal_params {
     double ic_rho, ic_u, ic_v, ic_w; /* Cu_ComputeIC hook: uniform state (custom.h:48-60)         */
     double s1, s2, s3, s4, s5, s6;   /* MRT relaxation rates of the non-hydrodynamic moments
                                         (input.txt keys s1..s6, solver_lbm_init.cu:55-61)       */
+    double G_NEAR_WALL_DISTANCE;  /* width of the refined band around a geometry on level 0
+                                     (parser.h:121; halved per level, mesh_voxelizer.cu:39-41) */
 } agal_params;
 
 typedef struct agal_ctx agal_ctx;
@@ -191,6 +193,40 @@ int  agal_freeze_refined(agal_ctx *ctx, int var);
  * M_Advance_RefineWithSolution -- the body of M_AdvanceLoop (mesh_advance.cu:421-463) for
  * coarse steps [i0, i0+n).  Rebuild time is reported separately in out_rebuild_ms (may be NULL). */
 int  agal_advance_loop(agal_ctx *ctx, int i0, int n, double *out_step_ms, double *out_rebuild_ms);
+/* Mesh::M_Advance_RefineNearWall (src/mesh/mesh_advance.cu:53-121): uniform refinement up to
+ * N_LEVEL_START, then per wall level the geometry's voxelization and block marks (when a geometry
+ * is set), the rebuild, and the initial condition of the level; then boundary cells and the freeze
+ * of the refined blocks.  Returns AGAL_ERR_POOL_EXHAUSTED if a rebuild was cancelled. */
+int  agal_refine_near_wall(agal_ctx *ctx);
+
+/* ---- geometry (3D velocity sets) -------------------------------------------------------------
+ * A closed triangle surface, 9 floats per triangle (x1 y1 z1 x2 y2 z2 x3 y3 z3), in domain
+ * coordinates.  Replaces Geometry::G_Init_Arrays_CoordsList_CPU + G_MakeBinsGPU
+ * (src/geometry/geometry_init.cu:79-212, geometry_bins_gpu.cu:586-1040): the library copies the
+ * triangles and bins them on a (y, z) grid for the voxelizer's rays. */
+int  agal_set_geometry(agal_ctx *ctx, const float *tri, long long n_tri);
+/* Mesh::M_Geometry_Voxelize_S1 (src/mesh/mesh_voxelizer.cu:33-136) on level L: a cell is SOLID when
+ * the +x ray from its centre crosses the surface an odd number of times; blocks holding one get
+ * the SOLID block mask; then the boundary / exterior / near-wall-band refinement marks with
+ * int(R/(4 sqrt2 dx_L))+1 propagation passes, R = G_NEAR_WALL_DISTANCE / 2^L. */
+int  agal_voxelize(agal_ctx *ctx, int L);
+/* Mesh::M_Geometry_Voxelize_S2 (mesh_voxelizer.cu:139-174) on level L: BOUNDARY cell masks (non-
+ * solid cells with a solid neighbour) in leaf SOLIDB / SOLIDA blocks. */
+int  agal_check_masks(agal_ctx *ctx, int L);
+/* Solver_LBM::S_ComputeForcesMEA (order 1) + S_ReportForces (solver_lbm_kernels_forces.cu:322-540,
+ * solver_lbm_compute.cu:165-218), evaluated after a coarse step on the finest level: out = force on
+ * the body, (sum over BOUNDARY cells x and links p into a solid cell of 2 f*_p c_p) dx^3 / dx with
+ * dx the finest spacing, accumulated in FP64.  Drag / lift coefficients are left to the caller. */
+int  agal_forces(agal_ctx *ctx, double out[3]);
+/* Host helpers, no context.  Icosphere: the icosahedron subdivided `subdivisions` times (20*4^s
+ * triangles), each vertex at radius r (1 + noise u), u uniform in [-1, 1) from a splitmix64
+ * stream seeded with `seed`, in vertex order.  ASCII STL: the facets of the file (the format of
+ * the reference's geometry inputs).  Both return the triangle count (out may be NULL to query
+ * it) or a negative value on error (-1 bad argument / unreadable file, -2 capacity too small,
+ * -3 malformed file). */
+long long agal_icosphere(double cx, double cy, double cz, double radius, int subdivisions, double noise,
+                         unsigned long long seed, float *out, long long capacity);
+long long agal_load_stl(const char *path, float *out, long long capacity);
 
 /* ---- data exchange (M_RetrieveFromGPU / M_LoadToGPU, src/mesh/mesh_output.cu:4-82) ---------
  * Arrays are copied for block IDs [0, n_blocks).  Element layouts follow the reference's SoA:

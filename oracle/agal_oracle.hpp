@@ -48,9 +48,10 @@ constexpr int N_SKIPID = -999;
 
 constexpr int REF_UNREFINED = 0, REF_UNREFINED_VIO = 9, REF_REFINED = 1, REF_REFINED_WCHILD = 2,
               REF_REFINED_PERM = 3, REF_REFINED_WCHILD_PERM = 10, REF_MARK_REFINE = 4,
-              REF_MARK_COARSEN = 5, REF_NEW = 6, REF_REMOVE = 7, REF_INACTIVE = 8;
-constexpr int CELL_INTERIOR = 0, CELL_INTERFACE = 1, CELL_GHOST = 2, CELL_SOLID = -1;
-constexpr int BLK_REGULAR = 0, BLK_INTERFACE = 1, BLK_SOLID = -3, BLK_SOLIDA = -2;
+              REF_MARK_COARSEN = 5, REF_NEW = 6, REF_REMOVE = 7, REF_INACTIVE = 8, REF_INDET_E = 11,
+              REF_INDET_O = 12;
+constexpr int CELL_INTERIOR = 0, CELL_INTERFACE = 1, CELL_GHOST = 2, CELL_SOLID = -1, CELL_BOUNDARY = -6;
+constexpr int BLK_REGULAR = 0, BLK_INTERFACE = 1, BLK_SOLID = -3, BLK_SOLIDA = -2, BLK_SOLIDB = -1;
 constexpr int INTERP_INTERFACE = 0, INTERP_ADDED = 1;
 constexpr int AVERAGE_INTERFACE = 0, AVERAGE_BLOCK = 1, AVERAGE_GRID = 2;
 
@@ -1475,18 +1476,236 @@ struct Oracle {
 
     // Mesh::M_Advance_RefineNearWall (src/mesh/mesh_advance.cu:53-121) without geometry: the
     // near-wall hook Cu_RefineRegion returns false (inc/custom.h:41).
+    // ---------------------------------------------------------------------------------------
+    // Geometry.  The solid is a closed triangle surface; a cell is solid when the ray from its
+    // centre along +x crosses the surface an odd number of times.  This classification stands in
+    // for the reference's nearest-face classification and x-sweep fill (Cu_Voxelize_V1,
+    // mesh_voxelizer_voxelize.cu:202-352; Cu_Voxelize_Propagate_*, mesh_voxelizer_propagate.cu:
+    // 93-252): both classify cell centres as inside or outside the closed surface.  Here every
+    // triangle is tested (no bins), so the binned GPU voxelizer is checked against it.
+    // ---------------------------------------------------------------------------------------
+    std::vector<float> geo_tri;  // 9 floats per triangle
+
+    static bool edge_owns(double ey, double ez) { return ez < 0.0 || (ez == 0.0 && ey > 0.0); }
+
+    // Crossing of the ray (p + s e_x, s > 0) with triangle t: the point's (y, z) projection lies in
+    // the triangle's (counter-clockwise) projection, points on an edge belonging to the triangle
+    // when the edge points down or, if level, to +y; then the plane's x there exceeds p.x.
+    static int ray_crosses(const float *t, double px, double py, double pz)
+    {
+        double ax = t[0], ay = t[1], az = t[2];
+        double bx = t[3], by = t[4], bz = t[5];
+        double cx = t[6], cy = t[7], cz = t[8];
+        const double area = (by - ay) * (cz - az) - (bz - az) * (cy - ay);
+        if (area == 0.0) return 0;
+        if (area < 0.0) {
+            std::swap(bx, cx);
+            std::swap(by, cy);
+            std::swap(bz, cz);
+        }
+        const double d1 = (by - ay) * (pz - az) - (bz - az) * (py - ay);
+        const double d2 = (cy - by) * (pz - bz) - (cz - bz) * (py - by);
+        const double d3 = (ay - cy) * (pz - cz) - (az - cz) * (py - cy);
+        const bool in1 = d1 > 0.0 || (d1 == 0.0 && edge_owns(by - ay, bz - az));
+        const bool in2 = d2 > 0.0 || (d2 == 0.0 && edge_owns(cy - by, cz - bz));
+        const bool in3 = d3 > 0.0 || (d3 == 0.0 && edge_owns(ay - cy, az - cz));
+        if (!(in1 && in2 && in3)) return 0;
+        const double ny = (bz - az) * (cx - ax) - (bx - ax) * (cz - az);
+        const double nz = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
+        const double nx = (by - ay) * (cz - az) - (bz - az) * (cy - ay);
+        const double xh = ax - (ny * (py - ay) + nz * (pz - az)) / nx;
+        return xh > px ? 1 : 0;
+    }
+
+    void set_geometry(const float *tri, long n_tri) { geo_tri.assign(tri, tri + 9 * n_tri); }
+
+    // Solid cells of level L, then Cu_Voxelize_UpdateMasks (mesh_voxelizer_masks.cu:269-304): a
+    // block holding a solid cell gets the SOLID block mask.
+    void voxelize(int L)
+    {
+        const long nt = (long)geo_tri.size() / 9;
+        const int *ids = idset(L);
+        const double dxL = (double)dxf_vec[L];
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int bi = 0; bi < n_ids[L]; bi++) {
+            const int b = ids[bi];
+            bool any = false;
+            for (int t = 0; t < M_CBLOCK; t++) {
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                const double px = (double)X[b] + ((double)I + 0.5) * dxL;
+                const double py = (double)X[b + n_maxcblocks] + ((double)J + 0.5) * dxL;
+                const double pz = (double)X[b + 2 * n_maxcblocks] + ((double)K + 0.5) * dxL;
+                int count = 0;
+                for (long k = 0; k < nt; k++) count += ray_crosses(&geo_tri[9 * k], px, py, pz);
+                const long cell = (long)b * M_CBLOCK + t;
+                if (count & 1) cells_mask[cell] = CELL_SOLID;
+                if (cells_mask[cell] == CELL_SOLID) any = true;
+            }
+            if (any) blk_mask[b] = BLK_SOLID;
+        }
+    }
+
+    bool any_nbr_skip(int b) const
+    {
+        for (int p = 0; p < Q_max; p++)
+            if (nbr[b + (size_t)p * n_maxcblocks] == N_SKIPID) return true;
+        return false;
+    }
+
+    // The block marks of M_Geometry_Voxelize_S1 (mesh_voxelizer.cu:82-125) with the kernels
+    // Cu_MarkBlocks_MarkBoundary / MarkExterior / Propagate / Finalize (mesh_voxelizer_refine.cu:
+    // 23-206, 278-414).  Each kernel reads only values it does not write, so sequential loops
+    // give the parallel result.
+    void mark_near_wall(int L)
+    {
+        const int idm = id_max[L];
+        if (n_ids[L] <= 0 || idm <= 0) return;
+        const bool hit_max = L == MAX_LEVELS - 1;
+        const float R = (float)P.G_NEAR_WALL_DISTANCE / (float)std::pow(2.0, (double)L);
+        const int nprop = (int)(R / (float)(4.0 * std::sqrt(2.0) * (double)dxf_vec[L])) + 1;
+        std::vector<int> tmp(idm, -1);
+        auto nb_of = [&](int b, int p) { return nbr[b + (size_t)p * n_maxcblocks]; };
+        for (int k = 0; k < idm; k++) {  // MarkBoundary
+            if (level[k] != L || blk_mask[k] != BLK_SOLID) continue;
+            bool bnd = false;
+            for (int p = 0; p < Q_max; p++) {
+                const int ip = nb_of(k, p);
+                if (ip > -1 && blk_mask[ip] == BLK_REGULAR) bnd = true;
+            }
+            if (!bnd || ref[k] != REF_UNREFINED) continue;
+            blk_mask[k] = BLK_SOLIDB;
+            if (!hit_max && !any_nbr_skip(k)) ref[k] = REF_MARK_REFINE;
+        }
+        for (int k = 0; k < idm; k++) {  // MarkExterior
+            if (level[k] != L) continue;
+            bool near = false;
+            for (int p = 0; p < Q_max; p++) {
+                const int ip = nb_of(k, p);
+                if (ip > -1 && blk_mask[ip] == BLK_SOLIDB) near = true;
+            }
+            if (!near || ref[k] != REF_UNREFINED) continue;
+            if (!hit_max && !any_nbr_skip(k)) ref[k] = REF_INDET_E;
+            if (blk_mask[k] == BLK_REGULAR) blk_mask[k] = BLK_SOLIDA;
+        }
+        for (int j = 0; j < nprop; j++) {  // Propagate: even passes ref -> tmp, odd passes tmp -> ref
+            const bool even = j % 2 == 0;
+            for (int k = 0; k < idm; k++) {
+                if (level[k] != L) continue;
+                const int m = blk_mask[k];
+                if (!(m > -1 || (j == 0 && m == BLK_SOLID))) continue;
+                if (even ? ref[k] != REF_UNREFINED : tmp[k] != -1) continue;
+                bool mark = false;
+                for (int p = 0; p < Q_max; p++) {
+                    const int ip = nb_of(k, p);
+                    if (ip > -1 && (even ? ref[ip] == REF_INDET_E : tmp[ip] == REF_INDET_O)) mark = true;
+                }
+                if (!mark || any_nbr_skip(k)) continue;
+                if (even)
+                    tmp[k] = REF_INDET_O;
+                else
+                    ref[k] = REF_INDET_E;
+            }
+        }
+        for (int k = 0; k < idm; k++)  // Finalize
+            if (ref[k] == REF_INDET_E || tmp[k] == REF_INDET_O) ref[k] = REF_MARK_REFINE;
+    }
+
+    // Cu_MarkBlocks_CheckMasks_S1/S2 (mesh_voxelizer_masks.cu:21-226): non-solid cells of SOLIDB /
+    // SOLIDA leaf blocks of level L with a solid neighbour cell become BOUNDARY cells; their
+    // blocks are flagged as on a boundary.
+    void check_masks(int L)
+    {
+        const int *ids = idset(L);
+        for (int bi = 0; bi < n_ids[L]; bi++) {
+            const int b = ids[bi];
+            const int bm = blk_mask[b];
+            if (nbr_child[b] >= 0 || !(bm < 0 && bm != BLK_SOLID)) continue;
+            int s_nbr[27];
+            load_block_nbrs(b, s_nbr);
+            bool any = false;
+            for (int t = 0; t < M_CBLOCK; t++) {
+                const long cell = (long)b * M_CBLOCK + t;
+                if (cells_mask[cell] == CELL_SOLID) continue;
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                bool near = false;
+                for (int o = 0; o < 27; o++) {
+                    if (o == 13) continue;  // the cell itself
+                    const int Ip = I + o % 3 - 1, Jp = J + (o / 3) % 3 - 1, Kp = K + o / 9 - 1;
+                    const int nb = s_nbr[nbr_map(DIM, Ip, Jp, Kp)];
+                    if (nb >= 0 && cells_mask[(long)nb * M_CBLOCK + nbr_cell_id(DIM, Ip, Jp, Kp)] == CELL_SOLID)
+                        near = true;
+                }
+                if (near) {
+                    cells_mask[cell] = CELL_BOUNDARY;
+                    any = true;
+                }
+            }
+            if (any) onb[b] = 1;
+        }
+    }
+
+    // Mesh::M_Advance_RefineNearWall (src/mesh/mesh_advance.cu:53-121).  The benchmark-case hook
+    // Cu_RefineRegion marks nothing as shipped (inc/custom.h:17-43); with a geometry each wall
+    // level is voxelized and marked before its rebuild, and the boundary cells are found once the
+    // hierarchy is complete.
     int refine_near_wall()
     {
+        const bool geom = !geo_tri.empty();
+        int status = AGAL_OK;
+        auto keep = [&](int s) {
+            if (s != AGAL_OK && status == AGAL_OK) status = s;
+        };
         for (int L = 0; L < N_LEVEL_START; L++) {
             mark_uniform(L);
-            refine_coarsen(nullptr);
+            keep(refine_coarsen(nullptr));
         }
         for (int L = N_LEVEL_START; L < MAX_LEVELS_WALL; L++) {
-            if (MAX_LEVELS > 1 && L < MAX_LEVELS_WALL - 1) refine_coarsen(nullptr);
+            if (geom) {
+                voxelize(L);
+                mark_near_wall(L);
+            }
+            if (MAX_LEVELS > 1 && L < MAX_LEVELS_WALL - 1) keep(refine_coarsen(nullptr));
             set_ic(L);
         }
+        if (geom)
+            for (int L = N_LEVEL_START; L < MAX_LEVELS_WALL; L++) check_masks(L);
         if (MAX_LEVELS > 1) freeze_refined(0);
-        return AGAL_OK;
+        return status;
+    }
+
+    // Momentum-exchange force on the body (Cu_ComputeForcesMEA order 1, solver_lbm_kernels_forces.
+    // cu:322-540, reported as in S_ReportForces, solver_lbm_compute.cu:165-218), evaluated after a
+    // coarse step: each link p from a BOUNDARY cell of the finest level into a solid cell adds
+    // 2 f*_p c_p, f*_p being the value the link bounced back (now in slot p); the sum times the cell
+    // volume dx^3 and 1/dx of the finest level.  Accumulated in FP64.
+    void forces(double out[3])
+    {
+        out[0] = out[1] = out[2] = 0.0;
+        const int L = MAX_LEVELS - 1;
+        const int *ids = idset(L);
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int bi = 0; bi < n_ids[L]; bi++) {
+            const int b = ids[bi];
+            int s_nbr[27];
+            load_block_nbrs(b, s_nbr);
+            for (int t = 0; t < M_CBLOCK; t++) {
+                const long cell = (long)b * M_CBLOCK + t;
+                if (cells_mask[cell] != CELL_BOUNDARY) continue;
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                for (int p = 1; p < Q; p++) {
+                    const int Ip = I + V.c[0][p], Jp = J + V.c[1][p], Kp = K + V.c[2][p];
+                    const int nb = s_nbr[nbr_map(DIM, Ip, Jp, Kp)];
+                    if (nb < 0 || cells_mask[(long)nb * M_CBLOCK + nbr_cell_id(DIM, Ip, Jp, Kp)] != CELL_SOLID) continue;
+                    const double f = (double)Fp(p)[cell];
+                    for (int d = 0; d < 3; d++) acc[d] += 2.0 * f * (double)V.c[d][p];
+                }
+            }
+        }
+        const double dxf = (double)dxf_vec[L];
+        for (int d = 0; d < 3; d++) out[d] = acc[d] * (dxf * dxf * dxf) / dxf;
     }
 
     // The body of Mesh::M_AdvanceLoop (src/mesh/mesh_advance.cu:421-463) for coarse steps [i0,i0+n).

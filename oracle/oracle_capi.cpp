@@ -125,6 +125,28 @@ int oracle_ref_criteria(void *h) { return dispatch(h, [&](auto &O) { O.ref_crite
 int oracle_refine_coarsen(void *h, int *counts) { return dispatch(h, [&](auto &O) { return O.refine_coarsen(counts); }); }
 int oracle_refine_with_solution(void *h, int *counts) { return dispatch(h, [&](auto &O) { return O.refine_with_solution(counts); }); }
 int oracle_refine_near_wall(void *h) { return dispatch(h, [&](auto &O) { return O.refine_near_wall(); }); }
+int oracle_set_geometry(void *h, const float *tri, long long n_tri)
+{
+    if (!tri || n_tri <= 0) return AGAL_ERR_ARG;
+    return dispatch(h, [&](auto &O) { O.set_geometry(tri, (long)n_tri); return AGAL_OK; });
+}
+int oracle_voxelize(void *h, int L)
+{
+    return dispatch(h, [&](auto &O) {
+        O.voxelize(L);
+        O.mark_near_wall(L);
+        return AGAL_OK;
+    });
+}
+int oracle_check_masks(void *h, int L) { return dispatch(h, [&](auto &O) { O.check_masks(L); return AGAL_OK; }); }
+int oracle_forces(void *h, double *out) { return dispatch(h, [&](auto &O) { O.forces(out); return AGAL_OK; }); }
+/* Crossing count of the +x ray from (px, py, pz) with n_tri triangles (the voxelizer's predicate). */
+long long oracle_ray_count(const float *tri, long long n_tri, double px, double py, double pz)
+{
+    long long c = 0;
+    for (long long k = 0; k < n_tri; k++) c += Oracle<double>::ray_crosses(tri + 9 * k, px, py, pz);
+    return c;
+}
 int oracle_mark_uniform(void *h, int L) { return dispatch(h, [&](auto &O) { O.mark_uniform(L); return AGAL_OK; }); }
 int oracle_freeze_refined(void *h, int var) { return dispatch(h, [&](auto &O) { O.freeze_refined(var); return AGAL_OK; }); }
 

@@ -29,7 +29,7 @@ def oracle_library():
             "collide": (C.c_int, [C.c_void_p, C.c_int]),
             "impose_bc": (C.c_int, [C.c_void_p, C.c_int]),
             "stream": (C.c_int, [C.c_void_p, C.c_int]),
-            "refine_near_wall": (C.c_int, [C.c_void_p]),
+            "ray_count": (C.c_longlong, [C.POINTER(C.c_float), C.c_longlong, C.c_double, C.c_double, C.c_double]),
             "upload_cell_masks": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_longlong]),
             "get_level_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
             "velset": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -63,9 +63,6 @@ class OracleMesh(LbmMesh):
 
     def stream(self, L):
         self._call_raw("stream", L)
-
-    def refine_near_wall(self):
-        self._call_raw("refine_near_wall")
 
     def level_params(self):
         n = self.sizes.max_levels

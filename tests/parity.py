@@ -18,6 +18,23 @@ def ldc_params(vs=capi.D3Q19, N=16, Re=50.0, U=0.05, levels=1, prec=capi.FP32, *
     return capi.default_params(**base)
 
 
+def sphere_params(vs=capi.D3Q27, N=16, levels=3, Re=200.0, U=0.04, D=0.16, prec=capi.FP32,
+                  collision=capi.MRT, **kw):
+    """External flow past a body of diameter D (domain units): inflow / moving side walls at U and
+    an outflow face (the FPSC3D hook), uniform initial flow, nu = U*D/Re, the wall levels refined
+    near the body."""
+    base = dict(velocity_set=vs, precision=prec, collision=collision, Nx=N, MAX_LEVELS_WALL=levels,
+                MAX_LEVELS_INTERIOR=1, v0=U * D / Re, bc_preset=capi.BC_FPSC3D, bc_velocity=U, ic_u=U,
+                P_REFINE=1000)
+    base.update(kw)
+    return capi.default_params(**base)
+
+
+def body(center=(0.3, 0.5, 0.5), radius=0.08, subdivisions=3, noise=0.05, seed=1):
+    """The seeded noisy icosphere, (n, 3, 3) float32."""
+    return capi.icosphere(center, radius, subdivisions, noise, seed)
+
+
 def active_mask(state):
     return state["ref"] != capi.REF_INACTIVE
 

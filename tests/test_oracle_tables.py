@@ -93,7 +93,7 @@ def test_parameter_defaults_match_reference_parser():
               "PERIODIC_Z", "S_CRITERION", "P_REFINE"):
         assert getattr(p, k) == ints[k], k
     for k in ("L_c", "L_fy", "L_fz", "v0", "N_REFINE_START", "N_REFINE_INC", "N_REFINE_MAX",
-              "s1", "s2", "s3", "s4", "s5", "s6"):
+              "s1", "s2", "s3", "s4", "s5", "s6", "G_NEAR_WALL_DISTANCE"):
         assert getattr(p, k) == dbls[k], k
 
 
