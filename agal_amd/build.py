@@ -7,7 +7,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
-SOURCES = ["capi.hip", "lbm_kernels.hip", "amr_kernels.hip", "geometry.hip"]
+SOURCES = ["capi.hip", "lbm_kernels.hip", "amr_kernels.hip", "geometry.hip", "predictor.hip"]
 
 # -ffp-contract=off: no FMA contraction, so FP32/FP64 results are reproducible operation for
 # operation (the refinement flags depend on them).  The packed-FP32 feature is disabled for all

@@ -80,6 +80,43 @@ class Params(C.Structure):
     ]
 
 
+class LearnOpts(C.Structure):
+    _fields_ = [
+        ("train_steps", C.c_int),
+        ("infer_every", C.c_int),
+        ("batch", C.c_int),
+        ("adam_steps", C.c_int),
+        ("lr", C.c_double),
+        ("threshold", C.c_double),
+        ("pos_weight", C.c_double),
+        ("seed", C.c_ulonglong),
+        ("use_predictions", C.c_int),
+    ]
+
+
+class LearnStats(C.Structure):
+    _fields_ = [
+        ("flagged", C.c_longlong),
+        ("missed", C.c_longlong),
+        ("flagged_train", C.c_longlong),
+        ("missed_train", C.c_longlong),
+        ("predicted_refinements", C.c_longlong),
+        ("adam_steps_done", C.c_int),
+        ("rebuilds", C.c_int),
+        ("early_rebuilds", C.c_int),
+        ("last_loss", C.c_double),
+        ("step_ms", C.c_double),
+        ("rebuild_ms", C.c_double),
+        ("predictor_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+MLP_NFEAT, MLP_NPARAM = 12, 5057
+
+
 class Sizes(C.Structure):
     _fields_ = [
         ("n_dim", C.c_int),
@@ -173,6 +210,15 @@ PROTOTYPES = {
     "voxelize": (_I, [_P, _I]),
     "check_masks": (_I, [_P, _I]),
     "forces": (_I, [_P, C.POINTER(C.c_double)]),
+    "predictor_init": (_I, [_P, C.c_ulonglong, _I]),
+    "predictor_set": (_I, [_P, C.POINTER(C.c_float)]),
+    "predictor_get": (_I, [_P, _I, C.POINTER(C.c_float)]),
+    "predictor_forward": (_I, [_P, C.POINTER(C.c_float), _I, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "predictor_train_step": (_I, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), _I, C.c_double, C.c_double,
+                                  C.POINTER(C.c_double)]),
+    "predictor_features": (_I, [_P, C.POINTER(C.c_float), _LL]),
+    "criterion_labels": (_I, [_P, C.POINTER(_I), _LL]),
+    "advance_loop_learned": (_I, [_P, _I, _I, C.POINTER(LearnOpts), C.POINTER(LearnStats)]),
 }
 
 # Host helpers without a context.
@@ -180,6 +226,7 @@ HOST_PROTOTYPES = {
     "icosphere": (_LL, [C.c_double, C.c_double, C.c_double, C.c_double, _I, C.c_double, C.c_ulonglong,
                         C.POINTER(C.c_float), _LL]),
     "load_stl": (_LL, [C.c_char_p, C.POINTER(C.c_float), _LL]),
+    "learn_opts_default": (None, [C.POINTER(LearnOpts)]),
 }
 
 # Every symbol include/agal_capi.h declares (checked by tests/test_capi_symbols.py).
@@ -255,3 +302,14 @@ def load_stl(path: str) -> np.ndarray:
     if lib.agal_load_stl(path.encode(), np_ptr(out, C.c_float), n) != n:
         raise AgalError(AGAL_ERR_ARG, f"agal_load_stl({path})")
     return out
+
+
+def learn_opts(**kw) -> LearnOpts:
+    """agal_learn_opts_default() with the given fields replaced."""
+    o = LearnOpts()
+    load_library().agal_learn_opts_default(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o

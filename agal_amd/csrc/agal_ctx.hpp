@@ -15,12 +15,34 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <string>
 #include <vector>
 
 #include "../../include/agal_capi.h"
 
 namespace agal {
+
+// Learned refinement predictor: 12 features per block, a 12-64-64-1 network (flat parameter
+// layout W1[12][64], b1[64], W2[64][64], b2[64], W3[64], b3).
+constexpr int NFEAT = 12, MLP_H = 64;
+constexpr int MLP_OFF_W1 = 0, MLP_OFF_B1 = NFEAT * MLP_H, MLP_OFF_W2 = MLP_OFF_B1 + MLP_H,
+              MLP_OFF_B2 = MLP_OFF_W2 + MLP_H * MLP_H, MLP_OFF_W3 = MLP_OFF_B2 + MLP_H, MLP_OFF_B3 = MLP_OFF_W3 + MLP_H,
+              MLP_NPARAM = MLP_OFF_B3 + 1;
+
+struct Predictor {
+    float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;  // master weights, Adam moments, gradient
+    _Float16 *wh = nullptr;                                         // FP16 copy used by every pass
+    int t = 0;                                                      // Adam steps taken
+    float *stats = nullptr, *dist = nullptr, *feat = nullptr, *prob = nullptr;  // per block ID
+    int *label = nullptr, *counts = nullptr;
+    float *snap = nullptr;  // feature snapshots, snap_slots x n_maxcblocks x NFEAT
+    int snap_slots = 0;
+    int batch_cap = 0;      // rows of the training buffers below
+    float *X = nullptr, *Y = nullptr, *Z1 = nullptr, *Z2 = nullptr, *z3 = nullptr, *dZ3 = nullptr, *dZ2 = nullptr,
+          *dZ1 = nullptr, *loss = nullptr;
+    int *ids = nullptr;
+};
 
 struct LevelTiming {
     bool enabled = false;
@@ -88,6 +110,8 @@ struct Ctx {
     double *geo_part = nullptr;   // 3 doubles per finest-level block: force partials
     double *geo_force = nullptr;  // 3 doubles, result of the force reduction
 
+    Predictor pred;
+
     LevelTiming timing;
 
     // One coarse step recorded as a HIP graph, per starting buffer parity of the levels.  A graph
@@ -147,7 +171,9 @@ int launch_interpolate(Ctx *c, int L, int var);
 int launch_average(Ctx *c, int L, int var);
 
 // Mesh kernels and the rebuild (amr_kernels.hip).
-int launch_ref_criteria(Ctx *c);
+// `label` (may be null): per block ID, 1 when the criterion wants the block to have children, 0
+// when not, -1 for inactive IDs (the learned predictor's training target).
+int launch_ref_criteria(Ctx *c, int *label = nullptr);
 int refine_coarsen(Ctx *c, int out_counts[4]);
 int rebuild_level_lists(Ctx *c, long long range_hint);
 int launch_fill(Ctx *c, int *p, long long n, int v);
@@ -163,5 +189,22 @@ int compute_forces(Ctx *c, double out[3]);
 long long icosphere(double cx, double cy, double cz, double r, int subdiv, double noise, unsigned long long seed,
                     float *out, long long cap);
 long long load_stl_ascii(const char *path, float *out, long long cap);
+
+// Learned refinement predictor (predictor.hip).
+void free_predictor(Ctx *c);
+void predictor_init_weights(unsigned long long seed, float *w);
+int predictor_setup(Ctx *c, unsigned long long seed, int batch_cap);
+int predictor_set_weights(Ctx *c, const float *w_host);
+int predictor_get(Ctx *c, int which, float *out);
+int predictor_forward_dev(Ctx *c, const float *X, int n, float *z3, float *p, float *Z1, float *Z2);
+int predictor_train_dev(Ctx *c, int n, float lr, float pos_weight, double *loss);
+int predictor_features(Ctx *c);
+int predictor_predict_blocks(Ctx *c);
+int predictor_mark(Ctx *c, float thr, int *n_marked);
+int predictor_keep(Ctx *c, float thr);
+int predictor_judge(Ctx *c, int out[2]);
+int predictor_snapshot(Ctx *c, int slot, int max_slots);
+int predictor_train_snapshots(Ctx *c, int n_snap, int batch, int steps, float lr, float pos_weight, uint64_t &rng,
+                              double *last_loss);
 
 }  // namespace agal

@@ -501,7 +501,8 @@ template <typename T, int DIM>
 __global__ __launch_bounds__(AWG) void k_criterion(int id_max, long long ncm, const T *__restrict__ M,
                                                    int *ref, const int *__restrict__ level,
                                                    const int *__restrict__ nbr, const int *__restrict__ cells_mask,
-                                                   long long nmax, T dx_L, int crit, T NRS, T NRI, T NRM, int MLI)
+                                                   long long nmax, T dx_L, int crit, T NRS, T NRI, T NRM, int MLI,
+                                                   int *__restrict__ label)
 {
     constexpr int CELLS = DIM == 3 ? 64 : 16, QMAX = DIM == 3 ? 27 : 9, HB = DIM == 3 ? 216 : 36;
     constexpr int BPG = AWG / CELLS;
@@ -585,11 +586,15 @@ __global__ __launch_bounds__(AWG) void k_criterion(int id_max, long long ncm, co
         if (eligible && level_kap != MLI - 1 && ref_kap == REF_UNREFINED && L_desired > level_kap)
             ref[b] = REF_MARK_REFINE;
         if (ref_kap == REF_REFINED && L_desired < level_kap + 1) ref[b] = REF_MARK_COARSEN;
+        // Label of the learned predictor: the criterion wants this block to have children.
+        if (label) label[b] = (eligible && level_kap != MLI - 1 && L_desired > level_kap) ? 1 : 0;
+    } else if (label && b < id_max && t == 0) {
+        label[b] = -1;  // inactive ID
     }
 }
 
 template <typename T>
-static int criterion_t(Ctx *c)
+static int criterion_t(Ctx *c, int *label)
 {
     int idm = c->id_max[c->MAX_LEVELS];
     if (idm <= 0) return AGAL_OK;
@@ -600,20 +605,20 @@ static int criterion_t(Ctx *c)
         hipLaunchKernelGGL((k_criterion<T, 3>), dim3(grid), dim3(AWG), 0, c->stream, idm, c->n_maxcells,
                            static_cast<const T *>(c->M), c->ref, c->level, c->nbr, c->cells_mask, c->n_maxcblocks, dx0,
                            c->P.S_CRITERION, (T)c->P.N_REFINE_START, (T)c->P.N_REFINE_INC, (T)c->P.N_REFINE_MAX,
-                           c->MAX_LEVELS_INTERIOR);
+                           c->MAX_LEVELS_INTERIOR, label);
     else
         hipLaunchKernelGGL((k_criterion<T, 2>), dim3(grid), dim3(AWG), 0, c->stream, idm, c->n_maxcells,
                            static_cast<const T *>(c->M), c->ref, c->level, c->nbr, c->cells_mask, c->n_maxcblocks, dx0,
                            c->P.S_CRITERION, (T)c->P.N_REFINE_START, (T)c->P.N_REFINE_INC, (T)c->P.N_REFINE_MAX,
-                           c->MAX_LEVELS_INTERIOR);
+                           c->MAX_LEVELS_INTERIOR, label);
     AGAL_HIP(c, hipGetLastError());
     return AGAL_OK;
 }
 
-int launch_ref_criteria(Ctx *c)
+int launch_ref_criteria(Ctx *c, int *label)
 {
-    if (c->precision == AGAL_FP64) return criterion_t<double>(c);
-    return criterion_t<float>(c);
+    if (c->precision == AGAL_FP64) return criterion_t<double>(c, label);
+    return criterion_t<float>(c, label);
 }
 
 // ---------------------------------------------------------------------------------------------

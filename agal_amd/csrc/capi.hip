@@ -174,6 +174,7 @@ void free_ctx(Ctx *c)
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_geometry(c);
+    free_predictor(c);
     if (c->hscal) (void)hipHostFree(c->hscal);
     for (auto &g : c->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -405,15 +406,14 @@ int step_once(Ctx *c, int L0)
     return AGAL_OK;
 }
 
-int refine_with_solution_impl(Ctx *c, int out_counts[4])
+int check_exceptions(Ctx *c);
+
+// The field updates of M_Advance_RefineWithSolution before the criterion
+// (src/mesh/mesh_advance.cu:204-217): the reference's S_Average / S_Interpolate refresh the
+// macroscopic fields first (solver_lbm_kernels_comm.cu:143,354); the criterion reads them.
+int prepare_fields(Ctx *c)
 {
     int st;
-    if (!(c->MAX_LEVELS_INTERIOR > 1)) {
-        if (out_counts) std::fill(out_counts, out_counts + 4, 0);
-        return AGAL_OK;
-    }
-    // The reference's S_Average / S_Interpolate refresh the macroscopic fields first
-    // (solver_lbm_kernels_comm.cu:143,354); the criterion reads those stored fields.
     for (int L = c->MAX_LEVELS - 2; L >= c->N_LEVEL_START; L--) {
         if ((st = launch_refresh(c, L + 1))) return st;
         if ((st = launch_average(c, L, AGAL_AVERAGE_GRID))) return st;
@@ -422,8 +422,136 @@ int refine_with_solution_impl(Ctx *c, int out_counts[4])
         if ((st = launch_refresh(c, L))) return st;
         if ((st = launch_interpolate(c, L, AGAL_INTERP_INTERFACE))) return st;
     }
+    if (c->MAX_LEVELS == 1 && (st = launch_refresh(c, 0))) return st;
+    return AGAL_OK;
+}
+
+int refine_with_solution_impl(Ctx *c, int out_counts[4])
+{
+    int st;
+    if (!(c->MAX_LEVELS_INTERIOR > 1)) {
+        if (out_counts) std::fill(out_counts, out_counts + 4, 0);
+        return AGAL_OK;
+    }
+    if ((st = prepare_fields(c))) return st;
     if ((st = launch_ref_criteria(c))) return st;
     return refine_coarsen(c, out_counts);
+}
+
+// The plain criterion's labels without changing the refinement marks (the marks are saved and
+// restored around the criterion).
+int criterion_labels(Ctx *c)
+{
+    const int idm = c->id_max[c->MAX_LEVELS];
+    if (idm <= 0) return AGAL_OK;
+    AGAL_HIP(c, hipMemcpyAsync(c->scan_b, c->ref, sizeof(int) * idm, hipMemcpyDeviceToDevice, c->stream));
+    int st = launch_ref_criteria(c, c->pred.label);
+    AGAL_HIP(c, hipMemcpyAsync(c->ref, c->scan_b, sizeof(int) * idm, hipMemcpyDeviceToDevice, c->stream));
+    return st;
+}
+
+// M_AdvanceLoop with the learned predictor (no reference counterpart).  Every `infer_every` coarse
+// steps the features of all blocks are computed; during the first `train_steps` steps they are
+// kept as snapshots, and at each solution-based rebuild the plain criterion's labels train the
+// network (`adam_steps` Adam steps on batches sampled from the snapshots).  After that the
+// network runs at every `infer_every` step: unrefined blocks it predicts are refined at once
+// (a refine-only rebuild), and at each rebuild its predicted blocks are kept from coarsening while
+// the plain criterion still refines what it flags (fallback).  The judge counts, at every rebuild,
+// the blocks the criterion wants refined and those of them still unrefined (missed).
+int advance_loop_learned_impl(Ctx *c, int i0, int n, const agal_learn_opts *o, agal_learn_stats *S)
+{
+    int st;
+    const int ML = c->MAX_LEVELS, L0 = c->N_LEVEL_START, iter_mult = 1 << L0;
+    const int every = std::max(1, o->infer_every);
+    const int max_slots = std::max(1, c->P.P_REFINE / every + 1);
+    const float thr = (float)o->threshold, lr = (float)o->lr, pw = (float)o->pos_weight;
+    std::memset(S, 0, sizeof(*S));
+    if (!c->pred.w || c->pred.batch_cap < o->batch)
+        if ((st = predictor_setup(c, o->seed, o->batch))) return st;
+    uint64_t rng = o->seed ^ 0x5DEECE66DULL;
+    int n_snap = 0;
+    auto ms_since = [](std::chrono::steady_clock::time_point t0) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
+    hipEvent_t e0, e1;
+    AGAL_HIP(c, hipEventCreate(&e0));
+    AGAL_HIP(c, hipEventCreate(&e1));
+    auto close_window = [&]() -> int {
+        float ms = 0;
+        AGAL_HIP(c, hipEventRecord(e1, c->stream));
+        AGAL_HIP(c, hipEventSynchronize(e1));
+        AGAL_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+        S->step_ms += ms;
+        return AGAL_OK;
+    };
+    AGAL_HIP(c, hipEventRecord(e0, c->stream));
+    for (int i = i0; i < i0 + n; i++) {
+        for (int j = 0; j < iter_mult; j++)
+            if ((st = step_once(c, L0))) return st;
+        const bool training = (i + 1) <= o->train_steps;
+        const bool rebuild = ML > 1 && ML != L0 + 1 && (i + 1) % c->P.P_REFINE == 0 && i > 0 && c->MAX_LEVELS_INTERIOR > 1;
+        const bool infer = (i + 1) % every == 0;
+        if (!rebuild && !infer) continue;
+        if ((st = close_window())) return st;
+        auto t0 = std::chrono::steady_clock::now();
+        if ((st = prepare_fields(c)) || (st = predictor_features(c))) return st;
+        if (!rebuild) {
+            if (training) {
+                if (n_snap < max_slots && (st = predictor_snapshot(c, n_snap++, max_slots))) return st;
+            } else if (o->use_predictions) {
+                int marked = 0;
+                if ((st = predictor_predict_blocks(c)) || (st = predictor_mark(c, thr, &marked))) return st;
+                if (marked > 0) {
+                    st = refine_coarsen(c, nullptr);
+                    if (st && st != AGAL_ERR_POOL_EXHAUSTED) return st;
+                    S->early_rebuilds++;
+                    S->predicted_refinements += marked;
+                }
+            }
+            AGAL_HIP(c, hipStreamSynchronize(c->stream));
+            S->predictor_ms += ms_since(t0);
+        } else {
+            // Judge and labels from the plain criterion, then its marks.
+            if ((st = criterion_labels(c))) return st;
+            int jc[2] = {0, 0};
+            if ((st = predictor_judge(c, jc))) return st;
+            if (training) {
+                S->flagged_train += jc[0];
+                S->missed_train += jc[1];
+                double loss = 0;
+                if ((st = predictor_train_snapshots(c, n_snap, o->batch, o->adam_steps, lr, pw, rng, &loss))) return st;
+                if (n_snap > 0) {
+                    S->adam_steps_done += o->adam_steps;
+                    S->last_loss = loss;
+                }
+                n_snap = 0;
+            } else {
+                S->flagged += jc[0];
+                S->missed += jc[1];
+            }
+            AGAL_HIP(c, hipStreamSynchronize(c->stream));
+            S->predictor_ms += ms_since(t0);
+            auto t1 = std::chrono::steady_clock::now();
+            if ((st = launch_ref_criteria(c))) return st;
+            if (!training && o->use_predictions) {
+                int marked = 0;
+                if ((st = predictor_predict_blocks(c)) || (st = predictor_keep(c, thr)) ||
+                    (st = predictor_mark(c, thr, &marked)))
+                    return st;
+                S->predicted_refinements += marked;
+            }
+            st = refine_coarsen(c, nullptr);
+            if (st && st != AGAL_ERR_POOL_EXHAUSTED) return st;
+            AGAL_HIP(c, hipStreamSynchronize(c->stream));
+            S->rebuilds++;
+            S->rebuild_ms += ms_since(t1);
+        }
+        AGAL_HIP(c, hipEventRecord(e0, c->stream));
+    }
+    if ((st = close_window())) return st;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return check_exceptions(c);
 }
 
 // Mesh::M_Advance_RefineNearWall (src/mesh/mesh_advance.cu:53-121).  The benchmark-case hook
@@ -877,6 +1005,115 @@ long long agal_load_stl(const char *path, float *out, long long capacity)
 {
     if (!path) return -1;
     return load_stl_ascii(path, out, capacity);
+}
+
+int agal_predictor_init(agal_ctx *ctx, unsigned long long seed, int batch_capacity)
+{
+    CTX_OR_ARG(ctx);
+    if (batch_capacity <= 0) return AGAL_ERR_ARG;
+    return predictor_setup(reinterpret_cast<Ctx *>(ctx), seed, batch_capacity);
+}
+
+int agal_predictor_set(agal_ctx *ctx, const float *params)
+{
+    CTX_OR_ARG(ctx);
+    if (!params) return AGAL_ERR_ARG;
+    return predictor_set_weights(reinterpret_cast<Ctx *>(ctx), params);
+}
+
+int agal_predictor_get(agal_ctx *ctx, int which, float *out)
+{
+    CTX_OR_ARG(ctx);
+    if (!out) return AGAL_ERR_ARG;
+    return predictor_get(reinterpret_cast<Ctx *>(ctx), which, out);
+}
+
+int agal_predictor_forward(agal_ctx *ctx, const float *x, int n, float *z_out, float *p_out)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c->pred.w) return AGAL_ERR_STATE;
+    if (n < 0 || n > c->pred.batch_cap || (n > 0 && !x)) return AGAL_ERR_ARG;
+    if (n == 0) return AGAL_OK;
+    AGAL_HIP(c, hipMemcpyAsync(c->pred.X, x, sizeof(float) * NFEAT * n, hipMemcpyHostToDevice, c->stream));
+    int st = predictor_forward_dev(c, c->pred.X, n, c->pred.z3, c->pred.Y, nullptr, nullptr);
+    if (st) return st;
+    if (z_out) AGAL_HIP(c, hipMemcpyAsync(z_out, c->pred.z3, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    if (p_out) AGAL_HIP(c, hipMemcpyAsync(p_out, c->pred.Y, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    return AGAL_OK;
+}
+
+int agal_predictor_train_step(agal_ctx *ctx, const float *x, const float *y, int n, double lr, double pos_weight,
+                              double *loss)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c->pred.w) return AGAL_ERR_STATE;
+    if (n <= 0 || n > c->pred.batch_cap || !x || !y) return AGAL_ERR_ARG;
+    AGAL_HIP(c, hipMemcpyAsync(c->pred.X, x, sizeof(float) * NFEAT * n, hipMemcpyHostToDevice, c->stream));
+    AGAL_HIP(c, hipMemcpyAsync(c->pred.Y, y, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    double l = 0;
+    int st = predictor_train_dev(c, n, (float)lr, (float)pos_weight, &l);
+    if (loss) *loss = l;
+    return st;
+}
+
+int agal_predictor_features(agal_ctx *ctx, float *out, long long n_blocks)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c->pred.w) return AGAL_ERR_STATE;
+    const int idm = c->id_max[c->MAX_LEVELS];
+    if (n_blocks < 0 || n_blocks > c->n_maxcblocks || (n_blocks > 0 && !out)) return AGAL_ERR_ARG;
+    int st = predictor_features(c);
+    if (st) return st;
+    const long long nc = std::min<long long>(n_blocks, idm);
+    if (nc > 0)
+        AGAL_HIP(c, hipMemcpyAsync(out, c->pred.feat, sizeof(float) * NFEAT * nc, hipMemcpyDeviceToHost, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    if (n_blocks > nc) std::memset(out + NFEAT * nc, 0, sizeof(float) * NFEAT * (n_blocks - nc));
+    return AGAL_OK;
+}
+
+int agal_criterion_labels(agal_ctx *ctx, int *out, long long n_blocks)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c->pred.w) return AGAL_ERR_STATE;
+    const int idm = c->id_max[c->MAX_LEVELS];
+    if (n_blocks < 0 || n_blocks > c->n_maxcblocks || (n_blocks > 0 && !out)) return AGAL_ERR_ARG;
+    int st = criterion_labels(c);
+    if (st) return st;
+    const long long nc = std::min<long long>(n_blocks, idm);
+    if (nc > 0) AGAL_HIP(c, hipMemcpyAsync(out, c->pred.label, sizeof(int) * nc, hipMemcpyDeviceToHost, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    for (long long b = nc; b < n_blocks; b++) out[b] = -1;
+    return AGAL_OK;
+}
+
+void agal_learn_opts_default(agal_learn_opts *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->train_steps = 2000;
+    o->infer_every = 25;
+    o->batch = 4096;
+    o->adam_steps = 8;
+    o->lr = 1e-3;
+    o->threshold = 0.5;
+    o->pos_weight = 4.0;
+    o->seed = 1;
+    o->use_predictions = 1;
+}
+
+int agal_advance_loop_learned(agal_ctx *ctx, int i0, int n, const agal_learn_opts *opts, agal_learn_stats *stats)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!opts || !stats || n < 0 || opts->batch <= 0 || opts->infer_every <= 0 || opts->adam_steps < 0) return AGAL_ERR_ARG;
+    if (!c->ic_done) return AGAL_ERR_STATE;
+    return advance_loop_learned_impl(c, i0, n, opts, stats);
 }
 
 int agal_mark_uniform(agal_ctx *ctx, int L)

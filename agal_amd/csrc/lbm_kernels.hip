@@ -153,20 +153,22 @@ __device__ __forceinline__ void collide_cell(const StepArgs<T, S> &a, const T *f
         return;
     }
     using MB = Mrt<Q>;
+    // Full unrolling keeps d, r and the rates in registers (otherwise the FP64 D3Q27 variant indexes
+    // them dynamically from scratch memory).
     T d[Q];
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int p = 0; p < Q; p++) {
         T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
         d[p] = f[p] - (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
     }
     T r[Q];
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int k = 0; k < Q; k++) {
         r[k] = (T)0;
         if (MB::rate(k) == 0) continue;
         T acc = (T)0;
         bool first = true;
-#pragma unroll
+#pragma clang loop unroll(full)
         for (int p = 0; p < Q; p++) {
             if (MB::m(k, p) == 0) continue;
             const T term = (T)MB::m(k, p) * d[p];
@@ -175,11 +177,11 @@ __device__ __forceinline__ void collide_cell(const StepArgs<T, S> &a, const T *f
         }
         r[k] = acc * a.ck[k];
     }
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int p = 0; p < Q; p++) {
         T acc = (T)0;
         bool first = true;
-#pragma unroll
+#pragma clang loop unroll(full)
         for (int k = 0; k < Q; k++) {
             if (MB::rate(k) == 0 || MB::m(k, p) == 0) continue;
             const T term = (T)MB::m(k, p) * r[k];

@@ -134,6 +134,55 @@ class LbmMesh:
         self._call("forces", out)
         return np.array(list(out))
 
+    # -- learned refinement predictor (no reference counterpart) -----------------------------------
+    def predictor_init(self, seed=1, batch_capacity=4096):
+        self._call("predictor_init", int(seed), int(batch_capacity))
+
+    def predictor_params(self, which=0) -> np.ndarray:
+        out = np.empty(capi.MLP_NPARAM, dtype=np.float32)
+        self._call("predictor_get", int(which), np_ptr(out, C.c_float))
+        return out
+
+    def predictor_set_params(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == capi.MLP_NPARAM
+        self._call("predictor_set", np_ptr(w, C.c_float))
+
+    def predictor_forward(self, x: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, capi.MLP_NFEAT)
+        n = x.shape[0]
+        z = np.empty(n, dtype=np.float32)
+        p = np.empty(n, dtype=np.float32)
+        self._call("predictor_forward", np_ptr(x, C.c_float), n, np_ptr(z, C.c_float), np_ptr(p, C.c_float))
+        return z, p
+
+    def predictor_train_step(self, x, y, lr=1e-3, pos_weight=4.0) -> float:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, capi.MLP_NFEAT)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = C.c_double()
+        self._call("predictor_train_step", np_ptr(x, C.c_float), np_ptr(y, C.c_float), x.shape[0], float(lr),
+                   float(pos_weight), C.byref(loss))
+        return loss.value
+
+    def predictor_features(self, n_blocks=None) -> np.ndarray:
+        nb = self.n_blocks() if n_blocks is None else int(n_blocks)
+        out = np.empty((nb, capi.MLP_NFEAT), dtype=np.float32)
+        self._call("predictor_features", np_ptr(out, C.c_float), nb)
+        return out
+
+    def criterion_labels(self, n_blocks=None) -> np.ndarray:
+        nb = self.n_blocks() if n_blocks is None else int(n_blocks)
+        out = np.empty(nb, dtype=np.int32)
+        self._call("criterion_labels", np_ptr(out, C.c_int), nb)
+        return out
+
+    def advance_loop_learned(self, i0, n, opts=None) -> dict:
+        o = capi.learn_opts() if opts is None else opts
+        s = capi.LearnStats()
+        self._call("advance_loop_learned", int(i0), int(n), C.byref(o), C.byref(s))
+        self.refresh_sizes()
+        return s.as_dict()
+
     def M_FreezeRefinedCells(self, var=0):
         self._call("freeze_refined", int(var))
 

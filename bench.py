@@ -25,6 +25,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "MLUPS + %HBM roofline, D3Q19 AMR; FP16-store vs FP32 at ≤1e-3 vel err"
+T_START = time.time()
+
+
+def log(msg: str):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.time() - T_START:7.1f}s] {msg}", file=sys.stderr, flush=True)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -79,10 +85,12 @@ def main():
     m = LbmMesh(p, args.capacity)
     m.S_SetIC(0)
     m.M_FreezeRefinedCells(0)
+    log(f"cavity: {args.warmup} warm-up coarse steps with rebuilds")
     t0 = time.time()
     st_w, _, reb_w = m.advance_loop(0, args.warmup)
     m.sync()
     warm_s = time.time() - t0
+    log(f"cavity: warm-up done in {warm_s:.1f} s; roofline pass and timed window")
     s = m.refresh_sizes()
     n_ids = [int(s.n_ids[L]) for L in range(levels)]
 
@@ -138,14 +146,21 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        log("CPU baseline sample")
         cpu = cpu_baseline(m, p, args.capacity, upd_actual, args.cpu_seconds)
     del m
     storage = None
     if not args.no_storage_study:
+        log("storage study: FP64 / FP32 / FP16 / BF16 on one mesh sequence")
         storage = storage_study(p, args)
     sphere = None
     if not args.no_sphere_study:
-        sphere = sphere_study(args)
+        log("sphere study: D3Q27 MRT flow past the noisy icosphere")
+        try:
+            sphere = sphere_study(args)
+        except Exception as e:  # reported in the line; the headline measurement stands on its own
+            log(f"sphere study failed: {e}")
+            sphere = {"error": str(e)}
 
     line = {
         "metric": METRIC,
@@ -231,6 +246,7 @@ def storage_study(p, args):
     bpp = {"fp64": 8, "fp32": 4, "fp16": 2, "bf16": 2}
     for k in ("fp64", "fp32", "fp16", "bf16"):
         m = ctx[k]
+        log(f"storage study: timing {k}")
         m.sync()
         _, step_ms, _ = m.advance_loop(args.warmup, args.steps)
         m.sync()
@@ -285,6 +301,7 @@ def sphere_study(args):
         m.sync()
         out.setdefault("near_wall_setup_s", {})[k] = round(time.time() - t0, 3)
         ctx[k] = m
+        log(f"sphere {k}: near-wall hierarchy {[int(m.sizes.n_ids[L]) for L in range(levels)]}")
     s = ctx["fp64"].refresh_sizes()
     n_ids = [int(s.n_ids[L]) for L in range(levels)]
     upd = sum(n_ids[L] * 64 * (1 << L) for L in range(levels))
@@ -293,7 +310,12 @@ def sphere_study(args):
     K = args.sphere_steps
     avg_from = max(0, K - 100)
     for k, m in ctx.items():
-        _, ms1, _ = m.advance_loop(0, avg_from)
+        log(f"sphere {k}: {K} coarse steps")
+        ms1 = 0.0
+        for i in range(0, avg_from, 250):
+            _, ms, _ = m.advance_loop(i, min(250, avg_from - i))
+            ms1 += ms
+            log(f"sphere {k}: {min(i + 250, avg_from)} steps, {ms1 / 1e3:.1f} s")
         cds = []
         ms2 = 0.0
         for i in range(avg_from, K, 10):

@@ -199,6 +199,68 @@ int  agal_advance_loop(agal_ctx *ctx, int i0, int n, double *out_step_ms, double
  * of the refined blocks.  Returns AGAL_ERR_POOL_EXHAUSTED if a rebuild was cancelled. */
 int  agal_refine_near_wall(agal_ctx *ctx);
 
+/* ---- learned refinement predictor (no counterpart in the reference) --------------------------
+ * A 12-64-64-1 network (ReLU, sigmoid output) over 12 features per block, FP16 weights with FP32
+ * accumulation, trained online with Adam on the plain criterion's labels (the criterion of
+ * agal_ref_criteria stays the fallback and the judge).  Features of block b, from the stored
+ * macroscopic fields: plog2(x + 2^-30)/16 of the block's max |u|, max |rho-1|, max |grad u|,
+ * max |vorticity| (plog2 = piecewise-linear log2, exact at powers of two), level/(levels-1), a
+ * signed distance (to the geometry's nearest triangle centroid, else to the nearest domain face,
+ * in root blocks, clamped to +-8, /8), the neighbour maxima of the four statistics, the fraction
+ * of neighbours with children and the neighbour minimum of the distance.  Parameters, flat FP32:
+ * W1[12][64], b1[64], W2[64][64], b2[64], W3[64], b3 (AGAL_MLP_NPARAM values). */
+#define AGAL_MLP_NFEAT  12
+#define AGAL_MLP_NPARAM 5057
+
+typedef struct agal_learn_opts {
+    int    train_steps;      /* coarse steps with online training (2000)                          */
+    int    infer_every;      /* coarse steps between feature snapshots / inferences (25)          */
+    int    batch;            /* rows per Adam step (4096)                                         */
+    int    adam_steps;       /* Adam steps per rebuild while training (8)                         */
+    double lr;               /* Adam learning rate (1e-3)                                         */
+    double threshold;        /* a block is predicted to need children when p > threshold (0.5)    */
+    double pos_weight;       /* loss weight of positive labels (4)                                */
+    unsigned long long seed; /* weight initialisation and batch sampling                          */
+    int    use_predictions;  /* 0: plain criterion only (training and judge counts still run)     */
+} agal_learn_opts;
+
+typedef struct agal_learn_stats {
+    long long flagged, missed;             /* after training: blocks the criterion wants refined at
+                                              rebuilds, and those of them still unrefined          */
+    long long flagged_train, missed_train; /* the same counts during training (plain method)       */
+    long long predicted_refinements;       /* blocks refined because the network predicted them    */
+    int    adam_steps_done, rebuilds, early_rebuilds;
+    double last_loss;                      /* mean loss of the last training batch                 */
+    double step_ms, rebuild_ms, predictor_ms;
+} agal_learn_stats;
+
+/* Allocates the predictor (training buffers for up to batch_capacity rows) and initialises the
+ * weights He-uniformly from a splitmix64 stream seeded with `seed`; Adam state reset. */
+int  agal_predictor_init(agal_ctx *ctx, unsigned long long seed, int batch_capacity);
+/* Sets the FP32 master parameters (AGAL_MLP_NPARAM values); Adam state reset. */
+int  agal_predictor_set(agal_ctx *ctx, const float *params);
+/* which: 0 master parameters, 1 their FP16 copy (as FP32), 2 last gradient, 3 / 4 Adam m / v. */
+int  agal_predictor_get(agal_ctx *ctx, int which, float *out);
+/* Logits and probabilities of n feature rows (n <= batch_capacity); either output may be NULL. */
+int  agal_predictor_forward(agal_ctx *ctx, const float *x, int n, float *z_out, float *p_out);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the weighted binary cross-entropy of n rows,
+ * dL/dz = w (sigmoid(z) - y) / n with w = pos_weight for y = 1; the mean loss in *loss. */
+int  agal_predictor_train_step(agal_ctx *ctx, const float *x, const float *y, int n, double lr,
+                               double pos_weight, double *loss);
+/* Feature rows of block IDs [0, n_blocks) from the stored macroscopic fields (zeros for IDs that
+ * are inactive or beyond the largest active ID). */
+int  agal_predictor_features(agal_ctx *ctx, float *out, long long n_blocks);
+/* The plain criterion's label per block ID: 1 wants children, 0 not, -1 inactive.  The stored
+ * refinement marks are left unchanged. */
+int  agal_criterion_labels(agal_ctx *ctx, int *out, long long n_blocks);
+void agal_learn_opts_default(agal_learn_opts *opts);
+/* M_AdvanceLoop (src/mesh/mesh_advance.cu:421-463) with the predictor: snapshots and training
+ * during the first train_steps coarse steps, then predicted blocks refined every infer_every steps
+ * and kept from coarsening at rebuilds, the plain criterion refining what it flags; judge counts
+ * and timings in *stats. */
+int  agal_advance_loop_learned(agal_ctx *ctx, int i0, int n, const agal_learn_opts *opts,
+                               agal_learn_stats *stats);
+
 /* ---- geometry (3D velocity sets) -------------------------------------------------------------
  * A closed triangle surface, 9 floats per triangle (x1 y1 z1 x2 y2 z2 x3 y3 z3), in domain
  * coordinates.  Replaces Geometry::G_Init_Arrays_CoordsList_CPU + G_MakeBinsGPU

@@ -1064,7 +1064,9 @@ struct Oracle {
         return (T)(+4.0) * d1 + (T)(-6.0) * d2 + (T)(+4.0) * d3 + (T)(-1.0) * d4;
     }
 
-    void ref_criteria()
+    // `label` (may be null): 1 when the criterion wants block b to have children, 0 when not, -1
+    // for inactive IDs (the learned predictor's training target; no reference counterpart).
+    void ref_criteria(int *label = nullptr)
     {
         const int idm = id_max[MAX_LEVELS];
         const T dx_L = dxf_vec[0];
@@ -1072,7 +1074,10 @@ struct Oracle {
         const int MLI = MAX_LEVELS_INTERIOR;
 #pragma omp parallel for schedule(static)
         for (int b = 0; b < idm; b++) {
-            if (ref[b] == REF_INACTIVE) continue;
+            if (ref[b] == REF_INACTIVE) {
+                if (label) label[b] = -1;
+                continue;
+            }
             T su[216], sv[216], sw[216];
             auto H = [&](int I, int J, int K) { return (DIM == 2) ? (I + 1) + 6 * (J + 1) : (I + 1) + 6 * (J + 1) + 36 * (K + 1); };
             for (int t = 0; t < M_CBLOCK; t++) {
@@ -1154,6 +1159,160 @@ struct Oracle {
                 ref[b] = REF_MARK_REFINE;
             if (ref_kap == REF_REFINED && L_desired < level_kap + 1)
                 ref[b] = REF_MARK_COARSEN;
+            if (label) label[b] = (eligible && level_kap != MLI - 1 && L_desired > level_kap) ? 1 : 0;
+        }
+    }
+
+    // ---------------------------------------------------------------------------------------
+    // Learned refinement predictor: per-block features (no reference counterpart).  The block
+    // statistics use the criterion's halo-extrapolated central differences with the block's own
+    // spacing: max |u|, max |rho - 1|, max |grad u|, max |vorticity| over non-ghost, non-solid cells.
+    // ---------------------------------------------------------------------------------------
+    static float plog2(float v)
+    {
+        int e;
+        const float m = std::frexp(v, &e);
+        return (float)(e - 1) + (m - 0.5f) * 2.0f;
+    }
+
+    void block_stats(std::vector<float> &stats)
+    {
+        const int idm = id_max[MAX_LEVELS];
+        stats.assign((size_t)4 * std::max(idm, 0), 0.f);
+        const T dx0 = dxf_vec[0];
+        for (int b = 0; b < idm; b++) {
+            if (ref[b] == REF_INACTIVE) continue;
+            T su[216], sv[216], sw[216];
+            auto H = [&](int I, int J, int K) { return (DIM == 2) ? (I + 1) + 6 * (J + 1) : (I + 1) + 6 * (J + 1) + 36 * (K + 1); };
+            for (int t = 0; t < M_CBLOCK; t++) {
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                long cell = (long)b * M_CBLOCK + t;
+                su[H(I, J, K)] = Fp(Q + 1)[cell];
+                sv[H(I, J, K)] = Fp(Q + 2)[cell];
+                sw[H(I, J, K)] = (DIM == 3) ? Fp(Q + 3)[cell] : (T)0;
+            }
+            auto halo = [&](T *s) {
+                for (int t = 0; t < M_CBLOCK; t++) {
+                    int I, J, K;
+                    cell_ijk(t, I, J, K);
+                    if (I == 0) {
+                        s[H(-1, J, K)] = extrap4(s[H(0, J, K)], s[H(1, J, K)], s[H(2, J, K)], s[H(3, J, K)]);
+                        s[H(4, J, K)] = extrap4(s[H(3, J, K)], s[H(2, J, K)], s[H(1, J, K)], s[H(0, J, K)]);
+                    }
+                    if (J == 0) {
+                        s[H(I, -1, K)] = extrap4(s[H(I, 0, K)], s[H(I, 1, K)], s[H(I, 2, K)], s[H(I, 3, K)]);
+                        s[H(I, 4, K)] = extrap4(s[H(I, 3, K)], s[H(I, 2, K)], s[H(I, 1, K)], s[H(I, 0, K)]);
+                    }
+                    if (DIM == 3 && K == 0) {
+                        s[H(I, J, -1)] = extrap4(s[H(I, J, 0)], s[H(I, J, 1)], s[H(I, J, 2)], s[H(I, J, 3)]);
+                        s[H(I, J, 4)] = extrap4(s[H(I, J, 3)], s[H(I, J, 2)], s[H(I, J, 1)], s[H(I, J, 0)]);
+                    }
+                }
+            };
+            halo(su);
+            halo(sv);
+            if (DIM == 3) halo(sw);
+            const T d2 = (T)2 * (dx0 / (T)(1 << level[b]));
+            float mx[4] = {0.f, 0.f, 0.f, 0.f};
+            for (int t = 0; t < M_CBLOCK; t++) {
+                long cell = (long)b * M_CBLOCK + t;
+                const int m = cells_mask[cell];
+                if (m == CELL_GHOST || m == CELL_SOLID) continue;
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                const T u = su[H(I, J, K)], v = sv[H(I, J, K)], w = sw[H(I, J, K)], rho = Fp(Q)[cell];
+                const T uX = (su[H(I + 1, J, K)] - su[H(I - 1, J, K)]) / d2, uY = (su[H(I, J + 1, K)] - su[H(I, J - 1, K)]) / d2;
+                const T vX = (sv[H(I + 1, J, K)] - sv[H(I - 1, J, K)]) / d2, vY = (sv[H(I, J + 1, K)] - sv[H(I, J - 1, K)]) / d2;
+                T uZ = 0, vZ = 0, wX = 0, wY = 0, wZ = 0;
+                if (DIM == 3) {
+                    uZ = (su[H(I, J, K + 1)] - su[H(I, J, K - 1)]) / d2;
+                    vZ = (sv[H(I, J, K + 1)] - sv[H(I, J, K - 1)]) / d2;
+                    wX = (sw[H(I + 1, J, K)] - sw[H(I - 1, J, K)]) / d2;
+                    wY = (sw[H(I, J + 1, K)] - sw[H(I, J - 1, K)]) / d2;
+                    wZ = (sw[H(I, J, K + 1)] - sw[H(I, J, K - 1)]) / d2;
+                }
+                const float st[4] = {
+                    (float)std::sqrt(u * u + v * v + w * w), (float)std::fabs(rho - (T)1),
+                    (float)std::sqrt(uX * uX + uY * uY + uZ * uZ + vX * vX + vY * vY + vZ * vZ + wX * wX + wY * wY + wZ * wZ),
+                    (float)std::sqrt((wY - vZ) * (wY - vZ) + (uZ - wX) * (uZ - wX) + (vX - uY) * (vX - uY))};
+                for (int k = 0; k < 4; k++) mx[k] = std::fmax(mx[k], st[k]);
+            }
+            for (int k = 0; k < 4; k++) stats[4 * (size_t)b + k] = mx[k];
+        }
+    }
+
+    // Signed distance feature: to the nearest triangle centroid (negative for blocks holding solid
+    // cells) with a geometry, else to the nearest domain face; root-block units, clamped to
+    // [-8, 8], divided by 8.
+    void block_distance(std::vector<float> &dist)
+    {
+        const int idm = id_max[MAX_LEVELS];
+        dist.assign(std::max(idm, 0), 0.f);
+        const float dx0 = (float)dxf_vec[0];
+        const float Lx = (float)P.L_c, Ly = (float)(P.L_fy * Lx), Lz = (float)(P.L_fz * Lx);
+        const long nt = (long)geo_tri.size() / 9;
+        for (int b = 0; b < idm; b++) {
+            if (ref[b] == REF_INACTIVE) continue;
+            const float h = 2.0f * dx0 / (float)(1 << level[b]);
+            const float xc = X[b] + h, yc = X[b + n_maxcblocks] + h, zc = DIM == 3 ? X[b + 2 * n_maxcblocks] + h : 0.f;
+            float d;
+            if (nt > 0) {
+                float best = 3.0e38f;
+                for (long k = 0; k < nt; k++) {
+                    const float *q = &geo_tri[9 * k];
+                    const float cx = (q[0] + q[3] + q[6]) * (1.0f / 3.0f) - xc;
+                    const float cy = (q[1] + q[4] + q[7]) * (1.0f / 3.0f) - yc;
+                    const float cz = (q[2] + q[5] + q[8]) * (1.0f / 3.0f) - zc;
+                    best = std::fmin(best, cx * cx + cy * cy + cz * cz);
+                }
+                d = std::sqrt(best);
+                if (blk_mask[b] == BLK_SOLID || blk_mask[b] == BLK_SOLIDB) d = -d;
+            } else {
+                d = std::fmin(std::fmin(xc, Lx - xc), std::fmin(yc, Ly - yc));
+                if (DIM == 3) d = std::fmin(d, std::fmin(zc, Lz - zc));
+            }
+            d = d / (4.0f * dx0);
+            dist[b] = std::fmin(std::fmax(d, -8.0f), 8.0f) * 0.125f;
+        }
+    }
+
+    // Feature rows (12 per block ID below id_max; zeros for inactive IDs): own statistics as
+    // plog2(stat + 2^-30) / 16, level / (MAX_LEVELS - 1), distance, neighbour maxima of the four
+    // statistics, fraction of existing neighbours with children, neighbour minimum of distance.
+    void features(float *out)
+    {
+        std::vector<float> st, dist;
+        block_stats(st);
+        block_distance(dist);
+        const int idm = id_max[MAX_LEVELS];
+        const float tiny = 9.313225746154785e-10f;
+        for (int b = 0; b < idm; b++) {
+            float *f = out + 12 * (size_t)b;
+            if (ref[b] == REF_INACTIVE) {
+                for (int k = 0; k < 12; k++) f[k] = 0.f;
+                continue;
+            }
+            float own[4], mx[4];
+            for (int k = 0; k < 4; k++) mx[k] = own[k] = plog2(st[4 * (size_t)b + k] + tiny) * 0.0625f;
+            float dmin = dist[b];
+            int n_nb = 0, n_ref = 0;
+            for (int p = 1; p < Q_max; p++) {
+                const int nb = nbr[b + (size_t)p * n_maxcblocks];
+                if (nb < 0) continue;
+                n_nb++;
+                if (nbr_child[nb] >= 0) n_ref++;
+                for (int k = 0; k < 4; k++) mx[k] = std::fmax(mx[k], plog2(st[4 * (size_t)nb + k] + tiny) * 0.0625f);
+                dmin = std::fmin(dmin, dist[nb]);
+            }
+            for (int k = 0; k < 4; k++) {
+                f[k] = own[k];
+                f[6 + k] = mx[k];
+            }
+            f[4] = (float)level[b] / (float)(MAX_LEVELS > 1 ? MAX_LEVELS - 1 : 1);
+            f[5] = dist[b];
+            f[10] = n_nb > 0 ? (float)n_ref / (float)n_nb : 0.f;
+            f[11] = dmin;
         }
     }
 
@@ -1721,6 +1880,107 @@ struct Oracle {
             }
         }
         return st;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// The learned predictor's network, restated: 12-64-64-1, ReLU, sigmoid output, FP16 copies of the
+// FP32 master weights in every pass, FP32 sums in index order, weighted binary cross-entropy on
+// logits, Adam with bias correction.  Parameter layout: W1[12][64], b1[64], W2[64][64], b2[64],
+// W3[64], b3.
+// ---------------------------------------------------------------------------------------------
+struct MlpRef {
+    static constexpr int F = 12, H = 64;
+    static constexpr int OW1 = 0, OB1 = F * H, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H, OB3 = OW3 + H,
+                         N = OB3 + 1;
+    std::vector<float> w, m, v, g;
+    int t = 0;
+
+    std::vector<float> half_copy() const
+    {
+        std::vector<float> h(N);
+        for (int q = 0; q < N; q++) h[q] = f16_to_f32(f16_from_f32(w[q]));
+        return h;
+    }
+
+    // z3 of one row; z1, z2 (64 each) when given.
+    static float forward_row(const float *wh, const float *x, float *z1, float *z2)
+    {
+        float h1[H];
+        for (int j = 0; j < H; j++) {
+            float z = wh[OB1 + j];
+            for (int i = 0; i < F; i++) z += x[i] * wh[OW1 + i * H + j];
+            if (z1) z1[j] = z;
+            h1[j] = z > 0.f ? z : 0.f;
+        }
+        float z3 = wh[OB3];
+        for (int k = 0; k < H; k++) {
+            float z = wh[OB2 + k];
+            for (int j = 0; j < H; j++) z += h1[j] * wh[OW2 + j * H + k];
+            if (z2) z2[k] = z;
+            z3 += (z > 0.f ? z : 0.f) * wh[OW3 + k];
+        }
+        return z3;
+    }
+
+    // One Adam step on n rows; returns the mean loss.
+    double train_step(const float *X, const float *Y, int n, float lr, float pos_weight)
+    {
+        const std::vector<float> wh = half_copy();
+        std::vector<float> Z1((size_t)n * H), Z2((size_t)n * H), dZ1((size_t)n * H), dZ2((size_t)n * H), dZ3(n);
+        double loss = 0;
+        for (int r = 0; r < n; r++) {
+            const float z = forward_row(wh.data(), X + (size_t)F * r, &Z1[(size_t)H * r], &Z2[(size_t)H * r]);
+            const float y = Y[r], wgt = y > 0.5f ? pos_weight : 1.0f;
+            const float sp_pos = std::fmax(z, 0.f) + std::log1p(std::exp(-std::fabs(z)));
+            const float sp_neg = sp_pos - z;
+            loss += wgt * (y * sp_neg + (1.0f - y) * sp_pos);
+            dZ3[r] = wgt * (1.0f / (1.0f + std::exp(-z)) - y) / (float)n;
+            for (int k = 0; k < H; k++) dZ2[(size_t)H * r + k] = Z2[(size_t)H * r + k] > 0.f ? dZ3[r] * wh[OW3 + k] : 0.f;
+            for (int j = 0; j < H; j++) {
+                float s = 0.f;
+                for (int k = 0; k < H; k++) s += wh[OW2 + j * H + k] * dZ2[(size_t)H * r + k];
+                dZ1[(size_t)H * r + j] = Z1[(size_t)H * r + j] > 0.f ? s : 0.f;
+            }
+        }
+        g.assign(N, 0.f);
+        for (int q = 0; q < N; q++) {
+            float s = 0.f;
+            if (q < OB1) {
+                const int i = q / H, j = q % H;
+                for (int r = 0; r < n; r++) s += X[(size_t)F * r + i] * dZ1[(size_t)H * r + j];
+            } else if (q < OW2) {
+                for (int r = 0; r < n; r++) s += dZ1[(size_t)H * r + (q - OB1)];
+            } else if (q < OB2) {
+                const int j = (q - OW2) / H, k = (q - OW2) % H;
+                for (int r = 0; r < n; r++) {
+                    const float z = Z1[(size_t)H * r + j];
+                    s += (z > 0.f ? z : 0.f) * dZ2[(size_t)H * r + k];
+                }
+            } else if (q < OW3) {
+                for (int r = 0; r < n; r++) s += dZ2[(size_t)H * r + (q - OB2)];
+            } else if (q < OB3) {
+                const int k = q - OW3;
+                for (int r = 0; r < n; r++) {
+                    const float z = Z2[(size_t)H * r + k];
+                    s += (z > 0.f ? z : 0.f) * dZ3[r];
+                }
+            } else {
+                for (int r = 0; r < n; r++) s += dZ3[r];
+            }
+            g[q] = s;
+        }
+        t++;
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+        const float bc1 = (float)(1.0 - std::pow(0.9, (double)t)), bc2 = (float)(1.0 - std::pow(0.999, (double)t));
+        if (m.empty()) m.assign(N, 0.f);
+        if (v.empty()) v.assign(N, 0.f);
+        for (int q = 0; q < N; q++) {
+            m[q] = b1 * m[q] + (1.0f - b1) * g[q];
+            v[q] = b2 * v[q] + (1.0f - b2) * g[q] * g[q];
+            w[q] = w[q] - lr * (m[q] / bc1) / (std::sqrt(v[q] / bc2) + eps);
+        }
+        return loss / n;
     }
 };
 

@@ -6,10 +6,12 @@
  * sequence they use for the HIP library.  Signatures mirror include/agal_capi.h with an
  * `oracle_` prefix; populations are exchanged in physical order (entry q = direction q).
  */
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -147,6 +149,59 @@ long long oracle_ray_count(const float *tri, long long n_tri, double px, double 
     for (long long k = 0; k < n_tri; k++) c += Oracle<double>::ray_crosses(tri + 9 * k, px, py, pz);
     return c;
 }
+/* Predictor features of block IDs [0, n_blocks) (zeros beyond the largest active ID). */
+int oracle_features(void *h, float *out, long long n_blocks)
+{
+    return dispatch(h, [&](auto &O) {
+        const long long idm = O.id_max[O.MAX_LEVELS];
+        std::vector<float> f((size_t)12 * std::max<long long>(idm, 1));
+        O.features(f.data());
+        for (long long b = 0; b < n_blocks; b++)
+            for (int k = 0; k < 12; k++) out[12 * b + k] = b < idm ? f[12 * b + k] : 0.f;
+        return AGAL_OK;
+    });
+}
+/* The plain criterion's labels; refinement marks left unchanged. */
+int oracle_criterion_labels(void *h, int *out, long long n_blocks)
+{
+    return dispatch(h, [&](auto &O) {
+        const long long idm = O.id_max[O.MAX_LEVELS];
+        std::vector<int> saved = O.ref, lab(std::max<long long>(idm, 1), -1);
+        O.ref_criteria(lab.data());
+        O.ref = saved;
+        for (long long b = 0; b < n_blocks; b++) out[b] = b < idm ? lab[b] : -1;
+        return AGAL_OK;
+    });
+}
+/* The network restated: logits of n rows with the FP16 copy of master parameters w. */
+int oracle_mlp_forward(const float *w, const float *x, int n, float *z)
+{
+    MlpRef M;
+    M.w.assign(w, w + MlpRef::N);
+    const std::vector<float> wh = M.half_copy();
+    for (int r = 0; r < n; r++) z[r] = MlpRef::forward_row(wh.data(), x + 12 * (size_t)r, nullptr, nullptr);
+    return AGAL_OK;
+}
+/* One Adam step; w, m, v (AGAL_MLP_NPARAM each) and the step count t are updated in place, the
+ * gradient goes to g (may be NULL). */
+int oracle_mlp_train_step(float *w, float *m, float *v, int *t, const float *x, const float *y, int n, double lr,
+                          double pos_weight, float *g, double *loss)
+{
+    MlpRef M;
+    M.w.assign(w, w + MlpRef::N);
+    M.m.assign(m, m + MlpRef::N);
+    M.v.assign(v, v + MlpRef::N);
+    M.t = *t;
+    const double l = M.train_step(x, y, n, (float)lr, (float)pos_weight);
+    std::copy(M.w.begin(), M.w.end(), w);
+    std::copy(M.m.begin(), M.m.end(), m);
+    std::copy(M.v.begin(), M.v.end(), v);
+    if (g) std::copy(M.g.begin(), M.g.end(), g);
+    *t = M.t;
+    if (loss) *loss = l;
+    return AGAL_OK;
+}
+float oracle_plog2(float v) { return Oracle<float>::plog2(v); }
 int oracle_mark_uniform(void *h, int L) { return dispatch(h, [&](auto &O) { O.mark_uniform(L); return AGAL_OK; }); }
 int oracle_freeze_refined(void *h, int var) { return dispatch(h, [&](auto &O) { O.freeze_refined(var); return AGAL_OK; }); }
 

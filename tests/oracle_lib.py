@@ -14,7 +14,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
 _NAMES = [k for k in capi.PROTOTYPES
-          if not k.startswith("timing") and k not in ("set_option", "upload_cell_masks", "upload_block_ints")]
+          if not k.startswith("timing") and not k.startswith("predictor") and
+          k not in ("set_option", "upload_cell_masks", "upload_block_ints", "criterion_labels", "advance_loop_learned")]
 _lib = None
 
 
@@ -30,6 +31,13 @@ def oracle_library():
             "impose_bc": (C.c_int, [C.c_void_p, C.c_int]),
             "stream": (C.c_int, [C.c_void_p, C.c_int]),
             "ray_count": (C.c_longlong, [C.POINTER(C.c_float), C.c_longlong, C.c_double, C.c_double, C.c_double]),
+            "features": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_longlong]),
+            "criterion_labels": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_longlong]),
+            "mlp_forward": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
+            "mlp_train_step": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                         C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
+                                         C.c_double, C.c_double, C.POINTER(C.c_float), C.POINTER(C.c_double)]),
+            "plog2": (C.c_float, [C.c_float]),
             "upload_cell_masks": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_longlong]),
             "get_level_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
             "velset": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -63,6 +71,20 @@ class OracleMesh(LbmMesh):
 
     def stream(self, L):
         self._call_raw("stream", L)
+
+    def predictor_features(self, n_blocks=None):
+        import numpy as np
+        nb = self.n_blocks() if n_blocks is None else int(n_blocks)
+        out = np.empty((nb, 12), dtype=np.float32)
+        oracle_library().oracle_features(self.h, out.ctypes.data_as(C.POINTER(C.c_float)), nb)
+        return out
+
+    def criterion_labels(self, n_blocks=None):
+        import numpy as np
+        nb = self.n_blocks() if n_blocks is None else int(n_blocks)
+        out = np.empty(nb, dtype=np.int32)
+        oracle_library().oracle_criterion_labels(self.h, out.ctypes.data_as(C.POINTER(C.c_int)), nb)
+        return out
 
     def level_params(self):
         n = self.sizes.max_levels
@@ -169,3 +191,37 @@ def from16(mode: int, bits):
     st = oracle_library().oracle_from16(mode, bits.ctypes.data, out.ctypes.data, bits.size)
     assert st == capi.AGAL_OK
     return out
+
+
+class MlpOracle:
+    """The restated network: FP32 master parameters, Adam state, one-step training."""
+
+    def __init__(self, w):
+        import numpy as np
+        self.w = np.array(w, dtype=np.float32)
+        self.m = np.zeros_like(self.w)
+        self.v = np.zeros_like(self.w)
+        self.g = np.zeros_like(self.w)
+        self.t = C.c_int(0)
+
+    def forward(self, x):
+        import numpy as np
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 12)
+        z = np.empty(x.shape[0], dtype=np.float32)
+        f = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        oracle_library().oracle_mlp_forward(f(self.w), f(x), x.shape[0], f(z))
+        return z
+
+    def train_step(self, x, y, lr=1e-3, pos_weight=4.0):
+        import numpy as np
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 12)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        f = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        loss = C.c_double()
+        oracle_library().oracle_mlp_train_step(f(self.w), f(self.m), f(self.v), C.byref(self.t), f(x), f(y),
+                                               x.shape[0], lr, pos_weight, f(self.g), C.byref(loss))
+        return loss.value
+
+
+def plog2(v: float) -> float:
+    return oracle_library().oracle_plog2(v)
