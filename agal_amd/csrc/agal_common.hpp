@@ -231,6 +231,68 @@ __host__ __device__ constexpr int mrt_norm(int k)
     return s;
 }
 
+// ---------------------------------------------------------------------------------------------
+// MRT on matrix cores (3D).  Per wavefront the 64 cells of a block are the N dimension of
+// v_mfma_f32_16x16x32_bf16 products.  Lane l = 16q + j of that instruction holds A[j][8q..8q+7]
+// and B[8q..8q+7][j] and receives C[4q+r][j] in result register r.  Forward m = M d: moment rows
+// in two halves h of 16, K = direction.  Back g = M^T (c . m): K runs over moments in the order the
+// forward result already occupies the lanes, K position 8q+s <-> moment 4q+s (s < 4) or
+// 16+4q+(s-4) (s >= 4), so no data moves between the two products.  The integer entries of M are
+// exact in bf16.
+// ---------------------------------------------------------------------------------------------
+__host__ __device__ constexpr uint16_t bf16_of_int(int v)  // exact for |v| < 256
+{
+    if (v == 0) return 0;
+    const uint32_t sign = v < 0 ? 0x8000u : 0u;
+    const uint32_t a = (uint32_t)(v < 0 ? -v : v);
+    int e = 0;
+    while ((a >> e) > 1u) e++;
+    const uint32_t mant = (a << (7 - e)) & 0x7Fu;
+    return (uint16_t)(sign | ((uint32_t)(127 + e) << 7) | mant);
+}
+
+__host__ __device__ constexpr int mfma_moment_of(int q, int s) { return s < 4 ? 4 * q + s : 16 + 4 * q + (s - 4); }
+
+template <int Q>
+struct alignas(16) MfmaTab {
+    uint16_t fw[64][2][8];  // forward A fragments (bf16 bits) per lane and moment half
+    uint16_t bk[64][2][8];  // back A fragments per lane and direction half
+    float invD[64][2][4];   // 1 / sum_p M_kp^2 of the C row in register r
+    uint8_t code[64][2][4]; // rate code of that row: 0 conserved, 1 omega, 2..7 s1..s6
+};
+
+template <int Q>
+constexpr MfmaTab<Q> make_mfma_tab()
+{
+    MfmaTab<Q> t{};
+    int M[32][32] = {};
+    int D[32] = {};
+    for (int k = 0; k < Q; k++)
+        for (int p = 0; p < Q; p++) {
+            M[k][p] = Mrt<Q>::m(k, p);
+            D[k] += M[k][p] * M[k][p];
+        }
+    uint16_t Mb[32][32] = {};
+    for (int k = 0; k < 32; k++)
+        for (int p = 0; p < 32; p++) Mb[k][p] = bf16_of_int(M[k][p]);
+    for (int l = 0; l < 64; l++) {
+        const int q = l / 16, j = l % 16;
+        for (int h = 0; h < 2; h++) {
+            for (int s = 0; s < 8; s++) {
+                const int i = 16 * h + j;
+                t.fw[l][h][s] = Mb[i][8 * q + s];
+                t.bk[l][h][s] = Mb[mfma_moment_of(q, s)][i];
+            }
+            for (int r = 0; r < 4; r++) {
+                const int i = 16 * h + 4 * q + r;
+                t.code[l][h][r] = (uint8_t)(i < Q ? Mrt<Q>::rate(i) : 0);
+                t.invD[l][h][r] = i < Q ? (float)(1.0 / (double)D[i]) : 0.0f;
+            }
+        }
+    }
+    return t;
+}
+
 // Boundary-condition presets of the reference hook Cu_ImposeBC (inc/custom.h:66-123).
 enum BcPreset { BC_NONE = 0, BC_LDC2D = 1, BC_LDC3D = 2, BC_FPSC2D = 3, BC_FPSC3D = 4 };
 

@@ -124,6 +124,7 @@ struct Ctx {
     StepGraph graphs[4];
     unsigned mesh_epoch = 1;
     bool use_graphs = true;
+    bool mrt_mfma = false;  // MRT moment transform on matrix cores (FP32 compute, D3Q19 / D3Q27)
 
     // Scheduler used while a step is being recorded.  Resources are (level, buffer parity) pairs,
     // bit 2L+b; each kernel waits only for the last writer and the readers since of the resources
@@ -169,6 +170,9 @@ int launch_refresh(Ctx *c, int L);
 int launch_collide_stream(Ctx *c, int L);
 int launch_interpolate(Ctx *c, int L, int var);
 int launch_average(Ctx *c, int L, int var);
+// average(L) in the advance's mode followed by interpolate(L, INTERFACE), fused for linear
+// interpolation (one launch, same values as the two kernels).
+int launch_average_interpolate(Ctx *c, int L);
 
 // Mesh kernels and the rebuild (amr_kernels.hip).
 // `label` (may be null): per block ID, 1 when the criterion wants the block to have children, 0

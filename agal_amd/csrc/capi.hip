@@ -308,8 +308,7 @@ int advance_rec(Ctx *c, int L)
             if (L == ML - 1) c->global_Liter++;
             if (L < ML - 1) {
                 if ((st = advance_rec(c, L + 1))) return st;
-                if ((st = launch_average(c, L, average_advance(c)))) return st;
-                if ((st = launch_interpolate(c, L, AGAL_INTERP_INTERFACE))) return st;
+                if ((st = launch_average_interpolate(c, L))) return st;
             }
             if ((st = launch_collide_stream(c, L))) return st;
             if (L == ML - 1) c->global_Liter++;
@@ -1393,6 +1392,11 @@ int agal_set_option(agal_ctx *ctx, int option, long long value)
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (option == AGAL_OPT_GRAPHS) {
         c->use_graphs = value != 0;
+        return AGAL_OK;
+    }
+    if (option == AGAL_OPT_MRT_MFMA) {
+        c->mrt_mfma = value != 0;
+        c->mesh_epoch++;  // recorded step graphs hold the other kernel
         return AGAL_OK;
     }
     return AGAL_ERR_ARG;

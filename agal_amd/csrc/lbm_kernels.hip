@@ -112,6 +112,7 @@ struct StepArgs {
     T omeg, omegp, U;
     int mrt;      // collision operator: 0 BGK, 1 MRT
     T ck[27];     // MRT: rate_k / D_k per moment row (0 for conserved rows)
+    T rates[8];   // MRT rates by code: 0, omega, s1..s6 (the matrix-core path)
     int bc_preset;
     int *exc_idx;
     T *exc_val;
@@ -192,6 +193,119 @@ __device__ __forceinline__ void collide_cell(const StepArgs<T, S> &a, const T *f
     }
 }
 
+// MRT transform on matrix cores for one 3D block per wavefront (Q = 19, 27; FP32 compute):
+// g = M^T (c . M d) for the non-equilibrium parts d of the 64 cells.  d and c . M d are split into
+// two bf16 terms each (hi + lo carry 16 significant bits; the products with the integer matrix are
+// exact and the sums FP32), so the result agrees with the vector path to about 2^-17 of d, below the
+// FP32 rounding of the populations themselves.  `lds` is this wavefront's 64 x 36-float region;
+// only the wavefront itself touches it.
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
+typedef __attribute__((ext_vector_type(4))) float f32x4_t;
+typedef __attribute__((ext_vector_type(4))) unsigned u32x4_t;
+
+__constant__ const MfmaTab<19> kMfma19 = make_mfma_tab<19>();
+__constant__ const MfmaTab<27> kMfma27 = make_mfma_tab<27>();
+
+constexpr int MFMA_SPLITS = 2;
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Next bf16 term of `rem` (round to nearest even, the hardware conversion); rem keeps the rest.
+__device__ __forceinline__ __bf16 split_term(float &rem)
+{
+    const __bf16 h = (__bf16)rem;
+    rem -= (float)h;
+    return h;
+}
+
+template <int Q>
+__device__ __forceinline__ void mrt_mfma(const float *d, float *g, const float *rates, float *lds)
+{
+    const MfmaTab<Q> &tab = *(Q == 19 ? reinterpret_cast<const MfmaTab<Q> *>(&kMfma19)
+                                      : reinterpret_cast<const MfmaTab<Q> *>(&kMfma27));
+    const int l = threadIdx.x & 63, q = l >> 4, j = l & 15;
+    const u32x4_t *fw = reinterpret_cast<const u32x4_t *>(&tab.fw[l][0][0]);
+    const u32x4_t *bk = reinterpret_cast<const u32x4_t *>(&tab.bk[l][0][0]);
+    const bf16x8_t afw0 = __builtin_bit_cast(bf16x8_t, fw[0]), afw1 = __builtin_bit_cast(bf16x8_t, fw[1]);
+    const bf16x8_t abk0 = __builtin_bit_cast(bf16x8_t, bk[0]), abk1 = __builtin_bit_cast(bf16x8_t, bk[1]);
+    bf16x8_t *ldsv = reinterpret_cast<bf16x8_t *>(lds);  // rows of 5 vectors (80 bytes)
+
+    // Forward: one pass per bf16 term of d through a 64-row transpose region.
+    f32x4_t C0[4], C1[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) C0[t] = C1[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    float rem[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++) rem[k] = k < Q ? d[k] : 0.f;
+#pragma unroll
+    for (int sp = 0; sp < MFMA_SPLITS; sp++) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            bf16x8_t v;
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = split_term(rem[8 * c + e]);
+            ldsv[5 * l + c] = v;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const bf16x8_t b = ldsv[5 * (16 * t + j) + q];
+            C0[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afw0, b, C0[t], 0, 0, 0);
+            C1[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afw1, b, C1[t], 0, 0, 0);
+        }
+        wave_lds_sync();
+    }
+    // Relaxation factors of the rows this lane holds.
+    float ck0[4], ck1[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int c0 = tab.code[l][0][r], c1 = tab.code[l][1][r];
+        float r0 = 0.f, r1 = 0.f;
+#pragma unroll
+        for (int cc = 1; cc < 8; cc++) {
+            r0 = c0 == cc ? rates[cc] : r0;
+            r1 = c1 == cc ? rates[cc] : r1;
+        }
+        ck0[r] = r0 * tab.invD[l][0][r];
+        ck1[r] = r1 * tab.invD[l][1][r];
+    }
+    // Back: B operand straight from the forward results of the same lane.
+    f32x4_t G0[4], G1[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        G0[t] = G1[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        float v[8];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            v[r] = C0[t][r] * ck0[r];
+            v[4 + r] = C1[t][r] * ck1[r];
+        }
+#pragma unroll
+        for (int sp = 0; sp < MFMA_SPLITS; sp++) {
+            bf16x8_t b;
+#pragma unroll
+            for (int e = 0; e < 8; e++) b[e] = split_term(v[e]);
+            G0[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abk0, b, G0[t], 0, 0, 0);
+            G1[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abk1, b, G1[t], 0, 0, 0);
+        }
+    }
+    // Transpose back: G row p of cell 16t + j -> LDS row of that cell (36 floats), then own row.
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        *reinterpret_cast<f32x4_t *>(lds + 36 * (16 * t + j) + 4 * q) = G0[t];
+        *reinterpret_cast<f32x4_t *>(lds + 36 * (16 * t + j) + 16 + 4 * q) = G1[t];
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int p = 0; p < Q; p++) g[p] = lds[36 * l + p];
+    wave_lds_sync();
+}
+
 // Population load of the fused step.  FP32/FP64 rows are read with non-temporal loads: they are
 // used once, and keeping them out of L2 leaves it to the pushes, whose partial cache lines from
 // neighbouring blocks merge there (measured: the FP32 fused step runs ~20% faster).  16-bit rows
@@ -256,8 +370,8 @@ __device__ __forceinline__ void push_link(const StepArgs<T, S> &a, int q, T fq, 
 // 3D: one wavefront per block, so the block, its flags and its 27 neighbour IDs are wave-uniform
 // (scalar registers); each lane picks its link's neighbour among at most eight of them with
 // selects on its own cell position -- no LDS traffic and no per-direction waits.
-template <typename T, typename S, int Q>
-__device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a)
+template <typename T, typename S, int Q, bool MF>
+__device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, float *lds)
 {
     using VS = VelSet<Q>;
     using ST = Store<T, S>;
@@ -285,7 +399,22 @@ __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a)
     T fs[Q];
     T rho, u, v, w;
     moments<T, Q>(f, rho, u, v, w);
-    collide_cell<T, S, Q>(a, f, fs, rho, u, v, w, inert);
+    if constexpr (MF) {
+        // MRT with the moment transform on matrix cores (whole wavefront, inert lanes carry zeros).
+        const T udotu = u * u + v * v + w * w;
+        float d[Q], g[Q];
+#pragma unroll
+        for (int p = 0; p < Q; p++) {
+            const T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+            const T feq = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
+            d[p] = inert ? 0.f : (float)(f[p] - feq);
+        }
+        mrt_mfma<Q>(d, g, a.rates, lds);
+#pragma unroll
+        for (int p = 0; p < Q; p++) fs[p] = inert ? f[VS::pb(p)] : f[p] - (T)g[p];
+    } else {
+        collide_cell<T, S, Q>(a, f, fs, rho, u, v, w, inert);
+    }
 
     a.B[cell] = ST::enc(fs[0], (T)VS::w(0));
     const int I = t & 3, J = (t >> 2) & 3, K = t >> 4;
@@ -308,14 +437,19 @@ __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a)
 template <typename T, typename S, int Q>
 __device__ __forceinline__ void collide_stream_2d(const StepArgs<T, S> &a);
 
-template <typename T, typename S, int Q>
+template <typename T, typename S, int Q, bool MF>
 __global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T, S> a)
 {
     using VS = VelSet<Q>;
-    if constexpr (VS::DIM == 3)
-        collide_stream_3d<T, S, Q>(a);
-    else
+    if constexpr (MF) {
+        // One 64 x 36-float transpose region per wavefront for the matrix-core MRT.
+        __shared__ float s_mrt[(WG / 64) * 64 * 36];
+        collide_stream_3d<T, S, Q, true>(a, s_mrt + (threadIdx.x >> 6) * 64 * 36);
+    } else if constexpr (VS::DIM == 3) {
+        collide_stream_3d<T, S, Q, false>(a, nullptr);
+    } else {
         collide_stream_2d<T, S, Q>(a);
+    }
 }
 
 // 2D: four 4x4 blocks per wavefront; neighbour IDs are distributed with ds_bpermute.
@@ -470,6 +604,137 @@ __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int
                   (s[21 + o] + s[1 + o] + s[4 + o] + s[16 + o] - s[5 + o] - s[17 + o] - s[20 + o] - s[0 + o]) * Ix * Iy * Iz;
         }
         Fc[(long long)p * ncm + ccell] = Store<T, S>::enc(val, (T)VS::w(p));
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fine -> coarse then coarse -> fine in one launch: the pair "average(L) in INTERFACE mode, then
+// linear interpolation(L) in INTERFACE mode" that the advance runs between the two sub-cycles of
+// level L+1 (solver_lbm_advance.cu:108-112).  Both act on the INTERFACE parents of level L, and each
+// reads and writes only one parent block and its children: the average writes parent cells whose
+// first child cell is an interface cell (mask 1), the interpolation reads the parent block and
+// writes the children's ghost cells (mask 2).  A workgroup therefore does both for its parents:
+// the averaged values go to the pool and, as the pool will hold them, to LDS, from where the
+// interpolation stage takes them.  Arithmetic and order are those of k_average and k_interp.
+// ---------------------------------------------------------------------------------------------
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(IWG) void k_avg_interp(const int *__restrict__ ids, int n, S *__restrict__ Fp,
+                                                    S *__restrict__ Fc, long long ncm, const int *__restrict__ nbr_child,
+                                                    const int *__restrict__ blk_mask, const int *__restrict__ cells_mask,
+                                                    T ratio_avg, T ratio_int)
+{
+    using VS = VelSet<Q>;
+    using ST = Store<T, S>;
+    constexpr int CELLS = VS::CELLS, DIM = VS::DIM, NCH = VS::CHILDREN;
+    constexpr int GROUP = NCH * CELLS;  // lanes per parent
+    constexpr int PPW = IWG / GROUP;    // parents per workgroup
+    __shared__ T s_all[Q * PPW * CELLS];  // rescaled parent populations for the interpolation
+    __shared__ T s_avg[Q * PPW * CELLS];  // averaged parent populations as stored
+    __shared__ int s_done[PPW * CELLS];   // parent cell was averaged
+    const int g = threadIdx.x / GROUP, r = threadIdx.x % GROUP;
+    const int bi = blockIdx.x * PPW + g;
+    int b = -1;
+    if (bi < n) b = ids[bi];
+    int fc = -1;
+    bool work = false;
+    if (b >= 0) {
+        fc = nbr_child[b];
+        work = fc >= 0 && blk_mask[b] == BLK_INTERFACE;
+    }
+    // ---- average: NCH lanes per parent cell (k_average) ----
+    {
+        const int k = r % NCH, ta = r / NCH;
+        const int lane = threadIdx.x % 64, gl = lane - k;
+        const int I = ta % 4, J = (ta / 4) % 4, K = (DIM == 3) ? ta / 16 : 0;
+        const int cq = (I / 2) + 2 * (J / 2) + 4 * (K / 2);
+        int c0 = 2 * (I % 2) + 4 * (2 * (J % 2));
+        if (DIM == 3) c0 += 4 * 4 * (2 * (K % 2));
+        const long long cb = (long long)(fc + cq) * CELLS;
+        bool avg = work;
+        if (avg) avg = cells_mask[cb + c0] == 1;  // uniform over the NCH lanes of the group
+        const int off = (k & 1) + 4 * ((k >> 1) & 1) + 16 * ((k >> 2) & 1);
+        T sv[Q];
+        if (avg) {
+            const long long ccell = cb + c0 + off;
+            T f[Q];
+#pragma unroll
+            for (int p = 0; p < Q; p++) f[p] = ST::dec(Fc[(long long)p * ncm + ccell], (T)VS::w(p));
+            T rho, u, v, w;
+            moments<T, Q>(f, rho, u, v, w);
+            const T udotu = (DIM == 3) ? u * u + v * v + w * w : u * u + v * v;
+#pragma unroll
+            for (int p = 0; p < Q; p++) {
+                T cdotu = (DIM == 3) ? (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w
+                                     : (T)VS::cx(p) * u + (T)VS::cy(p) * v;
+                T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
+                sv[p] = fe + (f[p] - fe) * ratio_avg;
+            }
+        } else {
+#pragma unroll
+            for (int p = 0; p < Q; p++) sv[p] = (T)0;
+        }
+        const T scale = (DIM == 2) ? (T)(0.25) : (T)(0.125);
+        const long long pcell = (long long)b * CELLS + ta;
+#pragma unroll
+        for (int p = 0; p < Q; p++) {
+            T acc = __shfl(sv[p], gl, 64);
+#pragma unroll
+            for (int j = 1; j < NCH; j++) acc = acc + __shfl(sv[p], gl + j, 64);
+            if (avg && (p % NCH) == k) {
+                const S enc = ST::enc(scale * acc, (T)VS::w(p));
+                Fp[(long long)p * ncm + pcell] = enc;
+                s_avg[p * PPW * CELLS + g * CELLS + ta] = ST::dec(enc, (T)VS::w(p));
+            }
+        }
+        if (k == 0) s_done[g * CELLS + ta] = avg ? 1 : 0;
+    }
+    __syncthreads();
+    // ---- interpolation: parent rescaled into LDS, then one lane per child cell (k_interp) ----
+    const int c = r / CELLS, t = r % CELLS;
+    T *sg = s_all + g * CELLS;
+    if (work && c == 0) {
+        const long long cell = (long long)b * CELLS + t;
+        const bool done = s_done[g * CELLS + t] != 0;
+        T f[Q];
+#pragma unroll
+        for (int p = 0; p < Q; p++)
+            f[p] = done ? s_avg[p * PPW * CELLS + g * CELLS + t] : ST::dec(Fp[(long long)p * ncm + cell], (T)VS::w(p));
+        T rho, u, v, w;
+        moments<T, Q>(f, rho, u, v, w);
+        const T udotu = u * u + v * v + w * w;
+#pragma unroll
+        for (int p = 0; p < Q; p++) {
+            T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+            T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
+            sg[p * PPW * CELLS + t] = fe + (f[p] - fe) * ratio_int;
+        }
+    }
+    __syncthreads();
+    if (!work) return;
+    const long long ccell = (long long)(fc + c) * CELLS + t;
+    if (cells_mask[ccell] != CELL_GHOST) return;
+    const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
+    const T Ix = (T)(-0.25) + (T)(0.5) * (T)I;
+    const T Iy = (T)(-0.25) + (T)(0.5) * (T)J;
+    const T Iz = (DIM == 3) ? (T)(-0.25) + (T)(0.5) * (T)K : (T)0;
+    const int ci = c % 2, cj = (c / 2) % 2, ck = (c / 4) % 2;
+#pragma unroll
+    for (int p = 0; p < Q; p++) {
+        const T *sp = sg + p * PPW * CELLS;
+        T val;
+        if (DIM == 2) {
+            const int o = 2 * ci + 2 * 4 * cj;
+            val = sp[0 + o] + (sp[1 + o] - sp[0 + o]) * Ix + (sp[4 + o] - sp[0 + o]) * Iy +
+                  (sp[5 + o] - sp[4 + o] - sp[1 + o] + sp[0 + o]) * Ix * Iy;
+        } else {
+            const int o = 2 * ci + 2 * 4 * cj + 2 * 4 * 4 * ck;
+            val = sp[0 + o] + (sp[1 + o] - sp[0 + o]) * Ix + (sp[4 + o] - sp[0 + o]) * Iy + (sp[16 + o] - sp[0 + o]) * Iz +
+                  (sp[5 + o] - sp[1 + o] - sp[4 + o] + sp[0 + o]) * Ix * Iy +
+                  (sp[17 + o] - sp[1 + o] - sp[16 + o] + sp[0 + o]) * Ix * Iz +
+                  (sp[20 + o] - sp[4 + o] - sp[16 + o] + sp[0 + o]) * Iy * Iz +
+                  (sp[21 + o] + sp[1 + o] + sp[4 + o] + sp[16 + o] - sp[5 + o] - sp[17 + o] - sp[20 + o] - sp[0 + o]) * Ix * Iy * Iz;
+        }
+        Fc[(long long)p * ncm + ccell] = ST::enc(val, (T)VS::w(p));
     }
 }
 
@@ -709,6 +974,9 @@ static int collide_stream_t(Ctx *c, int L)
             const T rate = rc == 1 ? a.omeg : (T)s[rc - 2];
             a.ck[k] = (T)((double)rate / (double)mrt_norm<Q>(k));
         }
+        a.rates[0] = (T)0;
+        a.rates[1] = a.omeg;
+        for (int k = 0; k < 6; k++) a.rates[2 + k] = (T)s[k];
     }
     a.U = (T)c->P.bc_velocity;
     a.bc_preset = c->P.bc_preset;
@@ -724,19 +992,24 @@ static int collide_stream_t(Ctx *c, int L)
     const int slot = L == c->MAX_LEVELS - 1 ? 0 : 1 + L;
     hipStream_t s = sched_begin(c, slot, rl, wl);
     const int grid = (nwg(n, VelSet<Q>::CELLS) + 7) / 8 * 8;  // a multiple of the XCD count
+    // The matrix-core MRT variant: FP32 compute, 3D sets of 19 / 27 directions, when enabled.
+    constexpr bool MF_OK = sizeof(T) == 4 && (Q == 19 || Q == 27);
+    const bool mf = MF_OK && a.mrt && c->mrt_mfma;
+    auto kern = mf ? (void (*)(StepArgs<T, S>))k_collide_stream<T, S, Q, MF_OK>
+                   : (void (*)(StepArgs<T, S>))k_collide_stream<T, S, Q, false>;
     if (c->timing.enabled) {
         // Events recorded by the dispatch itself: they bracket the kernel's execution.
         hipEvent_t e0 = nullptr, e1 = nullptr;
         AGAL_HIP(c, hipEventCreate(&e0));
         AGAL_HIP(c, hipEventCreate(&e1));
-        hipExtLaunchKernelGGL((k_collide_stream<T, S, Q>), dim3(grid), dim3(WG), 0, s, e0, e1, 0, a);
+        hipExtLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, s, e0, e1, 0, a);
         AGAL_HIP(c, hipGetLastError());
         c->timing.ev_pairs.push_back(e0);
         c->timing.ev_pairs.push_back(e1);
         c->timing.fused_launches++;
         c->timing.fused_cells += (double)n * VelSet<Q>::CELLS;
     } else {
-        hipLaunchKernelGGL((k_collide_stream<T, S, Q>), dim3(grid), dim3(WG), 0, s, a);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, s, a);
         AGAL_HIP(c, hipGetLastError());
     }
     hipLaunchKernelGGL((k_fixup<T, S, Q>), dim3(1), dim3(WG), 0, s, a.B, a.ncm, a.exc_idx, a.exc_val, a.exc_count,
@@ -808,6 +1081,27 @@ static int average_t(Ctx *c, int L, int var)
     return sched_done(c, s, 0, rl, wl);
 }
 
+template <typename T, typename S, int Q>
+static int avg_interp_t(Ctx *c, int L)
+{
+    if (L >= c->MAX_LEVELS - 1) return AGAL_OK;
+    if (c->P.interp_order == AGAL_INTERP_CUBIC) {  // cubic: BLOCK averaging and the cubic kernel
+        int st = average_t<T, S, Q>(c, L, AGAL_AVERAGE_BLOCK);
+        return st ? st : interpolate_t<T, S, Q>(c, L, AGAL_INTERP_INTERFACE);
+    }
+    const int n = c->n_iface[L];
+    if (n <= 0) return AGAL_OK;
+    const T ratio_avg = (T)c->tau_vec[L] / (T)c->tau_vec[L + 1];
+    const T ratio_int = (T)c->tau_vec[L + 1] / (T)c->tau_vec[L];
+    const unsigned m = res_bit(L, c->cur[L]) | res_bit(L + 1, c->cur[L + 1]);
+    hipStream_t s = sched_begin(c, 0, m, m);
+    hipLaunchKernelGGL((k_avg_interp<T, S, Q>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, c->iface_L(L), n,
+                       Fbuf<S>(c, c->cur[L]), Fbuf<S>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
+                       c->cells_mask, ratio_avg, ratio_int);
+    AGAL_HIP(c, hipGetLastError());
+    return sched_done(c, s, 0, m, m);
+}
+
 #define AGAL_DISPATCH_Q(c, FN, T, S, ...)                                                  \
     do {                                                                                   \
         if ((c)->Q == 9) return FN<T, S, 9>(__VA_ARGS__);                                  \
@@ -830,5 +1124,6 @@ int launch_refresh(Ctx *c, int L) { AGAL_DISPATCH(c, refresh_t, c, L); }
 int launch_collide_stream(Ctx *c, int L) { AGAL_DISPATCH(c, collide_stream_t, c, L); }
 int launch_interpolate(Ctx *c, int L, int var) { AGAL_DISPATCH(c, interpolate_t, c, L, var); }
 int launch_average(Ctx *c, int L, int var) { AGAL_DISPATCH(c, average_t, c, L, var); }
+int launch_average_interpolate(Ctx *c, int L) { AGAL_DISPATCH(c, avg_interp_t, c, L); }
 
 }  // namespace agal

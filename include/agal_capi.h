@@ -323,6 +323,13 @@ int  agal_download_gap_set(agal_ctx *ctx, int *dst, int n);
  * AGAL_OPT_GRAPHS: 1 (default) records each coarse step of agal_advance / agal_advance_loop as a
  * HIP graph and replays it until the mesh changes; 0 launches every kernel directly. */
 #define AGAL_OPT_GRAPHS 1
+/* AGAL_OPT_MRT_MFMA: 1 runs the MRT moment transform of D3Q19 / D3Q27 FP32-compute steps (FP32, FP16
+ * and BF16 pools) on matrix cores: per block, (moments x directions) . (directions x 64 cells)
+ * products in v_mfma_f32_16x16x32_bf16 with the non-equilibrium parts and the relaxed moments
+ * split into two bf16 terms (16 significant bits of a small correction; FP32 sums in the hardware's
+ * order).  0 (default) keeps the vector path, whose sums follow a fixed order shared with the CPU
+ * restatement. */
+#define AGAL_OPT_MRT_MFMA 2
 int  agal_set_option(agal_ctx *ctx, int option, long long value);
 
 /* ---- measurement ----------------------------------------------------------------------------- */
