@@ -462,17 +462,26 @@ __global__ void k_block_aux(const int *nbr, const int *nbr_child, const int *blk
     has_solid[kap] = hs;
 }
 
+// simple: the block and every existing neighbour participate and hold no solid cell (the fused
+// step then skips the guard reads).  2: in addition every neighbour slot that the velocity set
+// links to exists (so no link leaves the domain or the level) and slot 13 is the block itself --
+// the fused step's interior path.  max_nz: most nonzero components of a link (2 for D3Q19).
 template <int DIM>
-__global__ void k_block_simple(const int *nbr_slot, const int *part, const int *has_solid, int *simple, int n)
+__global__ void k_block_simple(const int *nbr_slot, const int *part, const int *has_solid, int *simple, int n,
+                               int max_nz)
 {
     constexpr int QMAX = DIM == 3 ? 27 : 9;
     int kap = blockIdx.x * AWG + threadIdx.x;
     if (kap >= n) return;
     int s = part[kap] && !has_solid[kap];
+    bool all = true;
     for (int S = 0; S < QMAX && s; S++) {
         int nb = nbr_slot[(long long)kap * QMAX + S];
         if (nb >= 0 && (!part[nb] || has_solid[nb])) s = 0;
+        const int nz = (S % 3 != 1) + ((S / 3) % 3 != 1) + (DIM == 3 ? (S / 9 != 1) : 0);
+        if (nb < 0 && nz <= max_nz) all = false;
     }
+    if (s && all && nbr_slot[(long long)kap * QMAX + (QMAX - 1) / 2] == kap) s = 2;
     simple[kap] = s;
 }
 
@@ -666,12 +675,12 @@ int rebuild_level_lists(Ctx *c, long long range_hint)
         hipLaunchKernelGGL((k_block_aux<3>), dim3(grid1(n)), dim3(AWG), 0, c->stream, c->nbr, c->nbr_child, c->blk_mask,
                            c->cells_mask, c->nbr_slot, c->part, c->scan_a, c->n_maxcblocks, n);
         hipLaunchKernelGGL((k_block_simple<3>), dim3(grid1(n)), dim3(AWG), 0, c->stream, c->nbr_slot, c->part, c->scan_a,
-                           c->simple, n);
+                           c->simple, n, c->Q == 27 ? 3 : 2);
     } else {
         hipLaunchKernelGGL((k_block_aux<2>), dim3(grid1(n)), dim3(AWG), 0, c->stream, c->nbr, c->nbr_child, c->blk_mask,
                            c->cells_mask, c->nbr_slot, c->part, c->scan_a, c->n_maxcblocks, n);
         hipLaunchKernelGGL((k_block_simple<2>), dim3(grid1(n)), dim3(AWG), 0, c->stream, c->nbr_slot, c->part, c->scan_a,
-                           c->simple, n);
+                           c->simple, n, 2);
     }
     AGAL_HIP(c, hipGetLastError());
     return AGAL_OK;

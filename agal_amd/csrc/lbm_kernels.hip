@@ -28,6 +28,59 @@ namespace agal {
 
 constexpr int WG = 256;
 
+// Moments in the reference's order (p-order sums, then division): Cu_RefreshVariables (:110-159).
+// The reference adds c * f for every component, zeros included; those terms are left out here.
+// A sum that starts at +0 never becomes -0 under round-to-nearest, so adding 0 * f = +-0 leaves
+// it unchanged: for finite populations the results are the same bits, at about a third fewer
+// vector instructions.
+template <typename T, int Q>
+__device__ __forceinline__ void moments(const T *f, T &rho, T &u, T &v, T &w)
+{
+    using VS = VelSet<Q>;
+    rho = 0;
+    u = 0;
+    v = 0;
+    w = 0;
+#pragma unroll
+    for (int p = 0; p < Q; p++) {
+        rho += f[p];
+        if (VS::cx(p) > 0) u += f[p];
+        if (VS::cx(p) < 0) u -= f[p];
+        if (VS::cy(p) > 0) v += f[p];
+        if (VS::cy(p) < 0) v -= f[p];
+        if (VS::DIM == 3 && VS::cz(p) > 0) w += f[p];
+        if (VS::DIM == 3 && VS::cz(p) < 0) w -= f[p];
+    }
+    u /= rho;
+    v /= rho;
+    if (VS::DIM == 3) w /= rho;
+}
+
+// c_p . u in the reference's left-to-right order (cx u + cy v) + cz w, without the terms of zero
+// components.  A nonzero sum is unchanged by them; a zero sum may differ in sign only, which the
+// equilibrium polynomial 1 + 3 c.u + 4.5 (c.u)^2 does not see.
+template <typename T, int Q>
+__device__ __forceinline__ T cdot(int p, T u, T v, T w)
+{
+    using VS = VelSet<Q>;
+    T s = (T)0;
+    bool first = true;
+    if (VS::cx(p) != 0) {
+        s = VS::cx(p) > 0 ? u : -u;
+        first = false;
+    }
+    if (VS::cy(p) != 0) {
+        const T t = VS::cy(p) > 0 ? v : -v;
+        s = first ? t : s + t;
+        first = false;
+    }
+    if (VS::DIM == 3 && VS::cz(p) != 0) {
+        const T t = VS::cz(p) > 0 ? w : -w;
+        s = first ? t : s + t;
+    }
+    return s;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Initial conditions: Cu_SetInitialConditions (solver_lbm_kernels.cu:13-77), uniform state hook.
 // ---------------------------------------------------------------------------------------------
@@ -45,33 +98,13 @@ __global__ __launch_bounds__(WG) void k_set_ic(const int *__restrict__ ids, int 
     const int m = cells_mask[cell];
 #pragma unroll
     for (int p = 0; p < Q; p++) {
-        T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+        T cdotu = cdot<T, Q>(p, u, v, w);
         T f = (T)(-1.0);
         if (m != CELL_SOLID) f = rho * (T)VS::w(p) * ((T)1.0 + (T)3.0 * cdotu + (T)4.5 * cdotu * cdotu - (T)1.5 * udotu);
         F[(long long)p * ncm + cell] = Store<T, S>::enc(f, (T)VS::w(p));
     }
 }
 
-// Moments in the reference's order (p-order sums, then division): Cu_RefreshVariables (:110-159).
-template <typename T, int Q>
-__device__ __forceinline__ void moments(const T *f, T &rho, T &u, T &v, T &w)
-{
-    using VS = VelSet<Q>;
-    rho = 0;
-    u = 0;
-    v = 0;
-    w = 0;
-#pragma unroll
-    for (int p = 0; p < Q; p++) {
-        rho += f[p];
-        u += (T)VS::cx(p) * f[p];
-        v += (T)VS::cy(p) * f[p];
-        if (VS::DIM == 3) w += (T)VS::cz(p) * f[p];
-    }
-    u /= rho;
-    v /= rho;
-    if (VS::DIM == 3) w /= rho;
-}
 
 template <typename T, typename S, int Q>
 __global__ __launch_bounds__(WG) void k_refresh(const int *__restrict__ ids, int n, const S *__restrict__ F,
@@ -148,7 +181,7 @@ __device__ __forceinline__ void collide_cell(const StepArgs<T, S> &a, const T *f
     if (!a.mrt) {
 #pragma unroll
         for (int p = 0; p < Q; p++) {
-            T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+            T cdotu = cdot<T, Q>(p, u, v, w);
             fs[p] = f[p] * a.omegp + ((T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu)) * a.omeg;
         }
         return;
@@ -159,7 +192,7 @@ __device__ __forceinline__ void collide_cell(const StepArgs<T, S> &a, const T *f
     T d[Q];
 #pragma clang loop unroll(full)
     for (int p = 0; p < Q; p++) {
-        T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+        T cdotu = cdot<T, Q>(p, u, v, w);
         d[p] = f[p] - (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
     }
     T r[Q];
@@ -354,7 +387,7 @@ __device__ __forceinline__ void push_link(const StepArgs<T, S> &a, int q, T fq, 
     if (exc) {
         int k = atomicAdd(&a.exc_count[0], 1);
         if (k < a.exc_cap) {
-            a.exc_idx[3 * (long long)k + 0] = (int)tgt;  // cell ids fit in int for pools < 2^31 cells
+            a.exc_idx[3 * (long long)k + 0] = (int)tgt;  // cell indices < 2^32, read back as unsigned
             a.exc_idx[3 * (long long)k + 1] = (int)cell;
             a.exc_idx[3 * (long long)k + 2] = q;
             a.exc_val[k] = fq;
@@ -367,9 +400,65 @@ __device__ __forceinline__ void push_link(const StepArgs<T, S> &a, int q, T fq, 
     a.B[at] = ST::enc(fq, (T)VS::w(q));
 }
 
+// Stream of an interior block (block flag simple == 2): every neighbour slot of the velocity set
+// exists, and the block and its neighbours participate with no solid cell.  No link then leaves
+// the domain or the level, no guard read is needed, half-set links always push, and the partner
+// side pushes unless its stored value fails the guard (then the link goes to the exception list,
+// as in push_link).  The target block comes from the wave's neighbour table (lane s holds slot s)
+// through ds_bpermute; the target cell from a carry-free add of the link to the lane's (I,J,K)
+// 2-bit fields.  Same stores, same values as push_link on these blocks, with about a fifth of its
+// vector instructions.
+template <typename T, typename S, int Q>
+__device__ __forceinline__ void push_interior(const StepArgs<T, S> &a, const T *fs, int my_nb, int t, unsigned cell)
+{
+    using VS = VelSet<Q>;
+    using ST = Store<T, S>;
+    const int I = t & 3, J = (t >> 2) & 3, K = t >> 4;
+    // ds_bpermute byte steps to the neighbour slot along each axis, nonzero on the block's faces.
+    const int xp = I == 3 ? 4 : 0, xm = I == 0 ? -4 : 0;
+    const int yp = J == 3 ? 12 : 0, ym = J == 0 ? -12 : 0;
+    const int zp = K == 3 ? 36 : 0, zm = K == 0 ? -36 : 0;
+    const unsigned tl = (unsigned)t & 0x15u, th = (unsigned)t & 0x2Au;
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+        const int cx = VS::cx(q), cy = VS::cy(q), cz = VS::cz(q);
+        int sa = 13 * 4;
+        if (cx > 0) sa += xp;
+        if (cx < 0) sa += xm;
+        if (cy > 0) sa += yp;
+        if (cy < 0) sa += ym;
+        if (cz > 0) sa += zp;
+        if (cz < 0) sa += zm;
+        const unsigned nbk = (unsigned)__builtin_amdgcn_ds_bpermute(sa, my_nb);
+        // ((I+cx)&3) + 4((J+cy)&3) + 16((K+cz)&3): field-wise sum without carries between fields.
+        const unsigned d = (unsigned)((cx & 3) | ((cy & 3) << 2) | ((cz & 3) << 4));
+        const unsigned tgt = (nbk << 6) + ((tl + (d & 0x15u)) ^ th ^ (d & 0x2Au));
+        const S e = ST::enc(fs[q], (T)VS::w(q));
+        bool push = true;
+        if (!VS::half(q)) {
+            // Partner of the link initiated by the cell x + c_q: the guard on the stored value.
+            const T fq = ST::dec(e, (T)VS::w(q));
+            if (!(fq > (T)0)) {
+                push = false;
+                int k = atomicAdd(&a.exc_count[0], 1);
+                if (k < a.exc_cap) {
+                    a.exc_idx[3 * (long long)k + 0] = (int)tgt;
+                    a.exc_idx[3 * (long long)k + 1] = (int)cell;
+                    a.exc_idx[3 * (long long)k + 2] = q;
+                    a.exc_val[k] = fq;
+                } else {
+                    a.exc_count[1] = 1;
+                }
+            }
+        }
+        if (push) (a.B + (long long)q * a.ncm)[tgt] = e;
+    }
+}
+
 // 3D: one wavefront per block, so the block, its flags and its 27 neighbour IDs are wave-uniform
 // (scalar registers); each lane picks its link's neighbour among at most eight of them with
-// selects on its own cell position -- no LDS traffic and no per-direction waits.
+// selects on its own cell position -- no LDS traffic and no per-direction waits.  Interior blocks
+// take push_interior instead.
 template <typename T, typename S, int Q, bool MF>
 __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, float *lds)
 {
@@ -379,22 +468,21 @@ __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, float
     const int bi = __builtin_amdgcn_readfirstlane(xcd_local_wg(gridDim.x) * (WG / 64) + (int)(threadIdx.x >> 6));
     if (bi >= a.n) return;  // whole wavefronts only; the kernel has no barrier
     const int b = __builtin_amdgcn_readfirstlane(a.ids[bi]);
-    const long long cell = (long long)b * 64 + t;
+    // Cell index within a population row (the pool holds fewer than 2^32 cells, checked at
+    // creation), so rows are addressed as a row base plus a 32-bit index.
+    const unsigned cell = (unsigned)b * 64u + (unsigned)t;
     const int my_nb = t < 27 ? a.nbr_slot[(long long)b * 27 + t] : 0;
     const int simple_u = __builtin_amdgcn_readfirstlane(a.simple[b]);
     const int part_u = __builtin_amdgcn_readfirstlane(a.part[b]);
     const int onb_u = __builtin_amdgcn_readfirstlane(a.onb[b]);
     T f[Q];
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = ST::dec(load_pop(&a.A[(long long)q * a.ncm + cell]), (T)VS::w(q));
+    for (int q = 0; q < Q; q++) f[q] = ST::dec(load_pop(&(a.A + (long long)q * a.ncm)[cell]), (T)VS::w(q));
     const bool simple = simple_u != 0;
     int mask = CELL_INTERIOR;
     if (!simple) mask = a.cells_mask[cell];
     const bool inert = part_u == 0 || mask == CELL_SOLID;
     const bool onb = onb_u == 1 && mask != CELL_SOLID;
-    int nbs[27];
-#pragma unroll
-    for (int s = 0; s < 27; s++) nbs[s] = __builtin_amdgcn_readlane(my_nb, s);
 
     T fs[Q];
     T rho, u, v, w;
@@ -405,7 +493,7 @@ __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, float
         float d[Q], g[Q];
 #pragma unroll
         for (int p = 0; p < Q; p++) {
-            const T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+            const T cdotu = cdot<T, Q>(p, u, v, w);
             const T feq = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
             d[p] = inert ? 0.f : (float)(f[p] - feq);
         }
@@ -417,6 +505,13 @@ __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, float
     }
 
     a.B[cell] = ST::enc(fs[0], (T)VS::w(0));
+    if (simple_u == 2) {
+        push_interior<T, S, Q>(a, fs, my_nb, t, cell);
+        return;
+    }
+    int nbs[27];
+#pragma unroll
+    for (int s = 0; s < 27; s++) nbs[s] = __builtin_amdgcn_readlane(my_nb, s);
     const int I = t & 3, J = (t >> 2) & 3, K = t >> 4;
 #pragma unroll
     for (int q = 1; q < Q; q++) {
@@ -430,7 +525,7 @@ __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, float
                   syz = nbs[13 + 3 * cy + 9 * cz], sx = nbs[13 + cx], sy = nbs[13 + 3 * cy], sz = nbs[13 + 9 * cz];
         const int nbk = ex ? (ey ? (ez ? sxyz : sxy) : (ez ? sxz : sx)) : (ey ? (ez ? syz : sy) : (ez ? sz : b));
         const long long tgt = (long long)nbk * 64 + (((I + cx) & 3) + 4 * ((J + cy) & 3) + 16 * ((K + cz) & 3));
-        push_link<T, S, Q>(a, q, fs[q], nbk, tgt, cell, inert, simple, onb, u, v, w);
+        push_link<T, S, Q>(a, q, fs[q], nbk, tgt, (long long)cell, inert, simple, onb, u, v, w);
     }
 }
 
@@ -514,7 +609,7 @@ __global__ __launch_bounds__(WG) void k_fixup(S *__restrict__ B, long long ncm, 
     using VS = VelSet<Q>;
     const long long n = min((long long)exc_count[0], exc_cap);
     for (long long k = threadIdx.x; k < n; k += WG) {
-        const long long i = exc_idx[3 * k + 0], c = exc_idx[3 * k + 1];
+        const long long i = (unsigned)exc_idx[3 * k + 0], c = (unsigned)exc_idx[3 * k + 1];
         const int q = exc_idx[3 * k + 2];
         int qb = 0;
         T wq = 0;
@@ -573,7 +668,7 @@ __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int
         const T udotu = u * u + v * v + w * w;
 #pragma unroll
         for (int p = 0; p < Q; p++) {
-            T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+            T cdotu = cdot<T, Q>(p, u, v, w);
             T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
             sg[p * PPW * CELLS + t] = fe + (f[p] - fe) * tau_ratio;
         }
@@ -664,8 +759,7 @@ __global__ __launch_bounds__(IWG) void k_avg_interp(const int *__restrict__ ids,
             const T udotu = (DIM == 3) ? u * u + v * v + w * w : u * u + v * v;
 #pragma unroll
             for (int p = 0; p < Q; p++) {
-                T cdotu = (DIM == 3) ? (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w
-                                     : (T)VS::cx(p) * u + (T)VS::cy(p) * v;
+                T cdotu = cdot<T, Q>(p, u, v, w);
                 T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
                 sv[p] = fe + (f[p] - fe) * ratio_avg;
             }
@@ -704,7 +798,7 @@ __global__ __launch_bounds__(IWG) void k_avg_interp(const int *__restrict__ ids,
         const T udotu = u * u + v * v + w * w;
 #pragma unroll
         for (int p = 0; p < Q; p++) {
-            T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+            T cdotu = cdot<T, Q>(p, u, v, w);
             T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
             sg[p * PPW * CELLS + t] = fe + (f[p] - fe) * ratio_int;
         }
@@ -783,7 +877,7 @@ __global__ __launch_bounds__(IWG) void k_interp_cubic(const int *__restrict__ id
         const T udotu = u * u + v * v + w * w;
 #pragma unroll
         for (int p = 0; p < Q; p++) {
-            T cdotu = (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w;
+            T cdotu = cdot<T, Q>(p, u, v, w);
             T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
             s_all[(p * PPW + g) * CELLS + t] = fe + (f[p] - fe) * tau_ratio;
         }
@@ -886,8 +980,7 @@ __global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int
         const T udotu = (DIM == 3) ? u * u + v * v + w * w : u * u + v * v;
 #pragma unroll
         for (int p = 0; p < Q; p++) {
-            T cdotu = (DIM == 3) ? (T)VS::cx(p) * u + (T)VS::cy(p) * v + (T)VS::cz(p) * w
-                                 : (T)VS::cx(p) * u + (T)VS::cy(p) * v;
+            T cdotu = cdot<T, Q>(p, u, v, w);
             T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
             s[p] = fe + (f[p] - fe) * tau_ratio;
         }
