@@ -88,6 +88,8 @@ struct Ctx {
     int *ref = nullptr, *level = nullptr, *onb = nullptr, *blk_mask = nullptr, *part = nullptr;
     int *simple = nullptr;
     int *id_set = nullptr, *gap_set = nullptr, *iface = nullptr;
+    // Per level, the same blocks as id_set in octree traversal order (the fused step's list).
+    int *order = nullptr;
     // Rebuild scratch.
     unsigned *map_stamp = nullptr;
     int *first_child = nullptr, *scan_a = nullptr, *scan_b = nullptr, *partials = nullptr;
@@ -146,6 +148,7 @@ struct Ctx {
     ResDeps deps[NRES];
 
     int *id_set_L(int L) const { return id_set + (size_t)L * n_maxcblocks; }
+    int *order_L(int L) const { return order + (size_t)L * n_maxcblocks; }
     int *iface_L(int L) const { return iface + (size_t)L * n_maxcblocks; }
 };
 

@@ -170,6 +170,28 @@ struct EmitListEntry {
     int *out;
     __device__ void operator()(long long i, int r) const { out[r] = list[i]; }
 };
+// Blocks of a list whose children are active blocks of level L (the traversal order below).
+struct PredActiveChildren {
+    const int *list, *nbr_child, *ref, *level;
+    int L;
+    __device__ bool operator()(long long i) const
+    {
+        const int fc = nbr_child[list[i]];
+        if (fc < 0) return false;
+        const int r = ref[fc];
+        return r != REF_INACTIVE && r != REF_REMOVE && level[fc] == L;
+    }
+};
+struct EmitChildren {
+    const int *list, *nbr_child;
+    int *out;
+    int nch;
+    __device__ void operator()(long long i, int r) const
+    {
+        const int fc = nbr_child[list[i]];
+        for (int k = 0; k < nch; k++) out[(long long)nch * r + k] = fc + k;
+    }
+};
 // first_child[kap] = gap_set[n_gaps-1-NCH*rank]: the reversed tail of the gap stack, every NCH-th
 // entry (thrust::reverse_copy + Cu_ContractByFrac + scatter, mesh_amr.cu:2334-2352).
 struct EmitFirstChild {
@@ -641,6 +663,35 @@ static int read_scalars(Ctx *c, int n)
 }
 
 // Sorted per-level ID sets, interface lists, id_max, and the per-block auxiliary tables.
+// The fused step's block list of every level in octree traversal order: level 0 in ID order (the
+// root grid, x fastest), then level L as the children of level L-1's list, eight (four in 2D)
+// consecutive IDs per parent in child order.  Neighbouring blocks then run close
+// together on one XCD, where the partial cache lines that their pushes write into each other merge
+// in L2; ID order drifts from that as refinement reuses freed IDs.  Only the order of the fused
+// step's wavefronts changes, not what any of them computes.  A level whose traversal does not
+// reproduce its ID set (blocks without an active parent) keeps the ID order.
+static int build_traversal_order(Ctx *c)
+{
+    int st;
+    const int ML = c->MAX_LEVELS;
+    if (c->n_ids[0] > 0)
+        AGAL_HIP(c, hipMemcpyAsync(c->order_L(0), c->id_set_L(0), sizeof(int) * c->n_ids[0], hipMemcpyDeviceToDevice,
+                                   c->stream));
+    if ((st = launch_fill(c, c->dscal + 16, 16, 0))) return st;
+    for (int L = 1; L < ML; L++) {
+        st = compact(c, PredActiveChildren{c->order_L(L - 1), c->nbr_child, c->ref, c->level, L}, c->n_ids[L - 1],
+                     EmitChildren{c->order_L(L - 1), c->nbr_child, c->order_L(L), c->NCH}, c->dscal + 16 + L);
+        if (st) return st;
+    }
+    AGAL_HIP(c, hipMemcpyAsync(c->hscal + 16, c->dscal + 16, sizeof(int) * 16, hipMemcpyDeviceToHost, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    for (int L = 1; L < ML; L++)
+        if ((long long)c->hscal[16 + L] * c->NCH != c->n_ids[L] && c->n_ids[L] > 0)
+            AGAL_HIP(c, hipMemcpyAsync(c->order_L(L), c->id_set_L(L), sizeof(int) * c->n_ids[L], hipMemcpyDeviceToDevice,
+                                       c->stream));
+    return AGAL_OK;
+}
+
 int rebuild_level_lists(Ctx *c, long long range_hint)
 {
     int st;
@@ -667,6 +718,7 @@ int rebuild_level_lists(Ctx *c, long long range_hint)
             AGAL_HIP(c, hipMemcpyAsync(&last[L], c->id_set_L(L) + c->n_ids[L] - 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     if ((st = read_scalars(c, 16))) return st;
     for (int L = 0; L < ML; L++) c->n_iface[L] = c->hscal[8 + L];
+    if ((st = build_traversal_order(c))) return st;
     c->mesh_epoch++;  // recorded step graphs carry the old list sizes
     for (int L = 1; L < ML; L++) c->id_max[L] = c->n_ids[L] > 0 ? last[L] + 1 : 0;
     c->id_max[ML] = *std::max_element(c->id_max.begin(), c->id_max.begin() + ML);

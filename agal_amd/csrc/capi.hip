@@ -168,7 +168,7 @@ void free_ctx(Ctx *c)
 {
     if (!c) return;
     void *ptrs[] = {c->F[0], c->F[1], c->M, c->cells_mask, c->X, c->nbr, c->nbr_child, c->nbr_slot, c->ref, c->level,
-                    c->onb, c->blk_mask, c->part, c->simple, c->id_set, c->gap_set, c->iface, c->map_stamp,
+                    c->onb, c->blk_mask, c->part, c->simple, c->id_set, c->order, c->gap_set, c->iface, c->map_stamp,
                     c->first_child, c->scan_a, c->scan_b, c->partials, c->dscal, c->exc_idx, c->exc_val,
                     c->exc_count};
     for (void *p : ptrs)
@@ -788,6 +788,7 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     ALLOC(c->part, sizeof(int) * nmax);
     ALLOC(c->simple, sizeof(int) * nmax);
     ALLOC(c->id_set, sizeof(int) * c->MAX_LEVELS * nmax);
+    ALLOC(c->order, sizeof(int) * c->MAX_LEVELS * nmax);
     ALLOC(c->iface, sizeof(int) * c->MAX_LEVELS * nmax);
     ALLOC(c->gap_set, sizeof(int) * nmax);
     ALLOC(c->map_stamp, sizeof(unsigned) * nmax);

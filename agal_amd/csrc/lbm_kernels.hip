@@ -628,12 +628,47 @@ __global__ __launch_bounds__(WG) void k_fixup(S *__restrict__ B, long long ncm, 
 }
 
 // ---------------------------------------------------------------------------------------------
-// Coarse -> fine: Cu_Interpolate_Linear (solver_lbm_kernels_comm.cu:13-138).  A 512-thread
-// workgroup owns one parent block in 3D (eight 4x4 parents in 2D): the first CELLS lanes of each
-// parent put the Q rescaled parent populations in LDS, then every child cell gets one lane, so the
-// eight children are written by eight wavefronts in parallel with row-contiguous stores.
+// Coarse -> fine: Cu_Interpolate_Linear (solver_lbm_kernels_comm.cu:13-138).  The first CELLS
+// lanes of each parent put the Q rescaled parent populations in LDS; then every lane takes one
+// cell in each half of the children (children c and c + NCH/2), so a parent's children are
+// written by half as many wavefronts with row-contiguous stores.  Half-width groups keep a level's
+// interface parents within one round of resident wavefronts (a 3D parent is one 256-lane
+// workgroup; 2D packs eight 4x4 parents).
 // ---------------------------------------------------------------------------------------------
-constexpr int IWG = 512;
+constexpr int IWG = 256;
+
+// Trilinear (bilinear) value of child c's cell t from the parent's rescaled populations s (row
+// stride `stride` per direction) in the reference's operation order, stored into every row.
+template <typename T, typename S, int Q>
+__device__ __forceinline__ void interp_child(const T *sg, int stride, int c, int t, S *__restrict__ Fc, long long ncm,
+                                             long long ccell)
+{
+    using VS = VelSet<Q>;
+    constexpr int DIM = VS::DIM;
+    const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
+    const T Ix = (T)(-0.25) + (T)(0.5) * (T)I;
+    const T Iy = (T)(-0.25) + (T)(0.5) * (T)J;
+    const T Iz = (DIM == 3) ? (T)(-0.25) + (T)(0.5) * (T)K : (T)0;
+    const int ci = c % 2, cj = (c / 2) % 2, ck = (c / 4) % 2;
+#pragma unroll
+    for (int p = 0; p < Q; p++) {
+        const T *s = sg + p * stride;
+        T val;
+        if (DIM == 2) {
+            const int o = 2 * ci + 2 * 4 * cj;
+            val = s[0 + o] + (s[1 + o] - s[0 + o]) * Ix + (s[4 + o] - s[0 + o]) * Iy +
+                  (s[5 + o] - s[4 + o] - s[1 + o] + s[0 + o]) * Ix * Iy;
+        } else {
+            const int o = 2 * ci + 2 * 4 * cj + 2 * 4 * 4 * ck;
+            val = s[0 + o] + (s[1 + o] - s[0 + o]) * Ix + (s[4 + o] - s[0 + o]) * Iy + (s[16 + o] - s[0 + o]) * Iz +
+                  (s[5 + o] - s[1 + o] - s[4 + o] + s[0 + o]) * Ix * Iy +
+                  (s[17 + o] - s[1 + o] - s[16 + o] + s[0 + o]) * Ix * Iz +
+                  (s[20 + o] - s[4 + o] - s[16 + o] + s[0 + o]) * Iy * Iz +
+                  (s[21 + o] + s[1 + o] + s[4 + o] + s[16 + o] - s[5 + o] - s[17 + o] - s[20 + o] - s[0 + o]) * Ix * Iy * Iz;
+        }
+        Fc[(long long)p * ncm + ccell] = Store<T, S>::enc(val, (T)VS::w(p));
+    }
+}
 
 template <typename T, typename S, int Q, bool ADDED>
 __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int n, const S *__restrict__ Fp,
@@ -642,8 +677,8 @@ __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int
                                                 const int *__restrict__ cells_mask, T tau_ratio)
 {
     using VS = VelSet<Q>;
-    constexpr int CELLS = VS::CELLS, DIM = VS::DIM, NCH = VS::CHILDREN;
-    constexpr int GROUP = NCH * CELLS;   // lanes per parent: one per child cell
+    constexpr int CELLS = VS::CELLS, NCH = VS::CHILDREN, HALF = NCH / 2;
+    constexpr int GROUP = HALF * CELLS;  // lanes per parent: one per cell of half the children
     constexpr int PPW = IWG / GROUP;     // parents per workgroup
     __shared__ T s_all[Q * PPW * CELLS];
     const int g = threadIdx.x / GROUP, r = threadIdx.x % GROUP;
@@ -657,6 +692,12 @@ __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int
         fc = nbr_child[b];
         work = fc >= 0 && (ADDED ? (ref[b] == REF_MARK_REFINE) : (blk_mask[b] == BLK_INTERFACE));
     }
+    // The children's cell masks, fetched before the barrier.
+    bool put[2] = {false, false};
+    if (work)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            put[h] = ADDED || cells_mask[(long long)(fc + c + h * HALF) * CELLS + t] == CELL_GHOST;
     T *sg = s_all + g * CELLS;
     if (work && c == 0) {
         const long long cell = (long long)b * CELLS + t;
@@ -675,30 +716,34 @@ __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int
     }
     __syncthreads();
     if (!work) return;
-    const long long ccell = (long long)(fc + c) * CELLS + t;
-    if (!ADDED && cells_mask[ccell] != CELL_GHOST) return;
-    const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
-    const T Ix = (T)(-0.25) + (T)(0.5) * (T)I;
-    const T Iy = (T)(-0.25) + (T)(0.5) * (T)J;
-    const T Iz = (DIM == 3) ? (T)(-0.25) + (T)(0.5) * (T)K : (T)0;
-    const int ci = c % 2, cj = (c / 2) % 2, ck = (c / 4) % 2;
 #pragma unroll
-    for (int p = 0; p < Q; p++) {
-        const T *s = sg + p * PPW * CELLS;
-        T val;
-        if (DIM == 2) {
-            const int o = 2 * ci + 2 * 4 * cj;
-            val = s[0 + o] + (s[1 + o] - s[0 + o]) * Ix + (s[4 + o] - s[0 + o]) * Iy +
-                  (s[5 + o] - s[4 + o] - s[1 + o] + s[0 + o]) * Ix * Iy;
-        } else {
-            const int o = 2 * ci + 2 * 4 * cj + 2 * 4 * 4 * ck;
-            val = s[0 + o] + (s[1 + o] - s[0 + o]) * Ix + (s[4 + o] - s[0 + o]) * Iy + (s[16 + o] - s[0 + o]) * Iz +
-                  (s[5 + o] - s[1 + o] - s[4 + o] + s[0 + o]) * Ix * Iy +
-                  (s[17 + o] - s[1 + o] - s[16 + o] + s[0 + o]) * Ix * Iz +
-                  (s[20 + o] - s[4 + o] - s[16 + o] + s[0 + o]) * Iy * Iz +
-                  (s[21 + o] + s[1 + o] + s[4 + o] + s[16 + o] - s[5 + o] - s[17 + o] - s[20 + o] - s[0 + o]) * Ix * Iy * Iz;
+    for (int h = 0; h < 2; h++)
+        if (put[h]) interp_child<T, S, Q>(sg, PPW * CELLS, c + h * HALF, t, Fc, ncm, (long long)(fc + c + h * HALF) * CELLS + t);
+}
+
+// Sum over the NG lanes of each lane group (NG consecutive lanes, NG = 8 in 3D, 4 in 2D) of the
+// values v[p], left to right in lane order as the reference sums the children (comm.cu:330-339),
+// for the directions p = k (mod NG) of lane k of the group: the wavefront writes its Q x 64 values
+// to its own LDS rows (stride 65: the group reads fall in distinct banks) and each lane reads back
+// the NG values of its own directions.  `lds` is this wavefront's Q * 65-float region.
+template <typename T, int Q, int NG>
+__device__ __forceinline__ void group_sums(const T *v, T *acc, T *lds)
+{
+    const int lane = threadIdx.x & 63, k = lane % NG, gl = lane - k;
+    wave_lds_sync();  // earlier reads of this region are done
+#pragma unroll
+    for (int p = 0; p < Q; p++) lds[p * 65 + lane] = v[p];
+    wave_lds_sync();
+#pragma unroll
+    for (int m = 0; m < (Q + NG - 1) / NG; m++) {
+        const int p = k + NG * m;
+        if (p < Q) {
+            const T *row = lds + p * 65 + gl;
+            T a = row[0];
+#pragma unroll
+            for (int j = 1; j < NG; j++) a = a + row[j];
+            acc[m] = a;
         }
-        Fc[(long long)p * ncm + ccell] = Store<T, S>::enc(val, (T)VS::w(p));
     }
 }
 
@@ -720,12 +765,13 @@ __global__ __launch_bounds__(IWG) void k_avg_interp(const int *__restrict__ ids,
 {
     using VS = VelSet<Q>;
     using ST = Store<T, S>;
-    constexpr int CELLS = VS::CELLS, DIM = VS::DIM, NCH = VS::CHILDREN;
-    constexpr int GROUP = NCH * CELLS;  // lanes per parent
-    constexpr int PPW = IWG / GROUP;    // parents per workgroup
+    constexpr int CELLS = VS::CELLS, DIM = VS::DIM, NCH = VS::CHILDREN, HALF = NCH / 2;
+    constexpr int GROUP = HALF * CELLS;  // lanes per parent
+    constexpr int PPW = IWG / GROUP;     // parents per workgroup
     __shared__ T s_all[Q * PPW * CELLS];  // rescaled parent populations for the interpolation
     __shared__ T s_avg[Q * PPW * CELLS];  // averaged parent populations as stored
     __shared__ int s_done[PPW * CELLS];   // parent cell was averaged
+    __shared__ T s_red[(IWG / 64) * Q * 65];  // per-wavefront rows of the group sums
     const int g = threadIdx.x / GROUP, r = threadIdx.x % GROUP;
     const int bi = blockIdx.x * PPW + g;
     int b = -1;
@@ -736,55 +782,73 @@ __global__ __launch_bounds__(IWG) void k_avg_interp(const int *__restrict__ ids,
         fc = nbr_child[b];
         work = fc >= 0 && blk_mask[b] == BLK_INTERFACE;
     }
-    // ---- average: NCH lanes per parent cell (k_average) ----
+    // ---- average: NCH lanes per parent cell, two parent cells per lane group (k_average) ----
     {
-        const int k = r % NCH, ta = r / NCH;
-        const int lane = threadIdx.x % 64, gl = lane - k;
-        const int I = ta % 4, J = (ta / 4) % 4, K = (DIM == 3) ? ta / 16 : 0;
-        const int cq = (I / 2) + 2 * (J / 2) + 4 * (K / 2);
-        int c0 = 2 * (I % 2) + 4 * (2 * (J % 2));
-        if (DIM == 3) c0 += 4 * 4 * (2 * (K % 2));
-        const long long cb = (long long)(fc + cq) * CELLS;
-        bool avg = work;
-        if (avg) avg = cells_mask[cb + c0] == 1;  // uniform over the NCH lanes of the group
+        const int k = r % NCH, ta0 = r / NCH;  // parent cells ta0 and ta0 + CELLS / 2
         const int off = (k & 1) + 4 * ((k >> 1) & 1) + 16 * ((k >> 2) & 1);
-        T sv[Q];
-        if (avg) {
-            const long long ccell = cb + c0 + off;
-            T f[Q];
+        bool avg[2];
+        long long cb[2];
+        int c0[2];
 #pragma unroll
-            for (int p = 0; p < Q; p++) f[p] = ST::dec(Fc[(long long)p * ncm + ccell], (T)VS::w(p));
-            T rho, u, v, w;
-            moments<T, Q>(f, rho, u, v, w);
-            const T udotu = (DIM == 3) ? u * u + v * v + w * w : u * u + v * v;
-#pragma unroll
-            for (int p = 0; p < Q; p++) {
-                T cdotu = cdot<T, Q>(p, u, v, w);
-                T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
-                sv[p] = fe + (f[p] - fe) * ratio_avg;
-            }
-        } else {
-#pragma unroll
-            for (int p = 0; p < Q; p++) sv[p] = (T)0;
+        for (int h = 0; h < 2; h++) {
+            const int ta = ta0 + h * (CELLS / 2);
+            const int I = ta % 4, J = (ta / 4) % 4, K = (DIM == 3) ? ta / 16 : 0;
+            const int cq = (I / 2) + 2 * (J / 2) + 4 * (K / 2);
+            c0[h] = 2 * (I % 2) + 4 * (2 * (J % 2));
+            if (DIM == 3) c0[h] += 4 * 4 * (2 * (K % 2));
+            cb[h] = (long long)(fc + cq) * CELLS;
+            avg[h] = work && cells_mask[cb[h] + c0[h]] == 1;  // uniform over the NCH lanes of the group
         }
+        T fl[2][Q];
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            if (avg[h])
+#pragma unroll
+                for (int p = 0; p < Q; p++) fl[h][p] = ST::dec(Fc[(long long)p * ncm + cb[h] + c0[h] + off], (T)VS::w(p));
         const T scale = (DIM == 2) ? (T)(0.25) : (T)(0.125);
-        const long long pcell = (long long)b * CELLS + ta;
 #pragma unroll
-        for (int p = 0; p < Q; p++) {
-            T acc = __shfl(sv[p], gl, 64);
+        for (int h = 0; h < 2; h++) {
+            const int ta = ta0 + h * (CELLS / 2);
+            T sv[Q];
+            if (avg[h]) {
+                const T *f = fl[h];
+                T rho, u, v, w;
+                moments<T, Q>(f, rho, u, v, w);
+                const T udotu = (DIM == 3) ? u * u + v * v + w * w : u * u + v * v;
 #pragma unroll
-            for (int j = 1; j < NCH; j++) acc = acc + __shfl(sv[p], gl + j, 64);
-            if (avg && (p % NCH) == k) {
-                const S enc = ST::enc(scale * acc, (T)VS::w(p));
-                Fp[(long long)p * ncm + pcell] = enc;
-                s_avg[p * PPW * CELLS + g * CELLS + ta] = ST::dec(enc, (T)VS::w(p));
+                for (int p = 0; p < Q; p++) {
+                    T cdotu = cdot<T, Q>(p, u, v, w);
+                    T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
+                    sv[p] = fe + (f[p] - fe) * ratio_avg;
+                }
+            } else {
+#pragma unroll
+                for (int p = 0; p < Q; p++) sv[p] = (T)0;
             }
+            const long long pcell = (long long)b * CELLS + ta;
+            T acc[(Q + NCH - 1) / NCH];
+            group_sums<T, Q, NCH>(sv, acc, s_red + (threadIdx.x >> 6) * Q * 65);
+#pragma unroll
+            for (int m = 0; m < (Q + NCH - 1) / NCH; m++) {
+                const int p = k + NCH * m;
+                if (avg[h] && p < Q) {
+                    const T wq = (T)VelSet<Q>::w(p);
+                    const S enc = ST::enc(scale * acc[m], wq);
+                    Fp[(long long)p * ncm + pcell] = enc;
+                    s_avg[p * PPW * CELLS + g * CELLS + ta] = ST::dec(enc, wq);
+                }
+            }
+            if (k == 0) s_done[g * CELLS + ta] = avg[h] ? 1 : 0;
         }
-        if (k == 0) s_done[g * CELLS + ta] = avg ? 1 : 0;
     }
-    __syncthreads();
-    // ---- interpolation: parent rescaled into LDS, then one lane per child cell (k_interp) ----
+    // The children's cell masks for the interpolation stage, fetched before the barrier.
     const int c = r / CELLS, t = r % CELLS;
+    bool put[2] = {false, false};
+    if (work)
+#pragma unroll
+        for (int h = 0; h < 2; h++) put[h] = cells_mask[(long long)(fc + c + h * HALF) * CELLS + t] == CELL_GHOST;
+    __syncthreads();
+    // ---- interpolation: parent rescaled into LDS, then two child cells per lane (k_interp) ----
     T *sg = s_all + g * CELLS;
     if (work && c == 0) {
         const long long cell = (long long)b * CELLS + t;
@@ -805,31 +869,9 @@ __global__ __launch_bounds__(IWG) void k_avg_interp(const int *__restrict__ ids,
     }
     __syncthreads();
     if (!work) return;
-    const long long ccell = (long long)(fc + c) * CELLS + t;
-    if (cells_mask[ccell] != CELL_GHOST) return;
-    const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
-    const T Ix = (T)(-0.25) + (T)(0.5) * (T)I;
-    const T Iy = (T)(-0.25) + (T)(0.5) * (T)J;
-    const T Iz = (DIM == 3) ? (T)(-0.25) + (T)(0.5) * (T)K : (T)0;
-    const int ci = c % 2, cj = (c / 2) % 2, ck = (c / 4) % 2;
 #pragma unroll
-    for (int p = 0; p < Q; p++) {
-        const T *sp = sg + p * PPW * CELLS;
-        T val;
-        if (DIM == 2) {
-            const int o = 2 * ci + 2 * 4 * cj;
-            val = sp[0 + o] + (sp[1 + o] - sp[0 + o]) * Ix + (sp[4 + o] - sp[0 + o]) * Iy +
-                  (sp[5 + o] - sp[4 + o] - sp[1 + o] + sp[0 + o]) * Ix * Iy;
-        } else {
-            const int o = 2 * ci + 2 * 4 * cj + 2 * 4 * 4 * ck;
-            val = sp[0 + o] + (sp[1 + o] - sp[0 + o]) * Ix + (sp[4 + o] - sp[0 + o]) * Iy + (sp[16 + o] - sp[0 + o]) * Iz +
-                  (sp[5 + o] - sp[1 + o] - sp[4 + o] + sp[0 + o]) * Ix * Iy +
-                  (sp[17 + o] - sp[1 + o] - sp[16 + o] + sp[0 + o]) * Ix * Iz +
-                  (sp[20 + o] - sp[4 + o] - sp[16 + o] + sp[0 + o]) * Iy * Iz +
-                  (sp[21 + o] + sp[1 + o] + sp[4 + o] + sp[16 + o] - sp[5 + o] - sp[17 + o] - sp[20 + o] - sp[0 + o]) * Ix * Iy * Iz;
-        }
-        Fc[(long long)p * ncm + ccell] = ST::enc(val, (T)VS::w(p));
-    }
+    for (int h = 0; h < 2; h++)
+        if (put[h]) interp_child<T, S, Q>(sg, PPW * CELLS, c + h * HALF, t, Fc, ncm, (long long)(fc + c + h * HALF) * CELLS + t);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -844,8 +886,10 @@ __global__ __launch_bounds__(IWG) void k_avg_interp(const int *__restrict__ ids,
 #include "cubic_table.inc"
 #undef AGAL_CUBIC_TABLE
 
+constexpr int CWG = 512;  // cubic interpolation: one lane per child cell of a parent
+
 template <typename T, typename S, int Q, bool ADDED>
-__global__ __launch_bounds__(IWG) void k_interp_cubic(const int *__restrict__ ids, int n, const S *__restrict__ Fp,
+__global__ __launch_bounds__(CWG) void k_interp_cubic(const int *__restrict__ ids, int n, const S *__restrict__ Fp,
                                                       S *__restrict__ Fc, long long ncm, const int *__restrict__ nbr_child,
                                                       const int *__restrict__ blk_mask, const int *__restrict__ ref,
                                                       const int *__restrict__ cells_mask, T tau_ratio)
@@ -853,7 +897,7 @@ __global__ __launch_bounds__(IWG) void k_interp_cubic(const int *__restrict__ id
     using VS = VelSet<Q>;
     constexpr int CELLS = VS::CELLS, DIM = VS::DIM, NCH = VS::CHILDREN;
     constexpr int GROUP = NCH * CELLS;
-    constexpr int PPW = IWG / GROUP;
+    constexpr int PPW = CWG / GROUP;
     __shared__ T s_all[Q * PPW * CELLS];  // rescaled parent populations, [p][parent][cell]
     __shared__ T a_all[Q * PPW * CELLS];  // monomial coefficients, [p][parent][i]
     const int g = threadIdx.x / GROUP, r = threadIdx.x % GROUP;
@@ -932,7 +976,9 @@ __global__ __launch_bounds__(IWG) void k_interp_cubic(const int *__restrict__ id
 // Fine -> coarse: Cu_Average (solver_lbm_kernels_comm.cu:174-347).  NG consecutive lanes (NG = 8
 // in 3D, 4 in 2D) serve one parent cell, each lane loading and rescaling one child cell of the
 // cell's 2x2(x2) group; the group sum is formed in the reference's order from the NG lanes by
-// shuffles and each lane stores the directions p = lane (mod NG).  No LDS, no barriers.
+// shuffles and each lane stores the directions p = lane (mod NG).  Each lane group serves two
+// parent cells (t and t + CELLS/2, both loaded before either is reduced), which halves the
+// wavefronts per parent.  No LDS, no barriers.
 // ---------------------------------------------------------------------------------------------
 template <typename T, typename S, int Q>
 __global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int n, S *__restrict__ Fp,
@@ -943,13 +989,12 @@ __global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int
     using VS = VelSet<Q>;
     constexpr int CELLS = VS::CELLS, DIM = VS::DIM;
     constexpr int NG = (DIM == 3) ? 8 : 4;     // child cells per parent cell
-    constexpr int PER_BLOCK = CELLS * NG;       // lanes per parent block
+    constexpr int PER_BLOCK = CELLS / 2 * NG;   // lanes per parent block: two parent cells per lane group
+    __shared__ T s_red[(WG / 64) * Q * 65];     // per-wavefront rows of the group sums
     const int k = threadIdx.x % NG;             // which child cell of the group
-    const int lane = threadIdx.x % 64;
-    const int gl = lane - k;                    // first lane of the group
     const long long gid = (long long)blockIdx.x * WG + threadIdx.x;
     const long long bi = gid / PER_BLOCK;
-    const int t = (int)((gid % PER_BLOCK) / NG);  // parent cell
+    const int t0 = (int)((gid % PER_BLOCK) / NG);  // parent cells t0 and t0 + CELLS / 2
     bool work = bi < n;
     int b = 0, fc = -1;
     if (work) {
@@ -957,46 +1002,60 @@ __global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int
         fc = nbr_child[b];
         work = (fc > -1) && (ave_type == 2 || blk_mask[b] == BLK_INTERFACE);
     }
-    const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
-    // Child block of this cell's quadrant and the first cell of its 2x2(x2) group.
-    const int c = (I / 2) + 2 * (J / 2) + 4 * (K / 2);
-    int c0 = 2 * (I % 2) + 4 * (2 * (J % 2));
-    if (DIM == 3) c0 += 4 * 4 * (2 * (K % 2));
-    const long long cb = (long long)(fc + c) * CELLS;
-    if (work) {
-        int m = cells_mask[cb + c0];
-        if (ave_type > 0 && m < 2) m = 1;
-        work = (m == 1);  // uniform over the NG lanes of the group
-    }
     const int off = (k & 1) + 4 * ((k >> 1) & 1) + 16 * ((k >> 2) & 1);
-    T s[Q];
-    if (work) {
-        const long long ccell = cb + c0 + off;
-        T f[Q];
+    bool avg[2];
+    long long ccell[2];
 #pragma unroll
-        for (int p = 0; p < Q; p++) f[p] = Store<T, S>::dec(Fc[(long long)p * ncm + ccell], (T)VS::w(p));
-        T rho, u, v, w;
-        moments<T, Q>(f, rho, u, v, w);
-        const T udotu = (DIM == 3) ? u * u + v * v + w * w : u * u + v * v;
-#pragma unroll
-        for (int p = 0; p < Q; p++) {
-            T cdotu = cdot<T, Q>(p, u, v, w);
-            T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
-            s[p] = fe + (f[p] - fe) * tau_ratio;
+    for (int h = 0; h < 2; h++) {
+        const int t = t0 + h * (CELLS / 2);
+        const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
+        // Child block of this cell's quadrant and the first cell of its 2x2(x2) group.
+        const int c = (I / 2) + 2 * (J / 2) + 4 * (K / 2);
+        int c0 = 2 * (I % 2) + 4 * (2 * (J % 2));
+        if (DIM == 3) c0 += 4 * 4 * (2 * (K % 2));
+        const long long cb = (long long)(fc + c) * CELLS;
+        avg[h] = false;
+        if (work) {
+            int m = cells_mask[cb + c0];
+            if (ave_type > 0 && m < 2) m = 1;
+            avg[h] = (m == 1);  // uniform over the NG lanes of the group
         }
-    } else {
-#pragma unroll
-        for (int p = 0; p < Q; p++) s[p] = (T)0;
+        ccell[h] = cb + c0 + off;
     }
+    T fl[2][Q];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+        if (avg[h])
+#pragma unroll
+            for (int p = 0; p < Q; p++) fl[h][p] = Store<T, S>::dec(Fc[(long long)p * ncm + ccell[h]], (T)VS::w(p));
     const T scale = (DIM == 2) ? (T)(0.25) : (T)(0.125);
-    const long long pcell = (long long)b * CELLS + t;
 #pragma unroll
-    for (int p = 0; p < Q; p++) {
+    for (int h = 0; h < 2; h++) {
+        T s[Q];
+        if (avg[h]) {
+            const T *f = fl[h];
+            T rho, u, v, w;
+            moments<T, Q>(f, rho, u, v, w);
+            const T udotu = (DIM == 3) ? u * u + v * v + w * w : u * u + v * v;
+#pragma unroll
+            for (int p = 0; p < Q; p++) {
+                T cdotu = cdot<T, Q>(p, u, v, w);
+                T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
+                s[p] = fe + (f[p] - fe) * tau_ratio;
+            }
+        } else {
+#pragma unroll
+            for (int p = 0; p < Q; p++) s[p] = (T)0;
+        }
+        const long long pcell = (long long)b * CELLS + t0 + h * (CELLS / 2);
         // s of child cell offsets 0,1,4,5,16,17,20,21 summed left to right (comm.cu:330-339)
-        T acc = __shfl(s[p], gl, 64);
+        T acc[(Q + NG - 1) / NG];
+        group_sums<T, Q, NG>(s, acc, s_red + (threadIdx.x >> 6) * Q * 65);
 #pragma unroll
-        for (int j = 1; j < NG; j++) acc = acc + __shfl(s[p], gl + j, 64);
-        if (work && (p % NG) == k) Fp[(long long)p * ncm + pcell] = Store<T, S>::enc(scale * acc, (T)VS::w(p));
+        for (int m = 0; m < (Q + NG - 1) / NG; m++) {
+            const int p = k + NG * m;
+            if (avg[h] && p < Q) Fp[(long long)p * ncm + pcell] = Store<T, S>::enc(scale * acc[m], (T)VelSet<Q>::w(p));
+        }
     }
 }
 
@@ -1011,7 +1070,14 @@ static inline int nwg(long long nblocks, int cells) { return (int)((nblocks * ce
 template <int Q>
 static inline int interp_grid(long long nparents)
 {
-    constexpr int ppw = IWG / (VelSet<Q>::CHILDREN * VelSet<Q>::CELLS);
+    constexpr int ppw = IWG / (VelSet<Q>::CHILDREN / 2 * VelSet<Q>::CELLS);
+    return (int)((nparents + ppw - 1) / ppw);
+}
+
+template <int Q>
+static inline int cubic_grid(long long nparents)
+{
+    constexpr int ppw = CWG / (VelSet<Q>::CHILDREN * VelSet<Q>::CELLS);
     return (int)((nparents + ppw - 1) / ppw);
 }
 
@@ -1044,7 +1110,7 @@ static int collide_stream_t(Ctx *c, int L)
     int n = c->n_ids[L];
     if (n <= 0) return AGAL_OK;
     StepArgs<T, S> a;
-    a.ids = c->id_set_L(L);
+    a.ids = c->order_L(L);  // the level's blocks in traversal order
     a.n = n;
     a.A = Fbuf<S>(c, c->cur[L]);
     a.B = Fbuf<S>(c, c->cur[L] ^ 1);
@@ -1131,11 +1197,11 @@ static int interpolate_t(Ctx *c, int L, int var)
     hipStream_t s = sched_begin(c, 0, rl, wl);
     if (c->P.interp_order == AGAL_INTERP_CUBIC) {
         if (var == AGAL_INTERP_ADDED)
-            hipLaunchKernelGGL((k_interp_cubic<T, S, Q, true>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
+            hipLaunchKernelGGL((k_interp_cubic<T, S, Q, true>), dim3(cubic_grid<Q>(n)), dim3(CWG), 0, s, ids, n,
                                Fbuf<S>(c, c->cur[L]), Fbuf<S>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child,
                                c->blk_mask, c->ref, c->cells_mask, ratio);
         else
-            hipLaunchKernelGGL((k_interp_cubic<T, S, Q, false>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
+            hipLaunchKernelGGL((k_interp_cubic<T, S, Q, false>), dim3(cubic_grid<Q>(n)), dim3(CWG), 0, s, ids, n,
                                Fbuf<S>(c, c->cur[L]), Fbuf<S>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child,
                                c->blk_mask, c->ref, c->cells_mask, ratio);
     } else if (var == AGAL_INTERP_ADDED)
@@ -1167,7 +1233,7 @@ static int average_t(Ctx *c, int L, int var)
     if (n <= 0) return AGAL_OK;
     const unsigned rl = res_bit(L + 1, c->cur[L + 1]), wl = res_bit(L, c->cur[L]);
     hipStream_t s = sched_begin(c, 0, rl, wl);
-    hipLaunchKernelGGL((k_average<T, S, Q>), dim3(nwg(n, VelSet<Q>::CELLS * (VelSet<Q>::DIM == 3 ? 8 : 4))), dim3(WG),
+    hipLaunchKernelGGL((k_average<T, S, Q>), dim3(nwg(n, VelSet<Q>::CELLS / 2 * (VelSet<Q>::DIM == 3 ? 8 : 4))), dim3(WG),
                        0, s, ids, n, Fbuf<S>(c, c->cur[L]), Fbuf<S>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child,
                        c->blk_mask, c->cells_mask, var, ratio);
     AGAL_HIP(c, hipGetLastError());
