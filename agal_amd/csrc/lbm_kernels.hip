@@ -637,35 +637,70 @@ __global__ __launch_bounds__(WG) void k_fixup(S *__restrict__ B, long long ncm, 
 // ---------------------------------------------------------------------------------------------
 constexpr int IWG = 256;
 
-// Trilinear (bilinear) value of child c's cell t from the parent's rescaled populations s (row
-// stride `stride` per direction) in the reference's operation order, stored into every row.
+// Interpolation coefficients of the reference's trilinear (bilinear) form, per parent, child
+// (octant) and direction: the corner differences that multiply 1, Ix, Iy, Iz, IxIy, IxIz, IyIz,
+// IxIyIz (1, Ix, Iy, IxIy in 2D), formed once from the parent's rescaled populations s (rows of
+// `stride` per direction) with the reference's operand order, instead of once per child cell.
+template <typename T, int Q>
+struct InterpCoef {
+    static constexpr int DIM = VelSet<Q>::DIM, NCH = VelSet<Q>::CHILDREN, N = DIM == 3 ? 8 : 4;
+};
+
+// Fills coef[(g * Q + p) * NCH + c][0..N) for the PPW parents of the workgroup (tasks strided over
+// the workgroup's lanes).
+template <typename T, int Q, int PPW, int NTHR>
+__device__ __forceinline__ void interp_coefficients(const T *s_all, T *coef)
+{
+    using IC = InterpCoef<T, Q>;
+    constexpr int CELLS = VelSet<Q>::CELLS, NCH = IC::NCH, N = IC::N;
+    for (int task = threadIdx.x; task < PPW * Q * NCH; task += NTHR) {
+        const int c = task % NCH, p = (task / NCH) % Q, g = task / (NCH * Q);
+        const T *s = s_all + p * PPW * CELLS + g * CELLS;
+        const int ci = c % 2, cj = (c / 2) % 2, ck = (c / 4) % 2;
+        T *a = coef + (long long)task * N;
+        if (IC::DIM == 2) {
+            const int o = 2 * ci + 2 * 4 * cj;
+            a[0] = s[0 + o];
+            a[1] = s[1 + o] - s[0 + o];
+            a[2] = s[4 + o] - s[0 + o];
+            a[3] = s[5 + o] - s[4 + o] - s[1 + o] + s[0 + o];
+        } else {
+            const int o = 2 * ci + 2 * 4 * cj + 2 * 4 * 4 * ck;
+            a[0] = s[0 + o];
+            a[1] = s[1 + o] - s[0 + o];
+            a[2] = s[4 + o] - s[0 + o];
+            a[3] = s[16 + o] - s[0 + o];
+            a[4] = s[5 + o] - s[1 + o] - s[4 + o] + s[0 + o];
+            a[5] = s[17 + o] - s[1 + o] - s[16 + o] + s[0 + o];
+            a[6] = s[20 + o] - s[4 + o] - s[16 + o] + s[0 + o];
+            a[7] = s[21 + o] + s[1 + o] + s[4 + o] + s[16 + o] - s[5 + o] - s[17 + o] - s[20 + o] - s[0 + o];
+        }
+    }
+}
+
+// Value at child c's cell t in the reference's operation order (Cu_Interpolate_Linear,
+// solver_lbm_kernels_comm.cu:13-138): s0 + a1 Ix + a2 Iy + a3 Iz + a4 Ix Iy + ... summed left to
+// right, each product formed left to right; stored into every row.
 template <typename T, typename S, int Q>
-__device__ __forceinline__ void interp_child(const T *sg, int stride, int c, int t, S *__restrict__ Fc, long long ncm,
+__device__ __forceinline__ void interp_child(const T *coef_g, int c, int t, S *__restrict__ Fc, long long ncm,
                                              long long ccell)
 {
     using VS = VelSet<Q>;
-    constexpr int DIM = VS::DIM;
+    using IC = InterpCoef<T, Q>;
+    constexpr int DIM = VS::DIM, NCH = IC::NCH, N = IC::N;
     const int I = t % 4, J = (t / 4) % 4, K = (DIM == 3) ? t / 16 : 0;
     const T Ix = (T)(-0.25) + (T)(0.5) * (T)I;
     const T Iy = (T)(-0.25) + (T)(0.5) * (T)J;
     const T Iz = (DIM == 3) ? (T)(-0.25) + (T)(0.5) * (T)K : (T)0;
-    const int ci = c % 2, cj = (c / 2) % 2, ck = (c / 4) % 2;
 #pragma unroll
     for (int p = 0; p < Q; p++) {
-        const T *s = sg + p * stride;
+        const T *a = coef_g + (p * NCH + c) * N;
         T val;
-        if (DIM == 2) {
-            const int o = 2 * ci + 2 * 4 * cj;
-            val = s[0 + o] + (s[1 + o] - s[0 + o]) * Ix + (s[4 + o] - s[0 + o]) * Iy +
-                  (s[5 + o] - s[4 + o] - s[1 + o] + s[0 + o]) * Ix * Iy;
-        } else {
-            const int o = 2 * ci + 2 * 4 * cj + 2 * 4 * 4 * ck;
-            val = s[0 + o] + (s[1 + o] - s[0 + o]) * Ix + (s[4 + o] - s[0 + o]) * Iy + (s[16 + o] - s[0 + o]) * Iz +
-                  (s[5 + o] - s[1 + o] - s[4 + o] + s[0 + o]) * Ix * Iy +
-                  (s[17 + o] - s[1 + o] - s[16 + o] + s[0 + o]) * Ix * Iz +
-                  (s[20 + o] - s[4 + o] - s[16 + o] + s[0 + o]) * Iy * Iz +
-                  (s[21 + o] + s[1 + o] + s[4 + o] + s[16 + o] - s[5 + o] - s[17 + o] - s[20 + o] - s[0 + o]) * Ix * Iy * Iz;
-        }
+        if (DIM == 2)
+            val = a[0] + a[1] * Ix + a[2] * Iy + a[3] * Ix * Iy;
+        else
+            val = a[0] + a[1] * Ix + a[2] * Iy + a[3] * Iz + a[4] * Ix * Iy + a[5] * Ix * Iz + a[6] * Iy * Iz +
+                  a[7] * Ix * Iy * Iz;
         Fc[(long long)p * ncm + ccell] = Store<T, S>::enc(val, (T)VS::w(p));
     }
 }
@@ -680,7 +715,9 @@ __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int
     constexpr int CELLS = VS::CELLS, NCH = VS::CHILDREN, HALF = NCH / 2;
     constexpr int GROUP = HALF * CELLS;  // lanes per parent: one per cell of half the children
     constexpr int PPW = IWG / GROUP;     // parents per workgroup
+    constexpr int NCOEF = InterpCoef<T, Q>::N;
     __shared__ T s_all[Q * PPW * CELLS];
+    __shared__ T s_coef[PPW * Q * NCH * NCOEF];
     const int g = threadIdx.x / GROUP, r = threadIdx.x % GROUP;
     const int c = r / CELLS, t = r % CELLS;
     const int bi = blockIdx.x * PPW + g;
@@ -715,10 +752,14 @@ __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int
         }
     }
     __syncthreads();
+    interp_coefficients<T, Q, PPW, IWG>(s_all, s_coef);
+    __syncthreads();
     if (!work) return;
 #pragma unroll
     for (int h = 0; h < 2; h++)
-        if (put[h]) interp_child<T, S, Q>(sg, PPW * CELLS, c + h * HALF, t, Fc, ncm, (long long)(fc + c + h * HALF) * CELLS + t);
+        if (put[h])
+            interp_child<T, S, Q>(s_coef + g * Q * NCH * NCOEF, c + h * HALF, t, Fc, ncm,
+                                  (long long)(fc + c + h * HALF) * CELLS + t);
 }
 
 // Sum over the NG lanes of each lane group (NG consecutive lanes, NG = 8 in 3D, 4 in 2D) of the
@@ -768,10 +809,16 @@ __global__ __launch_bounds__(IWG) void k_avg_interp(const int *__restrict__ ids,
     constexpr int CELLS = VS::CELLS, DIM = VS::DIM, NCH = VS::CHILDREN, HALF = NCH / 2;
     constexpr int GROUP = HALF * CELLS;  // lanes per parent
     constexpr int PPW = IWG / GROUP;     // parents per workgroup
-    __shared__ T s_all[Q * PPW * CELLS];  // rescaled parent populations for the interpolation
+    constexpr int NCOEF = InterpCoef<T, Q>::N;
+    // The averaging stage's group-sum rows and the interpolation stage's rescaled parent and
+    // coefficients share one region (a barrier separates the stages).
+    constexpr int RED = (IWG / 64) * Q * 65, INTERP = Q * PPW * CELLS + PPW * Q * NCH * NCOEF;
+    __shared__ T s_stage[RED > INTERP ? RED : INTERP];
     __shared__ T s_avg[Q * PPW * CELLS];  // averaged parent populations as stored
     __shared__ int s_done[PPW * CELLS];   // parent cell was averaged
-    __shared__ T s_red[(IWG / 64) * Q * 65];  // per-wavefront rows of the group sums
+    T *const s_red = s_stage;                        // per-wavefront rows of the group sums
+    T *const s_all = s_stage;                        // rescaled parent populations for the interpolation
+    T *const s_coef = s_stage + Q * PPW * CELLS;     // interpolation coefficients
     const int g = threadIdx.x / GROUP, r = threadIdx.x % GROUP;
     const int bi = blockIdx.x * PPW + g;
     int b = -1;
@@ -868,10 +915,14 @@ __global__ __launch_bounds__(IWG) void k_avg_interp(const int *__restrict__ ids,
         }
     }
     __syncthreads();
+    interp_coefficients<T, Q, PPW, IWG>(s_all, s_coef);
+    __syncthreads();
     if (!work) return;
 #pragma unroll
     for (int h = 0; h < 2; h++)
-        if (put[h]) interp_child<T, S, Q>(sg, PPW * CELLS, c + h * HALF, t, Fc, ncm, (long long)(fc + c + h * HALF) * CELLS + t);
+        if (put[h])
+            interp_child<T, S, Q>(s_coef + g * Q * NCH * NCOEF, c + h * HALF, t, Fc, ncm,
+                                  (long long)(fc + c + h * HALF) * CELLS + t);
 }
 
 // ---------------------------------------------------------------------------------------------
