@@ -139,6 +139,10 @@ struct Ctx {
     };
     static constexpr int NSLOT = 9, NRES = 16;
     bool sched_on = false;
+    // Coarse fused sub-steps held back to share the next finest sub-step's launch: (level, buffer
+    // read).  Only while a coarse step is being issued (defer_coarse).
+    bool defer_coarse = false;
+    std::vector<std::pair<int, int>> deferred;
     hipStream_t lstream[8] = {};
     bool slot_joined[NSLOT] = {};
     int seen[NSLOT][NSLOT] = {};  // latest event of slot t already waited for by slot s
@@ -171,6 +175,7 @@ int sched_done(Ctx *c, hipStream_t s, int slot, unsigned rmask, unsigned wmask);
 int launch_set_ic(Ctx *c, int L);
 int launch_refresh(Ctx *c, int L);
 int launch_collide_stream(Ctx *c, int L);
+int launch_flush_deferred(Ctx *c);
 int launch_interpolate(Ctx *c, int L, int var);
 int launch_average(Ctx *c, int L, int var);
 // average(L) in the advance's mode followed by interpolate(L, INTERFACE), fused for linear

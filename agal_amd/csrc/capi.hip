@@ -323,6 +323,17 @@ int advance_rec(Ctx *c, int L)
     return AGAL_OK;
 }
 
+// One coarse step from level L0 with the coarse levels' fused sub-steps folded into the finest
+// level's launches (the held-back launches are issued before returning).
+int advance_step(Ctx *c, int L0)
+{
+    c->defer_coarse = true;
+    int st = advance_rec(c, L0);
+    c->defer_coarse = false;
+    const int st2 = launch_flush_deferred(c);
+    return st ? st : st2;
+}
+
 unsigned cur_mask(const Ctx *c)
 {
     unsigned m = 0;
@@ -340,7 +351,7 @@ void set_cur_mask(Ctx *c, unsigned m)
 // (buffer parity per level) is advanced exactly as a direct run would.
 int step_once(Ctx *c, int L0)
 {
-    if (!c->use_graphs || c->timing.enabled || L0 != c->N_LEVEL_START) return advance_rec(c, L0);
+    if (!c->use_graphs || c->timing.enabled || L0 != c->N_LEVEL_START) return advance_step(c, L0);
     const unsigned before = cur_mask(c);
     for (auto &g : c->graphs)
         if (g.exec && g.epoch == c->mesh_epoch && g.cur_before == before) {
@@ -374,7 +385,7 @@ int step_once(Ctx *c, int L0)
     hipError_t e = hipEventRecord(c->sched_start, c->stream);
     if (e != hipSuccess) st = hip_fail(c, e, "hipEventRecord");
     c->sched_on = (st == AGAL_OK);
-    if (st == AGAL_OK) st = advance_rec(c, L0);
+    if (st == AGAL_OK) st = advance_step(c, L0);
     c->sched_on = false;
     for (int k = 1; k < Ctx::NSLOT; k++) {  // joined also after a failure so that the capture can end
         if (!c->slot_joined[k]) continue;

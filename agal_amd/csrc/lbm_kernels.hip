@@ -460,12 +460,12 @@ __device__ __forceinline__ void push_interior(const StepArgs<T, S> &a, const T *
 // selects on its own cell position -- no LDS traffic and no per-direction waits.  Interior blocks
 // take push_interior instead.
 template <typename T, typename S, int Q, bool MF>
-__device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, float *lds)
+__device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, int wave, float *lds)
 {
     using VS = VelSet<Q>;
     using ST = Store<T, S>;
     const int t = threadIdx.x & 63;
-    const int bi = __builtin_amdgcn_readfirstlane(xcd_local_wg(gridDim.x) * (WG / 64) + (int)(threadIdx.x >> 6));
+    const int bi = wave;  // one block per wavefront
     if (bi >= a.n) return;  // whole wavefronts only; the kernel has no barrier
     const int b = __builtin_amdgcn_readfirstlane(a.ids[bi]);
     // Cell index within a population row (the pool holds fewer than 2^32 cells, checked at
@@ -530,26 +530,48 @@ __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, float
 }
 
 template <typename T, typename S, int Q>
-__device__ __forceinline__ void collide_stream_2d(const StepArgs<T, S> &a);
+__device__ __forceinline__ void collide_stream_2d(const StepArgs<T, S> &a, int wave);
+
+// One launch of the fused step over the blocks of up to MAXLV levels: the levels' wavefront slots
+// are consecutive ranges [start[k], start[k+1]) of the launch (one 3D block or four 2D blocks per
+// slot), dealt to the XCDs in contiguous ranges.  Levels of one launch touch disjoint cells and
+// their own exception lists, so one launch does what separate launches of each level would.
+constexpr int MAXLV = 4;
+
+template <typename T, typename S>
+struct MultiStep {
+    int nlv;
+    int start[MAXLV + 1];
+    StepArgs<T, S> lv[MAXLV];
+};
 
 template <typename T, typename S, int Q, bool MF>
-__global__ __launch_bounds__(WG) void k_collide_stream(StepArgs<T, S> a)
+__global__ __launch_bounds__(WG) void k_collide_stream(MultiStep<T, S> m)
 {
     using VS = VelSet<Q>;
+    const int slot = __builtin_amdgcn_readfirstlane(xcd_local_wg(gridDim.x) * (WG / 64) + (int)(threadIdx.x >> 6));
+    if (slot >= m.start[m.nlv]) return;  // whole wavefronts only; the kernel has no barrier
+    int k = 0;
+#pragma unroll
+    for (int j = 1; j < MAXLV; j++)
+        if (j < m.nlv && slot >= m.start[j]) k = j;
+    k = __builtin_amdgcn_readfirstlane(k);
+    const StepArgs<T, S> &a = m.lv[k];
+    const int wave = slot - m.start[k];
     if constexpr (MF) {
         // One 64 x 36-float transpose region per wavefront for the matrix-core MRT.
         __shared__ float s_mrt[(WG / 64) * 64 * 36];
-        collide_stream_3d<T, S, Q, true>(a, s_mrt + (threadIdx.x >> 6) * 64 * 36);
+        collide_stream_3d<T, S, Q, true>(a, wave, s_mrt + (threadIdx.x >> 6) * 64 * 36);
     } else if constexpr (VS::DIM == 3) {
-        collide_stream_3d<T, S, Q, false>(a, nullptr);
+        collide_stream_3d<T, S, Q, false>(a, wave, nullptr);
     } else {
-        collide_stream_2d<T, S, Q>(a);
+        collide_stream_2d<T, S, Q>(a, wave);
     }
 }
 
 // 2D: four 4x4 blocks per wavefront; neighbour IDs are distributed with ds_bpermute.
 template <typename T, typename S, int Q>
-__device__ __forceinline__ void collide_stream_2d(const StepArgs<T, S> &a)
+__device__ __forceinline__ void collide_stream_2d(const StepArgs<T, S> &a, int wave)
 {
     using VS = VelSet<Q>;
     using ST = Store<T, S>;
@@ -557,7 +579,7 @@ __device__ __forceinline__ void collide_stream_2d(const StepArgs<T, S> &a)
     const int t = threadIdx.x % CELLS;
     const int lane = threadIdx.x % 64;
     const int gbase = lane - t;  // lane of cell 0 of this block within the wavefront
-    const int bi = xcd_local_wg(gridDim.x) * (WG / CELLS) + threadIdx.x / CELLS;
+    const int bi = wave * (64 / CELLS) + lane / CELLS;
     const bool active = bi < a.n;
     const int b = active ? a.ids[bi] : 0;
     // Lane-parallel load of the block's neighbour table; distributed with ds_bpermute (every
@@ -601,13 +623,28 @@ __device__ __forceinline__ void collide_stream_2d(const StepArgs<T, S> &a)
 }
 
 // Resolves guard exceptions: the initiator's pushed value returns to it (bounce-back) and the
-// partner keeps its own post-collision value.  One workgroup; clears the counter when done.
+// partner keeps its own post-collision value.  One workgroup per level of the fused launch, each on
+// its level's list; it clears the list's counter when done.
+template <typename T, typename S>
+struct FixupArgs {
+    long long ncm, exc_cap;
+    S *B[MAXLV];
+    const int *exc_idx[MAXLV];
+    const T *exc_val[MAXLV];
+    int *exc_count[MAXLV];
+};
+
 template <typename T, typename S, int Q>
-__global__ __launch_bounds__(WG) void k_fixup(S *__restrict__ B, long long ncm, const int *__restrict__ exc_idx,
-                                              const T *__restrict__ exc_val, int *exc_count, long long exc_cap)
+__global__ __launch_bounds__(WG) void k_fixup(FixupArgs<T, S> fa)
 {
     using VS = VelSet<Q>;
-    const long long n = min((long long)exc_count[0], exc_cap);
+    const int lv = blockIdx.x;
+    S *__restrict__ B = fa.B[lv];
+    const int *__restrict__ exc_idx = fa.exc_idx[lv];
+    const T *__restrict__ exc_val = fa.exc_val[lv];
+    int *exc_count = fa.exc_count[lv];
+    const long long ncm = fa.ncm;
+    const long long n = min((long long)exc_count[0], fa.exc_cap);
     for (long long k = threadIdx.x; k < n; k += WG) {
         const long long i = (unsigned)exc_idx[3 * k + 0], c = (unsigned)exc_idx[3 * k + 1];
         const int q = exc_idx[3 * k + 2];
@@ -1155,16 +1192,15 @@ static int refresh_t(Ctx *c, int L)
     return AGAL_OK;
 }
 
+// Fused-step arguments of level L reading population buffer rd (writing rd ^ 1).
 template <typename T, typename S, int Q>
-static int collide_stream_t(Ctx *c, int L)
+static StepArgs<T, S> step_args(Ctx *c, int L, int rd)
 {
-    int n = c->n_ids[L];
-    if (n <= 0) return AGAL_OK;
     StepArgs<T, S> a;
     a.ids = c->order_L(L);  // the level's blocks in traversal order
-    a.n = n;
-    a.A = Fbuf<S>(c, c->cur[L]);
-    a.B = Fbuf<S>(c, c->cur[L] ^ 1);
+    a.n = c->n_ids[L];
+    a.A = Fbuf<S>(c, rd);
+    a.B = Fbuf<S>(c, rd ^ 1);
     a.ncm = c->n_maxcells;
     a.nbr_slot = c->nbr_slot;
     a.part = c->part;
@@ -1190,43 +1226,116 @@ static int collide_stream_t(Ctx *c, int L)
     }
     a.U = (T)c->P.bc_velocity;
     a.bc_preset = c->P.bc_preset;
-    // Each level has its own exception list so that fused steps of different levels may run
-    // concurrently.
+    // Each level has its own exception list: levels share launches.
     a.exc_idx = c->exc_idx + (size_t)3 * c->exc_cap * L;
     a.exc_val = static_cast<T *>(c->exc_val) + (size_t)c->exc_cap * L;
     a.exc_count = c->exc_count + 2 * L;
     a.exc_cap = c->exc_cap;
-    // Reads buffer cur of level L, writes buffer cur^1.  The finest level's steps stay on the
-    // context stream; a coarser level's run beside the finer levels' sub-cycle.
-    const unsigned rl = res_bit(L, c->cur[L]), wl = res_bit(L, c->cur[L] ^ 1);
-    const int slot = L == c->MAX_LEVELS - 1 ? 0 : 1 + L;
-    hipStream_t s = sched_begin(c, slot, rl, wl);
-    const int grid = (nwg(n, VelSet<Q>::CELLS) + 7) / 8 * 8;  // a multiple of the XCD count
-    // The matrix-core MRT variant: FP32 compute, 3D sets of 19 / 27 directions, when enabled.
-    constexpr bool MF_OK = sizeof(T) == 4 && (Q == 19 || Q == 27);
-    const bool mf = MF_OK && a.mrt && c->mrt_mfma;
-    auto kern = mf ? (void (*)(StepArgs<T, S>))k_collide_stream<T, S, Q, MF_OK>
-                   : (void (*)(StepArgs<T, S>))k_collide_stream<T, S, Q, false>;
-    if (c->timing.enabled) {
-        // Events recorded by the dispatch itself: they bracket the kernel's execution.
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        AGAL_HIP(c, hipEventCreate(&e0));
-        AGAL_HIP(c, hipEventCreate(&e1));
-        hipExtLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, s, e0, e1, 0, a);
+    return a;
+}
+
+// Fused step of levels[j] reading buffers rd[j], up to MAXLV levels per launch, then one fixup
+// launch with a workgroup per level.  A coarse level alone goes to its level stream (the
+// recorded step's scheduler); launches that hold the finest level stay on the context stream.
+template <typename T, typename S, int Q>
+static int launch_fused(Ctx *c, const int *levels, const int *rd, int nl)
+{
+    constexpr int CELLS = VelSet<Q>::CELLS, BPW = 64 / CELLS;  // blocks per wavefront
+    for (int base = 0; base < nl; base += MAXLV) {
+        MultiStep<T, S> m;
+        FixupArgs<T, S> fa;
+        m.nlv = 0;
+        long long waves = 0;
+        double cells = 0;
+        unsigned rmask = 0, wmask = 0;
+        bool finest = false;
+        int first = -1;
+        for (int j = base; j < nl && j < base + MAXLV; j++) {
+            const int L = levels[j], n = c->n_ids[L];
+            rmask |= res_bit(L, rd[j]);
+            wmask |= res_bit(L, rd[j] ^ 1);
+            if (n <= 0) continue;
+            finest = finest || L == c->MAX_LEVELS - 1;
+            if (first < 0) first = L;
+            m.lv[m.nlv] = step_args<T, S, Q>(c, L, rd[j]);
+            m.start[m.nlv] = (int)waves;
+            fa.B[m.nlv] = m.lv[m.nlv].B;
+            fa.exc_idx[m.nlv] = m.lv[m.nlv].exc_idx;
+            fa.exc_val[m.nlv] = m.lv[m.nlv].exc_val;
+            fa.exc_count[m.nlv] = m.lv[m.nlv].exc_count;
+            waves += (n + BPW - 1) / BPW;
+            cells += (double)n * CELLS;
+            m.nlv++;
+        }
+        if (m.nlv == 0) continue;
+        for (int k = m.nlv; k <= MAXLV; k++) m.start[k] = (int)waves;
+        fa.ncm = c->n_maxcells;
+        fa.exc_cap = c->exc_cap;
+        const int slot = (finest || m.nlv > 1) ? 0 : 1 + first;
+        hipStream_t s = sched_begin(c, slot, rmask, wmask);
+        const int grid = (int)((waves + WG / 64 - 1) / (WG / 64) + 7) / 8 * 8;  // a multiple of the XCD count
+        // The matrix-core MRT variant: FP32 compute, 3D sets of 19 / 27 directions, when enabled.
+        constexpr bool MF_OK = sizeof(T) == 4 && (Q == 19 || Q == 27);
+        const bool mf = MF_OK && m.lv[0].mrt && c->mrt_mfma;
+        auto kern = mf ? (void (*)(MultiStep<T, S>))k_collide_stream<T, S, Q, MF_OK>
+                       : (void (*)(MultiStep<T, S>))k_collide_stream<T, S, Q, false>;
+        if (c->timing.enabled) {
+            // Events recorded by the dispatch itself: they bracket the kernel's execution.
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            AGAL_HIP(c, hipEventCreate(&e0));
+            AGAL_HIP(c, hipEventCreate(&e1));
+            hipExtLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, s, e0, e1, 0, m);
+            AGAL_HIP(c, hipGetLastError());
+            c->timing.ev_pairs.push_back(e0);
+            c->timing.ev_pairs.push_back(e1);
+            c->timing.fused_launches++;
+            c->timing.fused_cells += cells;
+        } else {
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, s, m);
+            AGAL_HIP(c, hipGetLastError());
+        }
+        hipLaunchKernelGGL((k_fixup<T, S, Q>), dim3(m.nlv), dim3(WG), 0, s, fa);
         AGAL_HIP(c, hipGetLastError());
-        c->timing.ev_pairs.push_back(e0);
-        c->timing.ev_pairs.push_back(e1);
-        c->timing.fused_launches++;
-        c->timing.fused_cells += (double)n * VelSet<Q>::CELLS;
-    } else {
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, s, a);
-        AGAL_HIP(c, hipGetLastError());
+        int st = sched_done(c, s, slot, rmask, wmask);
+        if (st) return st;
     }
-    hipLaunchKernelGGL((k_fixup<T, S, Q>), dim3(1), dim3(WG), 0, s, a.B, a.ncm, a.exc_idx, a.exc_val, a.exc_count,
-                       a.exc_cap);
-    AGAL_HIP(c, hipGetLastError());
+    return AGAL_OK;
+}
+
+// The fused step of level L.  Inside a recorded or direct coarse step (defer_coarse), a level
+// above the finest is held back and joins the launch of the next finest-level sub-step: between
+// the two the advance runs only kernels of finer levels (interpolation into level L+1's children,
+// finer sub-steps), which touch none of level L's cells, so the order of results is unchanged.
+template <typename T, typename S, int Q>
+static int collide_stream_t(Ctx *c, int L)
+{
+    if (c->defer_coarse && L < c->MAX_LEVELS - 1) {
+        c->deferred.push_back({L, c->cur[L]});
+        c->cur[L] ^= 1;
+        return AGAL_OK;
+    }
+    std::vector<int> lv{L}, rd{c->cur[L]};
     c->cur[L] ^= 1;
-    return sched_done(c, s, slot, rl, wl);
+    for (const auto &d : c->deferred) {
+        lv.push_back(d.first);
+        rd.push_back(d.second);
+    }
+    c->deferred.clear();
+    return launch_fused<T, S, Q>(c, lv.data(), rd.data(), (int)lv.size());
+}
+
+// Launches coarse sub-steps still held back (a step that ended without a finest sub-step).
+template <typename T, typename S, int Q>
+static int flush_deferred_t(Ctx *c)
+{
+    if (c->deferred.empty()) return AGAL_OK;
+    std::vector<int> lv, rd;
+    for (const auto &d : c->deferred) {
+        lv.push_back(d.first);
+        rd.push_back(d.second);
+    }
+    c->deferred.clear();
+    return launch_fused<T, S, Q>(c, lv.data(), rd.data(), (int)lv.size());
 }
 
 template <typename T, typename S, int Q>
@@ -1332,6 +1441,7 @@ static int avg_interp_t(Ctx *c, int L)
 int launch_set_ic(Ctx *c, int L) { AGAL_DISPATCH(c, set_ic_t, c, L); }
 int launch_refresh(Ctx *c, int L) { AGAL_DISPATCH(c, refresh_t, c, L); }
 int launch_collide_stream(Ctx *c, int L) { AGAL_DISPATCH(c, collide_stream_t, c, L); }
+int launch_flush_deferred(Ctx *c) { AGAL_DISPATCH(c, flush_deferred_t, c); }
 int launch_interpolate(Ctx *c, int L, int var) { AGAL_DISPATCH(c, interpolate_t, c, L, var); }
 int launch_average(Ctx *c, int L, int var) { AGAL_DISPATCH(c, average_t, c, L, var); }
 int launch_average_interpolate(Ctx *c, int L) { AGAL_DISPATCH(c, avg_interp_t, c, L); }
