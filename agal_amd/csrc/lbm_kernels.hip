@@ -21,6 +21,8 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "agal_common.hpp"
 #include "agal_ctx.hpp"
 
@@ -352,6 +354,28 @@ __device__ __forceinline__ S load_pop(const S *p)
         return *p;
 }
 
+template <typename S>
+__device__ __forceinline__ S load_pop(const __attribute__((address_space(1))) S *p)
+{
+    if constexpr (sizeof(S) >= 4)
+        return __builtin_nontemporal_load(p);
+    else if constexpr (std::is_same<S, bf16s>::value)
+        return bf16s{*reinterpret_cast<const __attribute__((address_space(1))) uint16_t *>(p)};
+    else
+        return *p;
+}
+
+// A global-memory pointer that is the same on every lane, forced into scalar registers: a lane's
+// element of it is then a scalar base plus a 32-bit offset, with no 64-bit vector address
+// arithmetic per access.
+template <typename S>
+__device__ __forceinline__ const __attribute__((address_space(1))) S *scalar_row(const S *p)
+{
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v), hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (const __attribute__((address_space(1))) S *)(((unsigned long long)hi << 32) | lo);
+}
+
 // One link of the fused step for the cell `cell`, direction q >= 1, whose neighbour block in that
 // direction is nbk (target cell tgt): boundary hook, swap guard, and the single store of f*_q
 // either into the neighbour (the link streams) or into the cell's own opposite slot (bounce-back).
@@ -471,13 +495,16 @@ __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, int w
     // Cell index within a population row (the pool holds fewer than 2^32 cells, checked at
     // creation), so rows are addressed as a row base plus a 32-bit index.
     const unsigned cell = (unsigned)b * 64u + (unsigned)t;
-    const int my_nb = t < 27 ? a.nbr_slot[(long long)b * 27 + t] : 0;
+    // Lane s < 27 holds neighbour slot s (lanes above repeat slot 26): a load on every lane keeps
+    // the block index out of divergent control flow, so it stays a scalar.
+    const int my_nb = a.nbr_slot[(long long)b * 27 + (t < 27 ? t : 26)];
     const int simple_u = __builtin_amdgcn_readfirstlane(a.simple[b]);
     const int part_u = __builtin_amdgcn_readfirstlane(a.part[b]);
     const int onb_u = __builtin_amdgcn_readfirstlane(a.onb[b]);
     T f[Q];
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = ST::dec(load_pop(&(a.A + (long long)q * a.ncm)[cell]), (T)VS::w(q));
+    for (int q = 0; q < Q; q++)  // row q of block b: a scalar base, lane t at offset t
+        f[q] = ST::dec(load_pop(scalar_row(a.A + (long long)q * a.ncm + (long long)b * 64) + t), (T)VS::w(q));
     const bool simple = simple_u != 0;
     int mask = CELL_INTERIOR;
     if (!simple) mask = a.cells_mask[cell];
