@@ -143,6 +143,9 @@ struct Ctx {
     // read).  Only while a coarse step is being issued (defer_coarse).
     bool defer_coarse = false;
     std::vector<std::pair<int, int>> deferred;
+    // Exception lists awaiting their fixup: (level, buffer the level's launch wrote).  Resolved
+    // before anything else touches the level, or by workgroups of a fused launch that does not.
+    std::vector<std::pair<int, int>> pending_fix;
     hipStream_t lstream[8] = {};
     bool slot_joined[NSLOT] = {};
     int seen[NSLOT][NSLOT] = {};  // latest event of slot t already waited for by slot s
@@ -176,6 +179,8 @@ int launch_set_ic(Ctx *c, int L);
 int launch_refresh(Ctx *c, int L);
 int launch_collide_stream(Ctx *c, int L);
 int launch_flush_deferred(Ctx *c);
+int launch_interp_collide(Ctx *c, int L);
+int launch_fix_pending(Ctx *c);
 int launch_interpolate(Ctx *c, int L, int var);
 int launch_average(Ctx *c, int L, int var);
 // average(L) in the advance's mode followed by interpolate(L, INTERFACE), fused for linear

@@ -295,25 +295,31 @@ int advance_rec(Ctx *c, int L)
     int st;
     const int ML = c->MAX_LEVELS, L0 = c->N_LEVEL_START;
     if (ML > 1 && ML != L0 + 1) {
+        // Interpolation of level L and its first fused sub-step share a launch inside a coarse
+        // step; the averages wait for the pending fixups of the levels they read.
         if (L == L0) {
             c->global_Liter = 0;
-            if ((st = launch_interpolate(c, L, AGAL_INTERP_INTERFACE))) return st;
-            if ((st = launch_collide_stream(c, L))) return st;
+            if ((st = launch_interp_collide(c, L))) return st;
             if ((st = advance_rec(c, L + 1))) return st;
+            if ((st = launch_fix_pending(c))) return st;
             if ((st = launch_average(c, L, average_advance(c)))) return st;
         } else {
-            if (L < ML - 1)
-                if ((st = launch_interpolate(c, L, AGAL_INTERP_INTERFACE))) return st;
-            if ((st = launch_collide_stream(c, L))) return st;
+            if (L < ML - 1) {
+                if ((st = launch_interp_collide(c, L))) return st;
+            } else if ((st = launch_collide_stream(c, L))) {
+                return st;
+            }
             if (L == ML - 1) c->global_Liter++;
             if (L < ML - 1) {
                 if ((st = advance_rec(c, L + 1))) return st;
+                if ((st = launch_fix_pending(c))) return st;
                 if ((st = launch_average_interpolate(c, L))) return st;
             }
             if ((st = launch_collide_stream(c, L))) return st;
             if (L == ML - 1) c->global_Liter++;
             if (L < ML - 1) {
                 if ((st = advance_rec(c, L + 1))) return st;
+                if ((st = launch_fix_pending(c))) return st;
                 if ((st = launch_average(c, L, average_advance(c)))) return st;
             }
         }

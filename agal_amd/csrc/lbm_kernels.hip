@@ -164,6 +164,13 @@ __device__ __forceinline__ int xcd_local_wg(int nwg_padded)
     return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
 }
 
+// The same for the first nwg_padded (a multiple of 8) workgroups of a larger grid.
+__device__ __forceinline__ int xcd_local_wg(int wg, int nwg_padded)
+{
+    const int per = nwg_padded / 8;
+    return (wg % 8) * per + wg / 8;
+}
+
 // Post-collision values f*_p of one cell.  BGK: Cu_Collide (solver_lbm_kernels.cu:275-282).  MRT:
 // f* = f - M^T (ck .* (M (f - feq))), the transforms summed in ascending row / column order with
 // zero entries and conserved rows left out (ck_k = rate_k / D_k).  Cells that do not collide
@@ -572,30 +579,6 @@ struct MultiStep {
     StepArgs<T, S> lv[MAXLV];
 };
 
-template <typename T, typename S, int Q, bool MF>
-__global__ __launch_bounds__(WG) void k_collide_stream(MultiStep<T, S> m)
-{
-    using VS = VelSet<Q>;
-    const int slot = __builtin_amdgcn_readfirstlane(xcd_local_wg(gridDim.x) * (WG / 64) + (int)(threadIdx.x >> 6));
-    if (slot >= m.start[m.nlv]) return;  // whole wavefronts only; the kernel has no barrier
-    int k = 0;
-#pragma unroll
-    for (int j = 1; j < MAXLV; j++)
-        if (j < m.nlv && slot >= m.start[j]) k = j;
-    k = __builtin_amdgcn_readfirstlane(k);
-    const StepArgs<T, S> &a = m.lv[k];
-    const int wave = slot - m.start[k];
-    if constexpr (MF) {
-        // One 64 x 36-float transpose region per wavefront for the matrix-core MRT.
-        __shared__ float s_mrt[(WG / 64) * 64 * 36];
-        collide_stream_3d<T, S, Q, true>(a, wave, s_mrt + (threadIdx.x >> 6) * 64 * 36);
-    } else if constexpr (VS::DIM == 3) {
-        collide_stream_3d<T, S, Q, false>(a, wave, nullptr);
-    } else {
-        collide_stream_2d<T, S, Q>(a, wave);
-    }
-}
-
 // 2D: four 4x4 blocks per wavefront; neighbour IDs are distributed with ds_bpermute.
 template <typename T, typename S, int Q>
 __device__ __forceinline__ void collide_stream_2d(const StepArgs<T, S> &a, int wave)
@@ -662,10 +645,9 @@ struct FixupArgs {
 };
 
 template <typename T, typename S, int Q>
-__global__ __launch_bounds__(WG) void k_fixup(FixupArgs<T, S> fa)
+__device__ __forceinline__ void fixup_level(const FixupArgs<T, S> &fa, int lv)
 {
     using VS = VelSet<Q>;
-    const int lv = blockIdx.x;
     S *__restrict__ B = fa.B[lv];
     const int *__restrict__ exc_idx = fa.exc_idx[lv];
     const T *__restrict__ exc_val = fa.exc_val[lv];
@@ -689,6 +671,12 @@ __global__ __launch_bounds__(WG) void k_fixup(FixupArgs<T, S> fa)
     }
     __syncthreads();
     if (threadIdx.x == 0) exc_count[0] = 0;
+}
+
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(WG) void k_fixup(FixupArgs<T, S> fa)
+{
+    fixup_level<T, S, Q>(fa, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -769,42 +757,58 @@ __device__ __forceinline__ void interp_child(const T *coef_g, int c, int t, S *_
     }
 }
 
+// Arguments of the linear interpolation of one level's parent list into the children.
+template <typename T, typename S>
+struct InterpArgs {
+    const int *ids;
+    int n;
+    const S *Fp;
+    S *Fc;
+    long long ncm;
+    const int *nbr_child, *blk_mask, *ref, *cells_mask;
+    T tau_ratio;
+};
+
+template <typename T, int Q>
+struct InterpLds {  // LDS of one interpolation workgroup
+    static constexpr int CELLS = VelSet<Q>::CELLS, NCH = VelSet<Q>::CHILDREN;
+    static constexpr int PPW = IWG / (NCH / 2 * CELLS);
+    static constexpr int ALL = Q * PPW * CELLS, COEF = PPW * Q * NCH * InterpCoef<T, Q>::N;
+};
+
+// Workgroup `wg` of the interpolation (IWG threads): its parents, their children's cells.
 template <typename T, typename S, int Q, bool ADDED>
-__global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int n, const S *__restrict__ Fp,
-                                                S *__restrict__ Fc, long long ncm, const int *__restrict__ nbr_child,
-                                                const int *__restrict__ blk_mask, const int *__restrict__ ref,
-                                                const int *__restrict__ cells_mask, T tau_ratio)
+__device__ __forceinline__ void interp_parents(const InterpArgs<T, S> &ia, int wg, T *s_all, T *s_coef)
 {
     using VS = VelSet<Q>;
     constexpr int CELLS = VS::CELLS, NCH = VS::CHILDREN, HALF = NCH / 2;
     constexpr int GROUP = HALF * CELLS;  // lanes per parent: one per cell of half the children
     constexpr int PPW = IWG / GROUP;     // parents per workgroup
     constexpr int NCOEF = InterpCoef<T, Q>::N;
-    __shared__ T s_all[Q * PPW * CELLS];
-    __shared__ T s_coef[PPW * Q * NCH * NCOEF];
+    const long long ncm = ia.ncm;
     const int g = threadIdx.x / GROUP, r = threadIdx.x % GROUP;
     const int c = r / CELLS, t = r % CELLS;
-    const int bi = blockIdx.x * PPW + g;
+    const int bi = wg * PPW + g;
     int b = -1;
-    if (bi < n) b = ids[bi];
+    if (bi < ia.n) b = ia.ids[bi];
     bool work = false;
     int fc = 0;
     if (b >= 0) {
-        fc = nbr_child[b];
-        work = fc >= 0 && (ADDED ? (ref[b] == REF_MARK_REFINE) : (blk_mask[b] == BLK_INTERFACE));
+        fc = ia.nbr_child[b];
+        work = fc >= 0 && (ADDED ? (ia.ref[b] == REF_MARK_REFINE) : (ia.blk_mask[b] == BLK_INTERFACE));
     }
     // The children's cell masks, fetched before the barrier.
     bool put[2] = {false, false};
     if (work)
 #pragma unroll
         for (int h = 0; h < 2; h++)
-            put[h] = ADDED || cells_mask[(long long)(fc + c + h * HALF) * CELLS + t] == CELL_GHOST;
+            put[h] = ADDED || ia.cells_mask[(long long)(fc + c + h * HALF) * CELLS + t] == CELL_GHOST;
     T *sg = s_all + g * CELLS;
     if (work && c == 0) {
         const long long cell = (long long)b * CELLS + t;
         T f[Q];
 #pragma unroll
-        for (int p = 0; p < Q; p++) f[p] = Store<T, S>::dec(Fp[(long long)p * ncm + cell], (T)VS::w(p));
+        for (int p = 0; p < Q; p++) f[p] = Store<T, S>::dec(ia.Fp[(long long)p * ncm + cell], (T)VS::w(p));
         T rho, u, v, w;
         moments<T, Q>(f, rho, u, v, w);
         const T udotu = u * u + v * v + w * w;
@@ -812,7 +816,7 @@ __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int
         for (int p = 0; p < Q; p++) {
             T cdotu = cdot<T, Q>(p, u, v, w);
             T fe = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
-            sg[p * PPW * CELLS + t] = fe + (f[p] - fe) * tau_ratio;
+            sg[p * PPW * CELLS + t] = fe + (f[p] - fe) * ia.tau_ratio;
         }
     }
     __syncthreads();
@@ -822,8 +826,17 @@ __global__ __launch_bounds__(IWG) void k_interp(const int *__restrict__ ids, int
 #pragma unroll
     for (int h = 0; h < 2; h++)
         if (put[h])
-            interp_child<T, S, Q>(s_coef + g * Q * NCH * NCOEF, c + h * HALF, t, Fc, ncm,
+            interp_child<T, S, Q>(s_coef + g * Q * NCH * NCOEF, c + h * HALF, t, ia.Fc, ncm,
                                   (long long)(fc + c + h * HALF) * CELLS + t);
+}
+
+template <typename T, typename S, int Q, bool ADDED>
+__global__ __launch_bounds__(IWG) void k_interp(InterpArgs<T, S> ia)
+{
+    using IL = InterpLds<T, Q>;
+    __shared__ T s_all[IL::ALL];
+    __shared__ T s_coef[IL::COEF];
+    interp_parents<T, S, Q, ADDED>(ia, blockIdx.x, s_all, s_coef);
 }
 
 // Sum over the NG lanes of each lane group (NG consecutive lanes, NG = 8 in 3D, 4 in 2D) of the
@@ -1175,6 +1188,60 @@ __global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int
 }
 
 // ---------------------------------------------------------------------------------------------
+// The fused step's launch.  Workgroups [0, step_wgs) run the fused step of the MultiStep levels;
+// the next interp_wgs workgroups interpolate one level's interface parents into their children
+// (k_interp, INTERFACE mode), and the last fix_wgs workgroups resolve the exception lists of
+// levels that an earlier launch left (k_fixup).  The three parts touch disjoint cells: a level's
+// interpolation reads the level's current buffer and writes the finer level's ghost cells, and
+// the launcher hands over only fixups of levels that none of the other parts touches.
+// ---------------------------------------------------------------------------------------------
+template <typename T, typename S>
+struct StepExtras {
+    int step_wgs, interp_wgs, fix_wgs;
+    InterpArgs<T, S> ia;
+    FixupArgs<T, S> fa;
+};
+
+static_assert(IWG == WG, "interpolation and fused-step workgroups share one launch");
+
+template <typename T, typename S, int Q, bool MF>
+__global__ __launch_bounds__(WG) void k_collide_stream(MultiStep<T, S> m, StepExtras<T, S> x)
+{
+    using VS = VelSet<Q>;
+    const int wg = blockIdx.x;
+    if (wg >= x.step_wgs) {
+        const int w2 = wg - x.step_wgs;
+        if (w2 < x.interp_wgs) {
+            using IL = InterpLds<T, Q>;
+            __shared__ T s_all[IL::ALL];
+            __shared__ T s_coef[IL::COEF];
+            interp_parents<T, S, Q, false>(x.ia, w2, s_all, s_coef);
+        } else {
+            fixup_level<T, S, Q>(x.fa, w2 - x.interp_wgs);
+        }
+        return;
+    }
+    const int slot = __builtin_amdgcn_readfirstlane(xcd_local_wg(wg, x.step_wgs) * (WG / 64) + (int)(threadIdx.x >> 6));
+    if (slot >= m.start[m.nlv]) return;  // whole wavefronts only; the fused step has no barrier
+    int k = 0;
+#pragma unroll
+    for (int j = 1; j < MAXLV; j++)
+        if (j < m.nlv && slot >= m.start[j]) k = j;
+    k = __builtin_amdgcn_readfirstlane(k);
+    const StepArgs<T, S> &a = m.lv[k];
+    const int wave = slot - m.start[k];
+    if constexpr (MF) {
+        // One 64 x 36-float transpose region per wavefront for the matrix-core MRT.
+        __shared__ float s_mrt[(WG / 64) * 64 * 36];
+        collide_stream_3d<T, S, Q, true>(a, wave, s_mrt + (threadIdx.x >> 6) * 64 * 36);
+    } else if constexpr (VS::DIM == 3) {
+        collide_stream_3d<T, S, Q, false>(a, wave, nullptr);
+    } else {
+        collide_stream_2d<T, S, Q>(a, wave);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------------------------
 template <typename T>
@@ -1261,16 +1328,86 @@ static StepArgs<T, S> step_args(Ctx *c, int L, int rd)
     return a;
 }
 
-// Fused step of levels[j] reading buffers rd[j], up to MAXLV levels per launch, then one fixup
-// launch with a workgroup per level.  A coarse level alone goes to its level stream (the
-// recorded step's scheduler); launches that hold the finest level stay on the context stream.
+template <typename T, typename S>
+static InterpArgs<T, S> interp_args(Ctx *c, int L, const int *ids, int n, T ratio)
+{
+    InterpArgs<T, S> ia;
+    ia.ids = ids;
+    ia.n = n;
+    ia.Fp = Fbuf<S>(c, c->cur[L]);
+    ia.Fc = Fbuf<S>(c, c->cur[L + 1]);
+    ia.ncm = c->n_maxcells;
+    ia.nbr_child = c->nbr_child;
+    ia.blk_mask = c->blk_mask;
+    ia.ref = c->ref;
+    ia.cells_mask = c->cells_mask;
+    ia.tau_ratio = ratio;
+    return ia;
+}
+
+// Exception lists waiting for their fixup: up to MAXLV of the pending levels in `mask` go into fa
+// (with the buffer that the launch which left the list wrote) and leave the pending set.
+template <typename T, typename S>
+static int take_fixups(Ctx *c, unsigned mask, FixupArgs<T, S> &fa, unsigned &wmask)
+{
+    int n = 0;
+    std::vector<std::pair<int, int>> keep;
+    for (const auto &pf : c->pending_fix) {
+        const int L = pf.first, buf = pf.second;
+        if (!((mask >> L) & 1) || n == MAXLV) {
+            keep.push_back(pf);
+            continue;
+        }
+        fa.B[n] = Fbuf<S>(c, buf);
+        fa.exc_idx[n] = c->exc_idx + (size_t)3 * c->exc_cap * L;
+        fa.exc_val[n] = static_cast<T *>(c->exc_val) + (size_t)c->exc_cap * L;
+        fa.exc_count[n] = c->exc_count + 2 * L;
+        wmask |= res_bit(L, buf);
+        n++;
+    }
+    c->pending_fix = keep;
+    fa.ncm = c->n_maxcells;
+    fa.exc_cap = c->exc_cap;
+    return n;
+}
+
+// Fixup launches for the pending levels in `mask`.
 template <typename T, typename S, int Q>
-static int launch_fused(Ctx *c, const int *levels, const int *rd, int nl)
+static int launch_fixups(Ctx *c, unsigned mask)
+{
+    for (;;) {
+        FixupArgs<T, S> fa;
+        unsigned wmask = 0;
+        const int n = take_fixups<T, S>(c, mask, fa, wmask);
+        if (n == 0) return AGAL_OK;
+        hipStream_t s = sched_begin(c, 0, 0, wmask);
+        hipLaunchKernelGGL((k_fixup<T, S, Q>), dim3(n), dim3(WG), 0, s, fa);
+        AGAL_HIP(c, hipGetLastError());
+        int st = sched_done(c, s, 0, 0, wmask);
+        if (st) return st;
+    }
+}
+
+// Fused step of levels[j] reading buffers rd[j], up to MAXLV levels per launch.  With `ia`, the
+// first launch also interpolates level interp_L's interface parents (reading the buffer those
+// sub-steps read).  Exception lists: inside a coarse step (defer_coarse) a launch leaves its
+// levels' lists pending; pending lists of levels the launch touches are resolved by a fixup
+// launch before it, the others by workgroups of the launch itself.  Outside a coarse step the
+// fixup follows at once.  A coarse level alone goes to its level stream (the recorded step's
+// scheduler); launches that hold the finest level stay on the context stream.
+template <typename T, typename S, int Q>
+static int launch_fused(Ctx *c, const int *levels, const int *rd, int nl, const InterpArgs<T, S> *ia = nullptr,
+                        int interp_L = -1)
 {
     constexpr int CELLS = VelSet<Q>::CELLS, BPW = 64 / CELLS;  // blocks per wavefront
-    for (int base = 0; base < nl; base += MAXLV) {
+    int st;
+    unsigned touch = 0;
+    for (int j = 0; j < nl; j++) touch |= 1u << levels[j];
+    if (ia) touch |= (1u << interp_L) | (1u << (interp_L + 1));
+    if ((st = launch_fixups<T, S, Q>(c, touch))) return st;
+    for (int base = 0; base < nl || (base == 0 && ia); base += MAXLV) {
         MultiStep<T, S> m;
-        FixupArgs<T, S> fa;
+        StepExtras<T, S> x;
         m.nlv = 0;
         long long waves = 0;
         double cells = 0;
@@ -1286,53 +1423,58 @@ static int launch_fused(Ctx *c, const int *levels, const int *rd, int nl)
             if (first < 0) first = L;
             m.lv[m.nlv] = step_args<T, S, Q>(c, L, rd[j]);
             m.start[m.nlv] = (int)waves;
-            fa.B[m.nlv] = m.lv[m.nlv].B;
-            fa.exc_idx[m.nlv] = m.lv[m.nlv].exc_idx;
-            fa.exc_val[m.nlv] = m.lv[m.nlv].exc_val;
-            fa.exc_count[m.nlv] = m.lv[m.nlv].exc_count;
             waves += (n + BPW - 1) / BPW;
             cells += (double)n * CELLS;
             m.nlv++;
         }
-        if (m.nlv == 0) continue;
         for (int k = m.nlv; k <= MAXLV; k++) m.start[k] = (int)waves;
-        fa.ncm = c->n_maxcells;
-        fa.exc_cap = c->exc_cap;
-        const int slot = (finest || m.nlv > 1) ? 0 : 1 + first;
+        x.step_wgs = m.nlv ? (int)((waves + WG / 64 - 1) / (WG / 64) + 7) / 8 * 8 : 0;  // a multiple of the XCD count
+        x.interp_wgs = 0;
+        x.fix_wgs = 0;
+        if (base == 0) {
+            if (ia && ia->n > 0) {
+                x.ia = *ia;
+                x.interp_wgs = interp_grid<Q>(ia->n);
+                rmask |= res_bit(interp_L, rd[0]);
+                wmask |= res_bit(interp_L + 1, c->cur[interp_L + 1]);
+            }
+            if (c->defer_coarse) x.fix_wgs = take_fixups<T, S>(c, ~touch, x.fa, wmask);
+        }
+        if (x.step_wgs + x.interp_wgs + x.fix_wgs == 0) continue;
+        const int slot = (finest || m.nlv != 1 || x.interp_wgs || x.fix_wgs) ? 0 : 1 + first;
         hipStream_t s = sched_begin(c, slot, rmask, wmask);
-        const int grid = (int)((waves + WG / 64 - 1) / (WG / 64) + 7) / 8 * 8;  // a multiple of the XCD count
+        const int grid = x.step_wgs + x.interp_wgs + x.fix_wgs;
         // The matrix-core MRT variant: FP32 compute, 3D sets of 19 / 27 directions, when enabled.
         constexpr bool MF_OK = sizeof(T) == 4 && (Q == 19 || Q == 27);
-        const bool mf = MF_OK && m.lv[0].mrt && c->mrt_mfma;
-        auto kern = mf ? (void (*)(MultiStep<T, S>))k_collide_stream<T, S, Q, MF_OK>
-                       : (void (*)(MultiStep<T, S>))k_collide_stream<T, S, Q, false>;
-        if (c->timing.enabled) {
+        const bool mf = MF_OK && c->P.collision == AGAL_MRT && c->mrt_mfma;
+        auto kern = mf ? (void (*)(MultiStep<T, S>, StepExtras<T, S>))k_collide_stream<T, S, Q, MF_OK>
+                       : (void (*)(MultiStep<T, S>, StepExtras<T, S>))k_collide_stream<T, S, Q, false>;
+        if (c->timing.enabled && m.nlv) {
             // Events recorded by the dispatch itself: they bracket the kernel's execution.
             hipEvent_t e0 = nullptr, e1 = nullptr;
             AGAL_HIP(c, hipEventCreate(&e0));
             AGAL_HIP(c, hipEventCreate(&e1));
-            hipExtLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, s, e0, e1, 0, m);
+            hipExtLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, s, e0, e1, 0, m, x);
             AGAL_HIP(c, hipGetLastError());
             c->timing.ev_pairs.push_back(e0);
             c->timing.ev_pairs.push_back(e1);
             c->timing.fused_launches++;
             c->timing.fused_cells += cells;
         } else {
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, s, m);
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, s, m, x);
             AGAL_HIP(c, hipGetLastError());
         }
-        hipLaunchKernelGGL((k_fixup<T, S, Q>), dim3(m.nlv), dim3(WG), 0, s, fa);
-        AGAL_HIP(c, hipGetLastError());
-        int st = sched_done(c, s, slot, rmask, wmask);
-        if (st) return st;
+        if ((st = sched_done(c, s, slot, rmask, wmask))) return st;
+        for (int j = base; j < nl && j < base + MAXLV; j++) c->pending_fix.push_back({levels[j], rd[j] ^ 1});
+        if (!c->defer_coarse && (st = launch_fixups<T, S, Q>(c, ~0u))) return st;
     }
     return AGAL_OK;
 }
 
-// The fused step of level L.  Inside a recorded or direct coarse step (defer_coarse), a level
-// above the finest is held back and joins the launch of the next finest-level sub-step: between
-// the two the advance runs only kernels of finer levels (interpolation into level L+1's children,
-// finer sub-steps), which touch none of level L's cells, so the order of results is unchanged.
+// The fused step of level L.  Inside a coarse step (defer_coarse), a level above the finest is
+// held back and joins the launch of the next finest-level sub-step: between the two the advance
+// runs only kernels of finer levels (interpolation into level L+1's children, finer sub-steps),
+// which touch none of level L's cells, so the order of results is unchanged.
 template <typename T, typename S, int Q>
 static int collide_stream_t(Ctx *c, int L)
 {
@@ -1351,18 +1493,51 @@ static int collide_stream_t(Ctx *c, int L)
     return launch_fused<T, S, Q>(c, lv.data(), rd.data(), (int)lv.size());
 }
 
-// Launches coarse sub-steps still held back (a step that ended without a finest sub-step).
+// Interpolation of level L (INTERFACE mode) followed by level L's fused sub-step: one launch
+// inside a coarse step with linear interpolation (the interpolation reads the buffer the sub-step
+// reads and writes level L+1's ghost cells, which the sub-step does not touch), held-back coarser
+// sub-steps included; two launches otherwise.
 template <typename T, typename S, int Q>
-static int flush_deferred_t(Ctx *c)
+static int interp_collide_t(Ctx *c, int L)
 {
-    if (c->deferred.empty()) return AGAL_OK;
-    std::vector<int> lv, rd;
+    int st;
+    if (!c->defer_coarse || c->P.interp_order == AGAL_INTERP_CUBIC || L >= c->MAX_LEVELS - 1) {
+        if ((st = launch_interpolate(c, L, AGAL_INTERP_INTERFACE))) return st;
+        return launch_collide_stream(c, L);
+    }
+    const T ratio = (T)c->tau_vec[L + 1] / (T)c->tau_vec[L];
+    const InterpArgs<T, S> ia = interp_args<T, S>(c, L, c->iface_L(L), c->n_iface[L], ratio);
+    std::vector<int> lv{L}, rd{c->cur[L]};
+    c->cur[L] ^= 1;
     for (const auto &d : c->deferred) {
         lv.push_back(d.first);
         rd.push_back(d.second);
     }
     c->deferred.clear();
-    return launch_fused<T, S, Q>(c, lv.data(), rd.data(), (int)lv.size());
+    return launch_fused<T, S, Q>(c, lv.data(), rd.data(), (int)lv.size(), &ia, L);
+}
+
+// Launches coarse sub-steps still held back, then resolves every pending exception list.
+template <typename T, typename S, int Q>
+static int flush_deferred_t(Ctx *c)
+{
+    int st;
+    if (!c->deferred.empty()) {
+        std::vector<int> lv, rd;
+        for (const auto &d : c->deferred) {
+            lv.push_back(d.first);
+            rd.push_back(d.second);
+        }
+        c->deferred.clear();
+        if ((st = launch_fused<T, S, Q>(c, lv.data(), rd.data(), (int)lv.size()))) return st;
+    }
+    return launch_fixups<T, S, Q>(c, ~0u);
+}
+
+template <typename T, typename S, int Q>
+static int fix_pending_t(Ctx *c)
+{
+    return launch_fixups<T, S, Q>(c, ~0u);
 }
 
 template <typename T, typename S, int Q>
@@ -1392,13 +1567,11 @@ static int interpolate_t(Ctx *c, int L, int var)
                                Fbuf<S>(c, c->cur[L]), Fbuf<S>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child,
                                c->blk_mask, c->ref, c->cells_mask, ratio);
     } else if (var == AGAL_INTERP_ADDED)
-        hipLaunchKernelGGL((k_interp<T, S, Q, true>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
-                           Fbuf<S>(c, c->cur[L]), Fbuf<S>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
-                           c->ref, c->cells_mask, ratio);
+        hipLaunchKernelGGL((k_interp<T, S, Q, true>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s,
+                           interp_args<T, S>(c, L, ids, n, ratio));
     else
-        hipLaunchKernelGGL((k_interp<T, S, Q, false>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s, ids, n,
-                           Fbuf<S>(c, c->cur[L]), Fbuf<S>(c, c->cur[L + 1]), c->n_maxcells, c->nbr_child, c->blk_mask,
-                           c->ref, c->cells_mask, ratio);
+        hipLaunchKernelGGL((k_interp<T, S, Q, false>), dim3(interp_grid<Q>(n)), dim3(IWG), 0, s,
+                           interp_args<T, S>(c, L, ids, n, ratio));
     AGAL_HIP(c, hipGetLastError());
     return sched_done(c, s, 0, rl, wl);
 }
@@ -1469,6 +1642,8 @@ int launch_set_ic(Ctx *c, int L) { AGAL_DISPATCH(c, set_ic_t, c, L); }
 int launch_refresh(Ctx *c, int L) { AGAL_DISPATCH(c, refresh_t, c, L); }
 int launch_collide_stream(Ctx *c, int L) { AGAL_DISPATCH(c, collide_stream_t, c, L); }
 int launch_flush_deferred(Ctx *c) { AGAL_DISPATCH(c, flush_deferred_t, c); }
+int launch_interp_collide(Ctx *c, int L) { AGAL_DISPATCH(c, interp_collide_t, c, L); }
+int launch_fix_pending(Ctx *c) { AGAL_DISPATCH(c, fix_pending_t, c); }
 int launch_interpolate(Ctx *c, int L, int var) { AGAL_DISPATCH(c, interpolate_t, c, L, var); }
 int launch_average(Ctx *c, int L, int var) { AGAL_DISPATCH(c, average_t, c, L, var); }
 int launch_average_interpolate(Ctx *c, int L) { AGAL_DISPATCH(c, avg_interp_t, c, L); }
