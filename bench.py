@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--no-learned-study", action="store_true",
                     help="skip the learned refinement predictor on the cavity workload")
     ap.add_argument("--roofline-steps", type=int, default=20)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel directly instead of replaying recorded step graphs (same kernels "
+                         "and results; for runs under a kernel tracer)")
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (torch) to calibrate HBM byte counters")
@@ -86,6 +89,8 @@ def main():
                             bc_velocity=U, P_REFINE=100, N_REFINE_START=-2.0, N_REFINE_INC=1.0,
                             N_REFINE_MAX=1.0, ic_rho=1.0, S_CRITERION=2)
     m = LbmMesh(p, args.capacity)
+    if args.no_graphs:
+        m.set_option(capi.OPT_GRAPHS, 0)
     m.S_SetIC(0)
     m.M_FreezeRefinedCells(0)
     log(f"cavity: {args.warmup} warm-up coarse steps with rebuilds")
@@ -192,6 +197,7 @@ def main():
             "blocks_per_level": n_ids_end,
             "cell_updates_per_step": upd_actual,
             "parallelism": "single GPU",
+            "step_launch": "direct launches" if args.no_graphs else "one recorded HIP graph per coarse step",
         },
         "mlups_reference_formula": round(mlups_ref, 1),
         "mlups_fine_equivalent": round(mlups_fine, 1),
