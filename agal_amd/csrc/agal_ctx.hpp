@@ -122,8 +122,9 @@ struct Ctx {
         hipGraphExec_t exec = nullptr;
         unsigned cur_before = 0, cur_after = 0;
         unsigned epoch = 0;
+        int nsteps = 1;  // coarse steps recorded in the graph
     };
-    StepGraph graphs[4];
+    StepGraph graphs[8];
     unsigned mesh_epoch = 1;
     bool use_graphs = true;
     bool mrt_mfma = false;  // MRT moment transform on matrix cores (FP32 compute, D3Q19 / D3Q27)

@@ -355,12 +355,19 @@ void set_cur_mask(Ctx *c, unsigned m)
 // One coarse step from level L0.  With graphs enabled the launch sequence of advance_rec is
 // recorded once per (mesh epoch, starting buffer parity) and replayed; the host-side bookkeeping
 // (buffer parity per level) is advanced exactly as a direct run would.
-int step_once(Ctx *c, int L0)
+// `k` consecutive coarse steps (no rebuild between them) as one recorded graph: one graph launch,
+// one wait between graph launches, per k steps.
+int steps_once(Ctx *c, int L0, int k)
 {
-    if (!c->use_graphs || c->timing.enabled || L0 != c->N_LEVEL_START) return advance_step(c, L0);
+    int st;
+    if (!c->use_graphs || c->timing.enabled || L0 != c->N_LEVEL_START) {
+        for (int j = 0; j < k; j++)
+            if ((st = advance_step(c, L0))) return st;
+        return AGAL_OK;
+    }
     const unsigned before = cur_mask(c);
     for (auto &g : c->graphs)
-        if (g.exec && g.epoch == c->mesh_epoch && g.cur_before == before) {
+        if (g.exec && g.epoch == c->mesh_epoch && g.cur_before == before && g.nsteps == k) {
             AGAL_HIP(c, hipGraphLaunch(g.exec, c->stream));
             set_cur_mask(c, g.cur_after);
             c->global_Liter = 1 << (c->MAX_LEVELS - 1 - L0);
@@ -387,11 +394,11 @@ int step_once(Ctx *c, int L0)
         c->slot_joined[k] = false;
         for (int t = 0; t < Ctx::NSLOT; t++) c->seen[k][t] = -1;
     }
-    int st = AGAL_OK;
+    st = AGAL_OK;
     hipError_t e = hipEventRecord(c->sched_start, c->stream);
     if (e != hipSuccess) st = hip_fail(c, e, "hipEventRecord");
     c->sched_on = (st == AGAL_OK);
-    if (st == AGAL_OK) st = advance_step(c, L0);
+    for (int j = 0; j < k && st == AGAL_OK; j++) st = advance_step(c, L0);
     c->sched_on = false;
     for (int k = 1; k < Ctx::NSLOT; k++) {  // joined also after a failure so that the capture can end
         if (!c->slot_joined[k]) continue;
@@ -418,9 +425,12 @@ int step_once(Ctx *c, int L0)
     slot->cur_before = before;
     slot->cur_after = after;
     slot->epoch = c->mesh_epoch;
+    slot->nsteps = k;
     AGAL_HIP(c, hipGraphLaunch(slot->exec, c->stream));
     return AGAL_OK;
 }
+
+int step_once(Ctx *c, int L0) { return steps_once(c, L0, 1); }
 
 int check_exceptions(Ctx *c);
 
@@ -1185,10 +1195,20 @@ int agal_advance_loop(agal_ctx *ctx, int i0, int n, double *out_step_ms, double 
     const int iter_mult = 1 << c->N_LEVEL_START;
     const int ML = c->MAX_LEVELS;
     AGAL_HIP(c, hipEventRecord(e0, c->stream));
+    const bool adaptive = ML > 1 && ML != c->N_LEVEL_START + 1;
+    auto rebuild_after = [&](int i) { return adaptive && (i + 1) % c->P.P_REFINE == 0 && i > 0; };
     for (int i = i0; i < i0 + n; i++) {
-        for (int j = 0; j < iter_mult; j++)
-            if ((st = step_once(c, c->N_LEVEL_START))) return st;
-        if (ML > 1 && ML != c->N_LEVEL_START + 1 && (i + 1) % c->P.P_REFINE == 0 && i > 0) {
+        if (iter_mult == 1) {
+            // Up to four coarse steps with no rebuild between them run as one recorded graph.
+            int k = 1;
+            while (k < 4 && i + k < i0 + n && !rebuild_after(i + k - 1)) k++;
+            if ((st = steps_once(c, c->N_LEVEL_START, k))) return st;
+            i += k - 1;
+        } else {
+            for (int j = 0; j < iter_mult; j++)
+                if ((st = step_once(c, c->N_LEVEL_START))) return st;
+        }
+        if (rebuild_after(i)) {
             float ms = 0;
             AGAL_HIP(c, hipEventRecord(e1, c->stream));
             AGAL_HIP(c, hipEventSynchronize(e1));

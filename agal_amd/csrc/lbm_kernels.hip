@@ -1188,16 +1188,19 @@ __global__ __launch_bounds__(WG) void k_average(const int *__restrict__ ids, int
 }
 
 // ---------------------------------------------------------------------------------------------
-// The fused step's launch.  Workgroups [0, step_wgs) run the fused step of the MultiStep levels;
-// the next interp_wgs workgroups interpolate one level's interface parents into their children
-// (k_interp, INTERFACE mode), and the last fix_wgs workgroups resolve the exception lists of
-// levels that an earlier launch left (k_fixup).  The three parts touch disjoint cells: a level's
-// interpolation reads the level's current buffer and writes the finer level's ghost cells, and
-// the launcher hands over only fixups of levels that none of the other parts touches.
+// The fused step's launch.  The first interp_wgs workgroups interpolate one level's interface
+// parents into their children (k_interp, INTERFACE mode), the next fix_wgs resolve the exception
+// lists of levels that an earlier launch left (k_fixup), and from `lead` on (those two counts
+// rounded up to a multiple of 8, so that the XCD mapping below still deals contiguous ranges)
+// step_wgs workgroups run the fused step of the MultiStep levels.  The few, latency-bound
+// interpolation workgroups go first so that they run beside the fused step instead of after it.
+// The three parts touch disjoint cells: a level's interpolation reads the level's current buffer
+// and writes the finer level's ghost cells, and the launcher hands over only fixups of levels that
+// none of the other parts touches.
 // ---------------------------------------------------------------------------------------------
 template <typename T, typename S>
 struct StepExtras {
-    int step_wgs, interp_wgs, fix_wgs;
+    int step_wgs, interp_wgs, fix_wgs, lead;
     InterpArgs<T, S> ia;
     FixupArgs<T, S> fa;
 };
@@ -1208,19 +1211,19 @@ template <typename T, typename S, int Q, bool MF>
 __global__ __launch_bounds__(WG) void k_collide_stream(MultiStep<T, S> m, StepExtras<T, S> x)
 {
     using VS = VelSet<Q>;
-    const int wg = blockIdx.x;
-    if (wg >= x.step_wgs) {
-        const int w2 = wg - x.step_wgs;
+    if ((int)blockIdx.x < x.lead) {
+        const int w2 = blockIdx.x;
         if (w2 < x.interp_wgs) {
             using IL = InterpLds<T, Q>;
             __shared__ T s_all[IL::ALL];
             __shared__ T s_coef[IL::COEF];
             interp_parents<T, S, Q, false>(x.ia, w2, s_all, s_coef);
-        } else {
+        } else if (w2 < x.interp_wgs + x.fix_wgs) {
             fixup_level<T, S, Q>(x.fa, w2 - x.interp_wgs);
         }
         return;
     }
+    const int wg = (int)blockIdx.x - x.lead;
     const int slot = __builtin_amdgcn_readfirstlane(xcd_local_wg(wg, x.step_wgs) * (WG / 64) + (int)(threadIdx.x >> 6));
     if (slot >= m.start[m.nlv]) return;  // whole wavefronts only; the fused step has no barrier
     int k = 0;
@@ -1438,12 +1441,13 @@ static int launch_fused(Ctx *c, const int *levels, const int *rd, int nl, const 
                 rmask |= res_bit(interp_L, rd[0]);
                 wmask |= res_bit(interp_L + 1, c->cur[interp_L + 1]);
             }
-            if (c->defer_coarse) x.fix_wgs = take_fixups<T, S>(c, ~touch, x.fa, wmask);
+            if (c->defer_coarse && !c->timing.enabled) x.fix_wgs = take_fixups<T, S>(c, ~touch, x.fa, wmask);
         }
+        x.lead = (x.interp_wgs + x.fix_wgs + 7) / 8 * 8;
         if (x.step_wgs + x.interp_wgs + x.fix_wgs == 0) continue;
         const int slot = (finest || m.nlv != 1 || x.interp_wgs || x.fix_wgs) ? 0 : 1 + first;
         hipStream_t s = sched_begin(c, slot, rmask, wmask);
-        const int grid = x.step_wgs + x.interp_wgs + x.fix_wgs;
+        const int grid = x.lead + x.step_wgs;
         // The matrix-core MRT variant: FP32 compute, 3D sets of 19 / 27 directions, when enabled.
         constexpr bool MF_OK = sizeof(T) == 4 && (Q == 19 || Q == 27);
         const bool mf = MF_OK && c->P.collision == AGAL_MRT && c->mrt_mfma;
@@ -1501,7 +1505,10 @@ template <typename T, typename S, int Q>
 static int interp_collide_t(Ctx *c, int L)
 {
     int st;
-    if (!c->defer_coarse || c->P.interp_order == AGAL_INTERP_CUBIC || L >= c->MAX_LEVELS - 1) {
+    // The timed pass (timing.enabled) keeps the interpolation apart, so that the events around a
+    // fused launch time the fused step alone.
+    if (!c->defer_coarse || c->timing.enabled || c->P.interp_order == AGAL_INTERP_CUBIC || L >= c->MAX_LEVELS - 1) {
+        if ((st = launch_fixups<T, S, Q>(c, (1u << L) | (2u << L)))) return st;  // levels L, L+1
         if ((st = launch_interpolate(c, L, AGAL_INTERP_INTERFACE))) return st;
         return launch_collide_stream(c, L);
     }
