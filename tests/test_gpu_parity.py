@@ -82,6 +82,22 @@ def test_solution_adaptive_three_levels(vs):
     assert g.sizes.n_ids[2] > 0
 
 
+@pytest.mark.parametrize("vs,N", [(capi.D3Q19, 8), (capi.D2Q9, 16)])
+def test_solution_adaptive_four_levels(vs, N):
+    """Four levels: deeper sub-cycling, where held-back second sub-steps of two coarse levels share
+    launches with finer levels' first sub-steps and their interpolations."""
+    p = ldc_params(vs=vs, N=N, Re=40.0, levels=4, P_REFINE=10, N_REFINE_START=-3.0)
+    g, o = pair(p, 16384)
+    both((g, o), lambda m: m.S_SetIC(0))
+    both((g, o), lambda m: m.M_FreezeRefinedCells(0))
+    for k in range(4):
+        rg = g.advance_loop(10 * k, 10)
+        ro = o.advance_loop(10 * k, 10)
+        assert rg[0] == ro[0]
+        assert_same(g, o)
+    assert g.sizes.n_ids[3] > 0
+
+
 @pytest.mark.parametrize("vs,N,prec", [(capi.D3Q19, 16, capi.FP32), (capi.D3Q19, 16, capi.FP64),
                                        (capi.D2Q9, 32, capi.FP32), (capi.D3Q27, 16, capi.FP16)])
 def test_cubic_interpolation_adaptive(vs, N, prec):
