@@ -653,7 +653,9 @@ __device__ __forceinline__ void fixup_level(const FixupArgs<T, S> &fa, int lv)
     const T *__restrict__ exc_val = fa.exc_val[lv];
     int *exc_count = fa.exc_count[lv];
     const long long ncm = fa.ncm;
-    const long long n = min((long long)exc_count[0], fa.exc_cap);
+    const int cnt = exc_count[0];
+    if (cnt == 0) return;  // the usual case: nothing to resolve, nothing to reset (whole workgroup)
+    const long long n = min((long long)cnt, fa.exc_cap);
     for (long long k = threadIdx.x; k < n; k += WG) {
         const long long i = (unsigned)exc_idx[3 * k + 0], c = (unsigned)exc_idx[3 * k + 1];
         const int q = exc_idx[3 * k + 2];
