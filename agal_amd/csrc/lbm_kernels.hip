@@ -486,6 +486,64 @@ __device__ __forceinline__ void push_interior(const StepArgs<T, S> &a, const T *
     }
 }
 
+// Stream of a simple block that is not interior (block flag simple == 1): the block and its
+// existing neighbours participate with no solid cell, but some neighbour slots are missing (a
+// domain face, or a coarser region).  Same lookup and target arithmetic as push_interior; a link
+// whose neighbour slot is missing bounces back into the cell's own opposite slot, after the face
+// hook when the block lies on the domain boundary (onb) and the slot is a face -- push_link's
+// rules for non-inert cells of simple blocks.
+template <typename T, typename S, int Q>
+__device__ __forceinline__ void push_simple(const StepArgs<T, S> &a, const T *fs, int my_nb, int t, unsigned cell,
+                                            bool onb, T u, T v, T w)
+{
+    using VS = VelSet<Q>;
+    using ST = Store<T, S>;
+    const int I = t & 3, J = (t >> 2) & 3, K = t >> 4;
+    const int xp = I == 3 ? 4 : 0, xm = I == 0 ? -4 : 0;
+    const int yp = J == 3 ? 12 : 0, ym = J == 0 ? -12 : 0;
+    const int zp = K == 3 ? 36 : 0, zm = K == 0 ? -36 : 0;
+    const unsigned tl = (unsigned)t & 0x15u, th = (unsigned)t & 0x2Au;
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+        const int cx = VS::cx(q), cy = VS::cy(q), cz = VS::cz(q);
+        int sa = 13 * 4;
+        if (cx > 0) sa += xp;
+        if (cx < 0) sa += xm;
+        if (cy > 0) sa += yp;
+        if (cy < 0) sa += ym;
+        if (cz > 0) sa += zp;
+        if (cz < 0) sa += zm;
+        const int nbk = __builtin_amdgcn_ds_bpermute(sa, my_nb);
+        const unsigned d = (unsigned)((cx & 3) | ((cy & 3) << 2) | ((cz & 3) << 4));
+        const unsigned tgt = ((unsigned)nbk << 6) + ((tl + (d & 0x15u)) ^ th ^ (d & 0x2Au));
+        T fq = fs[q];
+        if (onb && nbk < 0 && nbk != N_SKIPID)
+            fq = bc_hook<T>(a.bc_preset, a.U, nbk, fq, u, v, w, (T)VS::w(q), (T)VS::cx(q), (T)VS::cy(q), (T)VS::cz(q));
+        const S e = ST::enc(fq, (T)VS::w(q));
+        bool push = nbk >= 0;
+        if (!VS::half(q) && push) {
+            // Partner of the link initiated by the cell x + c_q: the guard on the stored value.
+            const T fv = ST::dec(e, (T)VS::w(q));
+            if (!(fv > (T)0)) {
+                int k = atomicAdd(&a.exc_count[0], 1);
+                if (k < a.exc_cap) {
+                    a.exc_idx[3 * (long long)k + 0] = (int)tgt;
+                    a.exc_idx[3 * (long long)k + 1] = (int)cell;
+                    a.exc_idx[3 * (long long)k + 2] = q;
+                    a.exc_val[k] = fv;
+                } else {
+                    a.exc_count[1] = 1;
+                }
+                continue;  // no store: the fixup places both values
+            }
+        }
+        if (push)
+            (a.B + (long long)q * a.ncm)[tgt] = e;
+        else
+            (a.B + (long long)VS::pb(q) * a.ncm)[cell] = e;
+    }
+}
+
 // 3D: one wavefront per block, so the block, its flags and its 27 neighbour IDs are wave-uniform
 // (scalar registers); each lane picks its link's neighbour among at most eight of them with
 // selects on its own cell position -- no LDS traffic and no per-direction waits.  Interior blocks
@@ -541,6 +599,10 @@ __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, int w
     a.B[cell] = ST::enc(fs[0], (T)VS::w(0));
     if (simple_u == 2) {
         push_interior<T, S, Q>(a, fs, my_nb, t, cell);
+        return;
+    }
+    if (simple_u == 1) {
+        push_simple<T, S, Q>(a, fs, my_nb, t, cell, onb_u == 1, u, v, w);
         return;
     }
     int nbs[27];
