@@ -100,7 +100,7 @@ struct Ctx {
     long long exc_cap = 0;
     int *exc_idx = nullptr;   // 3 ints per entry
     void *exc_val = nullptr;  // one value per entry
-    int *exc_count = nullptr; // per level: [2L] count, [2L+1] overflow flag
+    int *exc_count = nullptr; // per level (and the extra slot MAX_LEVELS): [2L] count, [2L+1] overflow flag
 
     // Triangle geometry: n_geo_tri triangles (9 floats each) binned on a G x G grid over the (y, z)
     // bounding box; bin (iy, iz) lists geo_bin_tri[geo_bin_start[iy + G iz] ...].
@@ -147,6 +147,8 @@ struct Ctx {
     // Exception lists awaiting their fixup: (level, buffer the level's launch wrote).  Resolved
     // before anything else touches the level, or by workgroups of a fused launch that does not.
     std::vector<std::pair<int, int>> pending_fix;
+    // 1 while the first, 2 while the second sub-step of a finest-level pair is issued, else 0.
+    int fine_pair = 0;
     hipStream_t lstream[8] = {};
     bool slot_joined[NSLOT] = {};
     int seen[NSLOT][NSLOT] = {};  // latest event of slot t already waited for by slot s

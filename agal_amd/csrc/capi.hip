@@ -306,8 +306,13 @@ int advance_rec(Ctx *c, int L)
         } else {
             if (L < ML - 1) {
                 if ((st = launch_interp_collide(c, L))) return st;
-            } else if ((st = launch_collide_stream(c, L))) {
-                return st;
+            } else {
+                // The finest level's two sub-steps run back to back: the first leaves its
+                // exception list for the second to apply as it loads (fine_pair 1, then 2).
+                c->fine_pair = 1;
+                st = launch_collide_stream(c, L);
+                c->fine_pair = 0;
+                if (st) return st;
             }
             if (L == ML - 1) c->global_Liter++;
             if (L < ML - 1) {
@@ -315,7 +320,10 @@ int advance_rec(Ctx *c, int L)
                 if ((st = launch_fix_pending(c))) return st;
                 if ((st = launch_average_interpolate(c, L))) return st;
             }
-            if ((st = launch_collide_stream(c, L))) return st;
+            if (L == ML - 1) c->fine_pair = 2;
+            st = launch_collide_stream(c, L);
+            c->fine_pair = 0;
+            if (st) return st;
             if (L == ML - 1) c->global_Liter++;
             if (L < ML - 1) {
                 if ((st = advance_rec(c, L + 1))) return st;
@@ -634,10 +642,10 @@ int refine_near_wall_impl(Ctx *c)
 
 int check_exceptions(Ctx *c)
 {
-    int v[16] = {};
-    AGAL_HIP(c, hipMemcpyAsync(v, c->exc_count, sizeof(int) * 2 * c->MAX_LEVELS, hipMemcpyDeviceToHost, c->stream));
+    std::vector<int> v(2 * (c->MAX_LEVELS + 1), 0);
+    AGAL_HIP(c, hipMemcpyAsync(v.data(), c->exc_count, sizeof(int) * v.size(), hipMemcpyDeviceToHost, c->stream));
     AGAL_HIP(c, hipStreamSynchronize(c->stream));
-    for (int L = 0; L < c->MAX_LEVELS; L++)
+    for (int L = 0; L <= c->MAX_LEVELS; L++)
         if (v[2 * L + 1]) {
             c->last_error = "stream-guard exception list of level " + std::to_string(L) + " overflowed";
             return AGAL_ERR_OVERFLOW;
@@ -824,9 +832,11 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     ALLOC(c->scan_b, sizeof(int) * nmax);
     ALLOC(c->partials, sizeof(int) * ((nmax + 2047) / 2048 + 64));
     ALLOC(c->dscal, sizeof(int) * 64);
-    ALLOC(c->exc_idx, sizeof(int) * 3 * c->exc_cap * c->MAX_LEVELS);
-    ALLOC(c->exc_val, cs * c->exc_cap * c->MAX_LEVELS);
-    ALLOC(c->exc_count, sizeof(int) * 2 * c->MAX_LEVELS);
+    // One exception list per level, and one more (slot MAX_LEVELS) that the first sub-step of each
+    // finest-level pair writes for the second to apply as it loads.
+    ALLOC(c->exc_idx, sizeof(int) * 3 * c->exc_cap * (c->MAX_LEVELS + 1));
+    ALLOC(c->exc_val, cs * c->exc_cap * (c->MAX_LEVELS + 1));
+    ALLOC(c->exc_count, sizeof(int) * 2 * (c->MAX_LEVELS + 1));
 #undef ALLOC
     e = hipHostMalloc((void **)&c->hscal, sizeof(int) * 64, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -840,7 +850,7 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     if ((st = fill(c->cells_mask, ncm, CELL_INTERIOR)) || (st = fill(c->blk_mask, nmax, BLK_REGULAR)) ||
         (st = fill(c->nbr, c->QMAX * nmax, N_SKIPID)) || (st = fill(c->nbr_child, c->QMAX * nmax, N_SKIPID)) ||
         (st = fill(c->ref, nmax, REF_INACTIVE)) || (st = fill(c->level, nmax, 0)) || (st = fill(c->onb, nmax, 0)) ||
-        (st = fill((int *)c->map_stamp, nmax, 0)) || (st = fill(c->exc_count, 2 * c->MAX_LEVELS, 0)) || (st = fill(c->part, nmax, 1)) ||
+        (st = fill((int *)c->map_stamp, nmax, 0)) || (st = fill(c->exc_count, 2 * (c->MAX_LEVELS + 1), 0)) || (st = fill(c->part, nmax, 1)) ||
         (st = fill(c->simple, nmax, 0)) || (st = fill((int *)c->X, c->DIM * nmax, 0))) {
         free_ctx(c);
         return st;

@@ -153,7 +153,39 @@ struct StepArgs {
     T *exc_val;
     int *exc_count;
     long long exc_cap;
+    // Finest-level pairs (3D): vmode 1 -- the exception list (exc_*) is the extra list that the
+    // next sub-step applies as it loads, and each block with an entry gets PEND_FLAG in flags;
+    // vmode 2 -- apply that list (va_*) to the loaded values of flagged blocks; 0 -- neither.
+    int vmode;
+    int *flags;
+    const int *va_idx;
+    const T *va_val;
+    const int *va_count;
 };
+
+// Block flag bit: the block has entries in the finest level's extra exception list (bits 0-1 of
+// the same word hold the simple / interior kind).
+constexpr int PEND_FLAG = 4;
+
+// Appends a guard exception (target cell, partner cell, direction, partner's stored value) to the
+// launch's list; in vmode 1 also flags the two blocks for the next sub-step.
+template <typename T, typename S>
+__device__ __forceinline__ void record_exception(const StepArgs<T, S> &a, long long tgt, long long cell, int q, T fv)
+{
+    int k = atomicAdd(&a.exc_count[0], 1);
+    if (k < a.exc_cap) {
+        a.exc_idx[3 * (long long)k + 0] = (int)tgt;  // cell indices < 2^31
+        a.exc_idx[3 * (long long)k + 1] = (int)cell;
+        a.exc_idx[3 * (long long)k + 2] = q;
+        a.exc_val[k] = fv;
+    } else {
+        a.exc_count[1] = 1;
+    }
+    if (a.vmode == 1) {
+        atomicOr(&a.flags[tgt >> 6], PEND_FLAG);
+        atomicOr(&a.flags[cell >> 6], PEND_FLAG);
+    }
+}
 
 // Workgroup -> position in the level's block list.  Workgroups are dealt round-robin to the
 // eight XCDs (each with its own L2); giving every XCD one contiguous range of the sorted ID list
@@ -416,15 +448,7 @@ __device__ __forceinline__ void push_link(const StepArgs<T, S> &a, int q, T fq, 
         }
     }
     if (exc) {
-        int k = atomicAdd(&a.exc_count[0], 1);
-        if (k < a.exc_cap) {
-            a.exc_idx[3 * (long long)k + 0] = (int)tgt;  // cell indices < 2^32, read back as unsigned
-            a.exc_idx[3 * (long long)k + 1] = (int)cell;
-            a.exc_idx[3 * (long long)k + 2] = q;
-            a.exc_val[k] = fq;
-        } else {
-            a.exc_count[1] = 1;
-        }
+        record_exception(a, tgt, cell, q, fq);
         return;
     }
     const long long at = push ? (long long)q * a.ncm + tgt : (long long)VS::pb(q) * a.ncm + cell;
@@ -471,15 +495,7 @@ __device__ __forceinline__ void push_interior(const StepArgs<T, S> &a, const T *
             const T fq = ST::dec(e, (T)VS::w(q));
             if (!(fq > (T)0)) {
                 push = false;
-                int k = atomicAdd(&a.exc_count[0], 1);
-                if (k < a.exc_cap) {
-                    a.exc_idx[3 * (long long)k + 0] = (int)tgt;
-                    a.exc_idx[3 * (long long)k + 1] = (int)cell;
-                    a.exc_idx[3 * (long long)k + 2] = q;
-                    a.exc_val[k] = fq;
-                } else {
-                    a.exc_count[1] = 1;
-                }
+                record_exception(a, (long long)tgt, (long long)cell, q, fq);
             }
         }
         if (push) (a.B + (long long)q * a.ncm)[tgt] = e;
@@ -525,15 +541,7 @@ __device__ __forceinline__ void push_simple(const StepArgs<T, S> &a, const T *fs
             // Partner of the link initiated by the cell x + c_q: the guard on the stored value.
             const T fv = ST::dec(e, (T)VS::w(q));
             if (!(fv > (T)0)) {
-                int k = atomicAdd(&a.exc_count[0], 1);
-                if (k < a.exc_cap) {
-                    a.exc_idx[3 * (long long)k + 0] = (int)tgt;
-                    a.exc_idx[3 * (long long)k + 1] = (int)cell;
-                    a.exc_idx[3 * (long long)k + 2] = q;
-                    a.exc_val[k] = fv;
-                } else {
-                    a.exc_count[1] = 1;
-                }
+                record_exception(a, (long long)tgt, (long long)cell, q, fv);
                 continue;  // no store: the fixup places both values
             }
         }
@@ -541,6 +549,43 @@ __device__ __forceinline__ void push_simple(const StepArgs<T, S> &a, const T *fs
             (a.B + (long long)q * a.ncm)[tgt] = e;
         else
             (a.B + (long long)VS::pb(q) * a.ncm)[cell] = e;
+    }
+}
+
+// The extra exception list of the previous finest sub-step, applied to this block's loaded values:
+// for an entry (initiator cell i, partner cell c, direction q) the fixup would move the value at
+// (pb(q), c) -- the initiator's push -- to (q, i) and store the partner's value at (pb(q), c).
+// The previous sub-step's output is this sub-step's input, read by no one else in between, so
+// the same values are taken here as the loads' results instead.  Whole-wavefront loop over the
+// list; only flagged blocks run it.
+template <typename T, typename S, int Q>
+__device__ __forceinline__ void apply_pending(const StepArgs<T, S> &a, T *f, int b, int t)
+{
+    using VS = VelSet<Q>;
+    using ST = Store<T, S>;
+    const long long n = min((long long)a.va_count[0], a.exc_cap);
+    for (long long k = 0; k < n; k++) {
+        const long long i = (unsigned)a.va_idx[3 * k + 0], c = (unsigned)a.va_idx[3 * k + 1];
+        const int q = a.va_idx[3 * k + 2];
+        const bool at_i = (i >> 6) == b && (int)(i & 63) == t;
+        const bool at_c = (c >> 6) == b && (int)(c & 63) == t;
+        if (!(at_i || at_c)) continue;
+        int qb = 0;
+        T wq = 0;
+#pragma unroll
+        for (int p = 0; p < Q; p++)
+            if (p == q) {
+                qb = VS::pb(p);
+                wq = (T)VS::w(p);
+            }
+        // Through storage, as the fixup's store and this sub-step's load would take them.
+        const T vi = ST::dec(a.A[(long long)qb * a.ncm + c], wq);
+        const T vc = ST::dec(ST::enc(a.va_val[k], wq), wq);
+#pragma unroll
+        for (int p = 1; p < Q; p++) {
+            if (at_i && p == q) f[p] = vi;
+            if (at_c && p == qb) f[p] = vc;
+        }
     }
 }
 
@@ -563,13 +608,18 @@ __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, int w
     // Lane s < 27 holds neighbour slot s (lanes above repeat slot 26): a load on every lane keeps
     // the block index out of divergent control flow, so it stays a scalar.
     const int my_nb = a.nbr_slot[(long long)b * 27 + (t < 27 ? t : 26)];
-    const int simple_u = __builtin_amdgcn_readfirstlane(a.simple[b]);
+    const int flags_u = __builtin_amdgcn_readfirstlane(a.simple[b]);
+    const int simple_u = flags_u & 3;
     const int part_u = __builtin_amdgcn_readfirstlane(a.part[b]);
     const int onb_u = __builtin_amdgcn_readfirstlane(a.onb[b]);
     T f[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++)  // row q of block b: a scalar base, lane t at offset t
         f[q] = ST::dec(load_pop(scalar_row(a.A + (long long)q * a.ncm + (long long)b * 64) + t), (T)VS::w(q));
+    if (a.vmode == 2 && (flags_u & PEND_FLAG)) {
+        apply_pending<T, S, Q>(a, f, b, t);
+        if (t == 0) a.flags[b] = simple_u;  // this sub-step sets no flags: no other writer
+    }
     const bool simple = simple_u != 0;
     int mask = CELL_INTERIOR;
     if (!simple) mask = a.cells_mask[cell];
@@ -664,7 +714,7 @@ __device__ __forceinline__ void collide_stream_2d(const StepArgs<T, S> &a, int w
     int mask = CELL_INTERIOR;
     T f[Q];
     if (active) {
-        simple = a.simple[b] != 0;
+        simple = (a.simple[b] & 3) != 0;
         const bool part = a.part[b] != 0;
         if (!simple) mask = a.cells_mask[cell];
         inert = !part || mask == CELL_SOLID;
@@ -704,6 +754,7 @@ struct FixupArgs {
     const int *exc_idx[MAXLV];
     const T *exc_val[MAXLV];
     int *exc_count[MAXLV];
+    int *va_count[MAXLV];  // the finest level: its extra list's counter, cleared here, else null
 };
 
 template <typename T, typename S, int Q>
@@ -715,6 +766,9 @@ __device__ __forceinline__ void fixup_level(const FixupArgs<T, S> &fa, int lv)
     const T *__restrict__ exc_val = fa.exc_val[lv];
     int *exc_count = fa.exc_count[lv];
     const long long ncm = fa.ncm;
+    // The extra list was applied by the sub-step that this list comes from; it starts empty for the
+    // next pair.
+    if (fa.va_count[lv] && threadIdx.x == 0 && fa.va_count[lv][0] != 0) fa.va_count[lv][0] = 0;
     const int cnt = exc_count[0];
     if (cnt == 0) return;  // the usual case: nothing to resolve, nothing to reset (whole workgroup)
     const long long n = min((long long)cnt, fa.exc_cap);
@@ -1355,7 +1409,7 @@ static int refresh_t(Ctx *c, int L)
 
 // Fused-step arguments of level L reading population buffer rd (writing rd ^ 1).
 template <typename T, typename S, int Q>
-static StepArgs<T, S> step_args(Ctx *c, int L, int rd)
+static StepArgs<T, S> step_args(Ctx *c, int L, int rd, int vmode = 0)
 {
     StepArgs<T, S> a;
     a.ids = c->order_L(L);  // the level's blocks in traversal order
@@ -1392,7 +1446,26 @@ static StepArgs<T, S> step_args(Ctx *c, int L, int rd)
     a.exc_val = static_cast<T *>(c->exc_val) + (size_t)c->exc_cap * L;
     a.exc_count = c->exc_count + 2 * L;
     a.exc_cap = c->exc_cap;
+    const int XL = c->MAX_LEVELS;  // the extra list's slot
+    a.vmode = vmode;
+    a.flags = c->simple;
+    a.va_idx = c->exc_idx + (size_t)3 * c->exc_cap * XL;
+    a.va_val = static_cast<T *>(c->exc_val) + (size_t)c->exc_cap * XL;
+    a.va_count = c->exc_count + 2 * XL;
+    if (vmode == 1) {
+        a.exc_idx = c->exc_idx + (size_t)3 * c->exc_cap * XL;
+        a.exc_val = static_cast<T *>(c->exc_val) + (size_t)c->exc_cap * XL;
+        a.exc_count = c->exc_count + 2 * XL;
+    }
     return a;
+}
+
+// The extra-list mode of level L in the fused launch being issued: only for the finest level of a
+// 3D set inside a coarse step.
+template <int Q>
+static int pair_mode(const Ctx *c, int L)
+{
+    return (VelSet<Q>::DIM == 3 && c->defer_coarse && L == c->MAX_LEVELS - 1) ? c->fine_pair : 0;
 }
 
 template <typename T, typename S>
@@ -1429,6 +1502,7 @@ static int take_fixups(Ctx *c, unsigned mask, FixupArgs<T, S> &fa, unsigned &wma
         fa.exc_idx[n] = c->exc_idx + (size_t)3 * c->exc_cap * L;
         fa.exc_val[n] = static_cast<T *>(c->exc_val) + (size_t)c->exc_cap * L;
         fa.exc_count[n] = c->exc_count + 2 * L;
+        fa.va_count[n] = L == c->MAX_LEVELS - 1 ? c->exc_count + 2 * c->MAX_LEVELS : nullptr;
         wmask |= res_bit(L, buf);
         n++;
     }
@@ -1488,7 +1562,7 @@ static int launch_fused(Ctx *c, const int *levels, const int *rd, int nl, const 
             if (n <= 0) continue;
             finest = finest || L == c->MAX_LEVELS - 1;
             if (first < 0) first = L;
-            m.lv[m.nlv] = step_args<T, S, Q>(c, L, rd[j]);
+            m.lv[m.nlv] = step_args<T, S, Q>(c, L, rd[j], pair_mode<Q>(c, L));
             m.start[m.nlv] = (int)waves;
             waves += (n + BPW - 1) / BPW;
             cells += (double)n * CELLS;
@@ -1533,7 +1607,9 @@ static int launch_fused(Ctx *c, const int *levels, const int *rd, int nl, const 
             AGAL_HIP(c, hipGetLastError());
         }
         if ((st = sched_done(c, s, slot, rmask, wmask))) return st;
-        for (int j = base; j < nl && j < base + MAXLV; j++) c->pending_fix.push_back({levels[j], rd[j] ^ 1});
+        // A list left for the next finest sub-step (pair mode 1) is not pending a fixup.
+        for (int j = base; j < nl && j < base + MAXLV; j++)
+            if (pair_mode<Q>(c, levels[j]) != 1) c->pending_fix.push_back({levels[j], rd[j] ^ 1});
         if (!c->defer_coarse && (st = launch_fixups<T, S, Q>(c, ~0u))) return st;
     }
     return AGAL_OK;
