@@ -415,6 +415,23 @@ __device__ __forceinline__ const __attribute__((address_space(1))) S *scalar_row
     return (const __attribute__((address_space(1))) S *)(((unsigned long long)hi << 32) | lo);
 }
 
+// Store of the fused step into a row (the same pointer on every lane) at element idx.  A 16-bit
+// row holds fewer than 2^32 bytes (the pool has fewer than 2^31 cells, checked at creation), so the
+// address is a scalar base plus a 32-bit byte offset, with no 64-bit vector address arithmetic per
+// store; wider rows keep 64-bit addresses.
+template <typename S>
+__device__ __forceinline__ void store_pop(S *row, unsigned idx, S v)
+{
+    if constexpr (sizeof(S) == 2) {
+        const unsigned long long r = reinterpret_cast<unsigned long long>(row);
+        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)r), hi = __builtin_amdgcn_readfirstlane((unsigned)(r >> 32));
+        auto *base = (__attribute__((address_space(1))) char *)(((unsigned long long)hi << 32) | lo);
+        *(__attribute__((address_space(1))) uint16_t *)(base + idx * 2u) = __builtin_bit_cast(uint16_t, v);
+    } else {
+        row[idx] = v;
+    }
+}
+
 // One link of the fused step for the cell `cell`, direction q >= 1, whose neighbour block in that
 // direction is nbk (target cell tgt): boundary hook, swap guard, and the single store of f*_q
 // either into the neighbour (the link streams) or into the cell's own opposite slot (bounce-back).
@@ -498,7 +515,7 @@ __device__ __forceinline__ void push_interior(const StepArgs<T, S> &a, const T *
                 record_exception(a, (long long)tgt, (long long)cell, q, fq);
             }
         }
-        if (push) (a.B + (long long)q * a.ncm)[tgt] = e;
+        if (push) store_pop(a.B + (long long)q * a.ncm, tgt, e);
     }
 }
 
@@ -546,9 +563,9 @@ __device__ __forceinline__ void push_simple(const StepArgs<T, S> &a, const T *fs
             }
         }
         if (push)
-            (a.B + (long long)q * a.ncm)[tgt] = e;
+            store_pop(a.B + (long long)q * a.ncm, tgt, e);
         else
-            (a.B + (long long)VS::pb(q) * a.ncm)[cell] = e;
+            store_pop(a.B + (long long)VS::pb(q) * a.ncm, cell, e);
     }
 }
 
@@ -646,7 +663,7 @@ __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, int w
         collide_cell<T, S, Q>(a, f, fs, rho, u, v, w, inert);
     }
 
-    a.B[cell] = ST::enc(fs[0], (T)VS::w(0));
+    store_pop(a.B, cell, ST::enc(fs[0], (T)VS::w(0)));
     if (simple_u == 2) {
         push_interior<T, S, Q>(a, fs, my_nb, t, cell);
         return;
