@@ -344,6 +344,7 @@ template <typename T, typename S>
 struct Store {  // S == T: the pool holds f
     __device__ static T dec(S s, T) { return s; }
     __device__ static S enc(T f, T) { return f; }
+    __device__ static bool positive(S s, T) { return s > (T)0; }  // dec(s, w) > 0
     static constexpr bool quantized = false;
 };
 
@@ -351,6 +352,10 @@ template <>
 struct Store<float, _Float16> {
     __device__ static float dec(_Float16 s, float w) { return (float)s + w; }
     __device__ static _Float16 enc(float f, float w) { return (_Float16)(f - w); }
+    // dec(s, w) > 0 without the add: a rounded float sum has the sign of the exact sum (a nonzero
+    // exact sum of these two is a multiple of 2^-133, far from underflow), and that sign is the
+    // exact comparison below.
+    __device__ static bool positive(_Float16 s, float w) { return (float)s > -w; }
     static constexpr bool quantized = true;
 };
 
@@ -358,6 +363,7 @@ template <>
 struct Store<float, bf16s> {
     __device__ static float dec(bf16s s, float w) { return __builtin_bit_cast(float, (uint32_t)s.bits << 16) + w; }
     __device__ static bf16s enc(float f, float w) { return bf16s{bf16_bits_rne(f - w)}; }
+    __device__ static bool positive(bf16s s, float w) { return __builtin_bit_cast(float, (uint32_t)s.bits << 16) > -w; }
     static constexpr bool quantized = true;
 };
 

@@ -509,10 +509,9 @@ __device__ __forceinline__ void push_interior(const StepArgs<T, S> &a, const T *
         bool push = true;
         if (!VS::half(q)) {
             // Partner of the link initiated by the cell x + c_q: the guard on the stored value.
-            const T fq = ST::dec(e, (T)VS::w(q));
-            if (!(fq > (T)0)) {
+            if (!ST::positive(e, (T)VS::w(q))) {
                 push = false;
-                record_exception(a, (long long)tgt, (long long)cell, q, fq);
+                record_exception(a, (long long)tgt, (long long)cell, q, ST::dec(e, (T)VS::w(q)));
             }
         }
         if (push) store_pop(a.B + (long long)q * a.ncm, tgt, e);
@@ -556,9 +555,8 @@ __device__ __forceinline__ void push_simple(const StepArgs<T, S> &a, const T *fs
         bool push = nbk >= 0;
         if (!VS::half(q) && push) {
             // Partner of the link initiated by the cell x + c_q: the guard on the stored value.
-            const T fv = ST::dec(e, (T)VS::w(q));
-            if (!(fv > (T)0)) {
-                record_exception(a, (long long)tgt, (long long)cell, q, fv);
+            if (!ST::positive(e, (T)VS::w(q))) {
+                record_exception(a, (long long)tgt, (long long)cell, q, ST::dec(e, (T)VS::w(q)));
                 continue;  // no store: the fixup places both values
             }
         }

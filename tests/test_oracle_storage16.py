@@ -106,3 +106,20 @@ def test_16bit_pool_tracks_fp32(prec, tol):
         dq = f[q, :n] - wq
         assert np.array_equal(O.from16(1 if prec == capi.FP16 else 2, O.to16(1 if prec == capi.FP16 else 2, dq)) + wq,
                               f[q, :n])
+
+
+def test_guard_sign_without_the_add():
+    """The fused step tests a stored 16-bit deviation s against the guard dec(s, w) > 0 as
+    float(s) > -w, with no add.  Every binary16 and bfloat16 pattern against every lattice weight
+    (D2Q9, D3Q19, D3Q27) in float32 arithmetic: the two tests agree, NaNs and infinities
+    included."""
+    bits = np.arange(1 << 16, dtype=np.uint32)
+    halves = bits.astype(np.uint16).view(np.float16).astype(np.float32)
+    bfloats = (bits << 16).view(np.float32)
+    weights = np.array([4 / 9, 1 / 9, 1 / 36, 1 / 3, 1 / 18, 8 / 27, 2 / 27, 1 / 54, 1 / 216], dtype=np.float32)
+    with np.errstate(invalid="ignore", over="ignore"):
+        for vals in (halves, bfloats):
+            for w in weights:
+                with_add = (vals + w) > np.float32(0)
+                without = vals > -w
+                assert np.array_equal(with_add, without), float(w)
