@@ -571,35 +571,50 @@ __device__ __forceinline__ void push_simple(const StepArgs<T, S> &a, const T *fs
 // for an entry (initiator cell i, partner cell c, direction q) the fixup would move the value at
 // (pb(q), c) -- the initiator's push -- to (q, i) and store the partner's value at (pb(q), c).
 // The previous sub-step's output is this sub-step's input, read by no one else in between, so
-// the same values are taken here as the loads' results instead.  Whole-wavefront loop over the
-// list; only flagged blocks run it.
+// the same values are taken here as the loads' results instead.  No two entries share a slot (an
+// entry's q is the partner's, non-initiating direction and pb(q) an initiating one), so the order
+// of application is free.  The wavefront scans the list 64 entries at a time, one per lane, and
+// applies the hits on block b one by one; only flagged blocks scan.
 template <typename T, typename S, int Q>
 __device__ __forceinline__ void apply_pending(const StepArgs<T, S> &a, T *f, int b, int t)
 {
     using VS = VelSet<Q>;
     using ST = Store<T, S>;
-    const long long n = min((long long)a.va_count[0], a.exc_cap);
-    for (long long k = 0; k < n; k++) {
-        const long long i = (unsigned)a.va_idx[3 * k + 0], c = (unsigned)a.va_idx[3 * k + 1];
-        const int q = a.va_idx[3 * k + 2];
-        const bool at_i = (i >> 6) == b && (int)(i & 63) == t;
-        const bool at_c = (c >> 6) == b && (int)(c & 63) == t;
-        if (!(at_i || at_c)) continue;
-        int qb = 0;
-        T wq = 0;
+    const int n = (int)min((long long)a.va_count[0], a.exc_cap);
+    const unsigned my = ((unsigned)b << 6) | (unsigned)t;
+    for (int k0 = 0; k0 < n; k0 += 64) {
+        const int k = k0 + t;
+        unsigned i = ~0u, c = ~0u;
+        int q = 0;
+        if (k < n) {
+            i = (unsigned)a.va_idx[3 * (long long)k + 0];
+            c = (unsigned)a.va_idx[3 * (long long)k + 1];
+            q = a.va_idx[3 * (long long)k + 2];
+        }
+        unsigned long long hits = __ballot((i >> 6) == (unsigned)b || (c >> 6) == (unsigned)b);
+        while (hits) {
+            const int l = __builtin_ctzll(hits);
+            hits &= hits - 1;
+            const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)i, l);
+            const unsigned hc = (unsigned)__builtin_amdgcn_readlane((int)c, l);
+            const int hq = __builtin_amdgcn_readlane(q, l);
+            int qb = 0;
+            T wq = 0;
 #pragma unroll
-        for (int p = 0; p < Q; p++)
-            if (p == q) {
-                qb = VS::pb(p);
-                wq = (T)VS::w(p);
+            for (int p = 0; p < Q; p++)
+                if (p == hq) {
+                    qb = VS::pb(p);
+                    wq = (T)VS::w(p);
+                }
+            // Through storage, as the fixup's store and this sub-step's load would take them.
+            const T vi = ST::dec(a.A[(long long)qb * a.ncm + hc], wq);
+            const T vc = ST::dec(ST::enc(a.va_val[k0 + l], wq), wq);
+            const bool at_i = hi == my, at_c = hc == my;
+#pragma unroll
+            for (int p = 1; p < Q; p++) {
+                if (at_i && p == hq) f[p] = vi;
+                if (at_c && p == qb) f[p] = vc;
             }
-        // Through storage, as the fixup's store and this sub-step's load would take them.
-        const T vi = ST::dec(a.A[(long long)qb * a.ncm + c], wq);
-        const T vc = ST::dec(ST::enc(a.va_val[k], wq), wq);
-#pragma unroll
-        for (int p = 1; p < Q; p++) {
-            if (at_i && p == q) f[p] = vi;
-            if (at_c && p == qb) f[p] = vc;
         }
     }
 }
