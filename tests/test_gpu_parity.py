@@ -206,8 +206,7 @@ def test_stream_guard_exceptions():
 def test_stream_guard_exceptions_across_levels(prec):
     """Guard exceptions on several levels at once: the coarse levels' fused sub-steps share the
     finest level's launches and the fixup serves each level's own list, while the first of each
-    pair of finest sub-steps leaves its list for the second to apply as it loads (hundreds of
-    entries here, so the second scans the list in several 64-entry chunks).  A developed
+    pair of finest sub-steps leaves its list for the second to apply as it loads.  A developed
     three-level mesh gets perturbed populations (many non-positive post-collision values on every
     level), then coarse steps as recorded graphs and as direct launches, both against the
     restatement."""
