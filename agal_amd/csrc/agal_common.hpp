@@ -332,14 +332,6 @@ struct bf16s {
     uint16_t bits;
 };
 
-__host__ __device__ inline uint16_t bf16_bits_rne(float x)
-{
-    uint32_t u = __builtin_bit_cast(uint32_t, x);
-    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
 template <typename T, typename S>
 struct Store {  // S == T: the pool holds f
     __device__ static T dec(S s, T) { return s; }
@@ -362,7 +354,9 @@ struct Store<float, _Float16> {
 template <>
 struct Store<float, bf16s> {
     __device__ static float dec(bf16s s, float w) { return __builtin_bit_cast(float, (uint32_t)s.bits << 16) + w; }
-    __device__ static bf16s enc(float f, float w) { return bf16s{bf16_bits_rne(f - w)}; }
+    // A plain conversion to __bf16 is gfx950's v_cvt_pk_bf16_f32: round to nearest, ties to even,
+    // subnormals kept, overflow to infinity; a NaN stays a NaN.
+    __device__ static bf16s enc(float f, float w) { return bf16s{__builtin_bit_cast(uint16_t, (__bf16)(f - w))}; }
     __device__ static bool positive(bf16s s, float w) { return __builtin_bit_cast(float, (uint32_t)s.bits << 16) > -w; }
     static constexpr bool quantized = true;
 };
