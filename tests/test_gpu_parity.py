@@ -202,7 +202,7 @@ def test_stream_guard_exceptions():
     assert_same(g, o)
 
 
-@pytest.mark.parametrize("prec", [capi.FP32, capi.FP16])
+@pytest.mark.parametrize("prec", [capi.FP32, capi.FP16, capi.BF16])
 def test_stream_guard_exceptions_across_levels(prec):
     """Guard exceptions on several levels at once: the coarse levels' fused sub-steps share the
     finest level's launches and the fixup serves each level's own list, while the first of each
@@ -218,10 +218,10 @@ def test_stream_guard_exceptions_across_levels(prec):
     assert g.sizes.n_ids[2] > 0
     f = o.populations()
     rng = np.random.default_rng(5)
-    # A milder perturbation for 16-bit pools: the perturbed flow grows without bound, past what a
-    # 16-bit deviation holds (65504) within these steps at larger amplitudes, and the two sides may
-    # differ in which non-finite value they carry (0 * inf terms).
-    amp = 1.5 if prec == capi.FP32 else 0.35
+    # A milder perturbation for FP16 pools: the perturbed flow grows without bound, past what a
+    # binary16 deviation holds (65504) within these steps at larger amplitudes, and the two sides may
+    # differ in which non-finite value they carry (0 * inf terms).  BF16 has FP32's range.
+    amp = 0.35 if prec == capi.FP16 else 1.5
     f = f * (1.0 + amp * rng.standard_normal(f.shape).astype(f.dtype))
     both((g, o), lambda m: m.set_populations(f))
     for k in range(4):
