@@ -222,7 +222,7 @@ int predictor_features(Ctx *c);
 int predictor_predict_blocks(Ctx *c);
 int predictor_mark(Ctx *c, float thr, int *n_marked);
 int predictor_keep(Ctx *c, float thr);
-int predictor_judge(Ctx *c, int out[2]);
+int predictor_audit(Ctx *c, int out[2]);
 int predictor_snapshot(Ctx *c, int slot, int max_slots);
 int predictor_train_snapshots(Ctx *c, int n_snap, int batch, int steps, float lr, float pos_weight, uint64_t &rng,
                               double *last_loss);

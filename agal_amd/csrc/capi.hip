@@ -490,7 +490,7 @@ int criterion_labels(Ctx *c)
 // network (`adam_steps` Adam steps on batches sampled from the snapshots).  After that the
 // network runs at every `infer_every` step: unrefined blocks it predicts are refined at once
 // (a refine-only rebuild), and at each rebuild its predicted blocks are kept from coarsening while
-// the plain criterion still refines what it flags (fallback).  The judge counts, at every rebuild,
+// the plain criterion still refines what it flags (fallback).  The audit counts, at every rebuild,
 // the blocks the criterion wants refined and those of them still unrefined (missed).
 int advance_loop_learned_impl(Ctx *c, int i0, int n, const agal_learn_opts *o, agal_learn_stats *S)
 {
@@ -545,10 +545,10 @@ int advance_loop_learned_impl(Ctx *c, int i0, int n, const agal_learn_opts *o, a
             AGAL_HIP(c, hipStreamSynchronize(c->stream));
             S->predictor_ms += ms_since(t0);
         } else {
-            // Judge and labels from the plain criterion, then its marks.
+            // Audit and labels from the plain criterion, then its marks.
             if ((st = criterion_labels(c))) return st;
             int jc[2] = {0, 0};
-            if ((st = predictor_judge(c, jc))) return st;
+            if ((st = predictor_audit(c, jc))) return st;
             if (training) {
                 S->flagged_train += jc[0];
                 S->missed_train += jc[1];

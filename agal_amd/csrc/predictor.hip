@@ -3,7 +3,7 @@
 // plain criterion's labels, and inference over all blocks.
 //
 // No counterpart in the reference.  The plain criterion (Cu_ComputeRefCriteria,
-// solver_lbm_criterion.cu:438-592) stays the fallback and the judge: its label for a block is
+// solver_lbm_criterion.cu:438-592) stays the fallback and the referee: its label for a block is
 // "the criterion wants this block to have children" (the refine / keep decision of its marks).
 //
 // Parameters, one flat FP32 master array of NPARAM values (row-major):
@@ -375,9 +375,9 @@ __global__ void k_keep_predicted(int id_max, const float *__restrict__ prob, flo
     if (b < id_max && ref[b] == REF_MARK_COARSEN && prob[b] > thr) ref[b] = REF_REFINED;
 }
 
-// Judge counts of the plain criterion against the current mesh: [0] blocks the criterion wants
+// Audit counts of the plain criterion against the current mesh: [0] blocks the criterion wants
 // refined, [1] of those still unrefined (missed).
-__global__ void k_judge(int id_max, const int *__restrict__ label, const int *__restrict__ nbr_child,
+__global__ void k_audit(int id_max, const int *__restrict__ label, const int *__restrict__ nbr_child,
                         int *__restrict__ counts)
 {
     const int b = blockIdx.x * PWG + threadIdx.x;
@@ -609,13 +609,13 @@ int predictor_keep(Ctx *c, float thr)
     return AGAL_OK;
 }
 
-int predictor_judge(Ctx *c, int out[2])
+int predictor_audit(Ctx *c, int out[2])
 {
     Predictor &P = c->pred;
     const int idm = c->id_max[c->MAX_LEVELS];
     AGAL_HIP(c, hipMemsetAsync(P.counts, 0, sizeof(int) * 4, c->stream));
     if (idm > 0)
-        hipLaunchKernelGGL(k_judge, dim3(pgrid(idm)), dim3(PWG), 0, c->stream, idm, P.label, c->nbr_child, P.counts);
+        hipLaunchKernelGGL(k_audit, dim3(pgrid(idm)), dim3(PWG), 0, c->stream, idm, P.label, c->nbr_child, P.counts);
     AGAL_HIP(c, hipGetLastError());
     AGAL_HIP(c, hipMemcpyAsync(out, P.counts, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
     AGAL_HIP(c, hipStreamSynchronize(c->stream));

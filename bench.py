@@ -293,7 +293,7 @@ def learned_study(p, args):
     """The cavity workload with the learned refinement predictor (12-64-64-1 MLP, FP16 weights,
     Adam lr 1e-3, batches of 4096 blocks): features snapshotted every 25 coarse steps and trained on
     the plain criterion's labels at each rebuild over the first 2000 steps, then inference every 25
-    steps.  The same run without applying predictions gives the plain method's numbers; the judge is
+    steps.  The same run without applying predictions gives the plain method's numbers; the referee is
     the plain criterion at every rebuild (blocks it flags that are still unrefined = missed)."""
     from agal_amd import capi
     from agal_amd.mesh import LbmMesh

@@ -202,7 +202,7 @@ int  agal_refine_near_wall(agal_ctx *ctx);
 /* ---- learned refinement predictor (no counterpart in the reference) --------------------------
  * A 12-64-64-1 network (ReLU, sigmoid output) over 12 features per block, FP16 weights with FP32
  * accumulation, trained online with Adam on the plain criterion's labels (the criterion of
- * agal_ref_criteria stays the fallback and the judge).  Features of block b, from the stored
+ * agal_ref_criteria stays the fallback and the referee).  Features of block b, from the stored
  * macroscopic fields: plog2(x + 2^-30)/16 of the block's max |u|, max |rho-1|, max |grad u|,
  * max |vorticity| (plog2 = piecewise-linear log2, exact at powers of two), level/(levels-1), a
  * signed distance (to the geometry's nearest triangle centroid, else to the nearest domain face,
@@ -221,7 +221,7 @@ typedef struct agal_learn_opts {
     double threshold;        /* a block is predicted to need children when p > threshold (0.5)    */
     double pos_weight;       /* loss weight of positive labels (4)                                */
     unsigned long long seed; /* weight initialisation and batch sampling                          */
-    int    use_predictions;  /* 0: plain criterion only (training and judge counts still run)     */
+    int    use_predictions;  /* 0: plain criterion only (training and audit counts still run)     */
 } agal_learn_opts;
 
 typedef struct agal_learn_stats {
@@ -256,7 +256,7 @@ int  agal_criterion_labels(agal_ctx *ctx, int *out, long long n_blocks);
 void agal_learn_opts_default(agal_learn_opts *opts);
 /* M_AdvanceLoop (src/mesh/mesh_advance.cu:421-463) with the predictor: snapshots and training
  * during the first train_steps coarse steps, then predicted blocks refined every infer_every steps
- * and kept from coarsening at rebuilds, the plain criterion refining what it flags; judge counts
+ * and kept from coarsening at rebuilds, the plain criterion refining what it flags; audit counts
  * and timings in *stats. */
 int  agal_advance_loop_learned(agal_ctx *ctx, int i0, int n, const agal_learn_opts *opts,
                                agal_learn_stats *stats);
