@@ -254,6 +254,8 @@ def storage_study(p, args):
         q = capi.Params.from_buffer_copy(p)
         q.precision = v
         ctx[k] = LbmMesh(q, args.capacity)
+        if args.no_graphs:
+            ctx[k].set_option(capi.OPT_GRAPHS, 0)
         ctx[k].S_SetIC(0)
         ctx[k].M_FreezeRefinedCells(0)
     t0 = time.time()
