@@ -202,15 +202,16 @@ def test_stream_guard_exceptions():
     assert_same(g, o)
 
 
-@pytest.mark.parametrize("prec", [capi.FP32, capi.FP16, capi.BF16])
-def test_stream_guard_exceptions_across_levels(prec):
+@pytest.mark.parametrize("vs,prec", [(capi.D3Q19, capi.FP32), (capi.D3Q19, capi.FP16), (capi.D3Q19, capi.BF16),
+                                     (capi.D3Q27, capi.FP32)])
+def test_stream_guard_exceptions_across_levels(vs, prec):
     """Guard exceptions on several levels at once: the coarse levels' fused sub-steps share the
     finest level's launches and the fixup serves each level's own list, while the first of each
     pair of finest sub-steps leaves its list for the second to apply as it loads.  A developed
     three-level mesh gets perturbed populations (many non-positive post-collision values on every
     level), then coarse steps as recorded graphs and as direct launches, both against the
     restatement."""
-    p = ldc_params(vs=capi.D3Q19, N=16, Re=40.0, levels=3, P_REFINE=15, prec=prec)
+    p = ldc_params(vs=vs, N=16, Re=40.0, levels=3, P_REFINE=15, prec=prec)
     g, o = pair(p, 8192)
     both((g, o), lambda m: m.S_SetIC(0))
     both((g, o), lambda m: m.M_FreezeRefinedCells(0))
