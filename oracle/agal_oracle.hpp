@@ -272,6 +272,13 @@ struct Oracle {
     // collide + boundary hook (before the swap, whose guard sees stored values), by the initial
     // condition, by interpolation and averaging, and by uploads.
     int store_mode = 0;
+    // Planted mistake for the tests that show the pins catch it (0 = none, the restatement as
+    // written).  1: interpolation without the tau rescale; 2: averaging without it; 3: linear
+    // interpolation at child offsets +1/4 instead of -1/4; 4: averaging that drops one child cell
+    // of the 2^d group; 5: the advance skips the mid-cycle interpolation; 6: the advance skips
+    // the average after a level's second sub-cycle; 7: the root level averages before its finer
+    // levels advance; 8: the stream swaps without the f_pb > 0 guard.
+    int plant = 0;
 
     T quant(T f, int dir) const
     {
@@ -745,7 +752,7 @@ struct Oracle {
                     int nc = nbr_cell_id(DIM, Ip, Jp, Kp);
                     T f_pb = (T)(-1.0);
                     if (nb > -1) f_pb = Fp(V.pb[p])[(long)nb * M_CBLOCK + nc];
-                    if (valid_mask != CELL_SOLID && f_pb > 0 && dQ < 0) {
+                    if (valid_mask != CELL_SOLID && (f_pb > 0 || (plant == 8 && nb > -1)) && dQ < 0) {
                         Fp(V.pb[p])[(long)nb * M_CBLOCK + nc] = f_p;
                         Fp(p)[cell] = f_pb;
                     }
@@ -790,7 +797,7 @@ struct Oracle {
         if (L >= MAX_LEVELS - 1) return;
         refresh(L);
         if (n_ids[L] <= 0) return;
-        const T tau_ratio = tau_vec[L + 1] / tau_vec[L];
+        const T tau_ratio = plant == 1 ? (T)1 : tau_vec[L + 1] / tau_vec[L];
         const T M_ONE_TWELFTH = (T)(-0.083333333333333), ONE_SIXTH = (T)(0.166666666666667);
         const T TWO_THIRDS = (T)(0.666666666666667), ZERO_THIRDS = (T)(0.0);
         const int NCH = N_CHILDREN;
@@ -856,7 +863,8 @@ struct Oracle {
         if (L >= MAX_LEVELS - 1) return;  // M_Interpolate guard (src/mesh/mesh_comm.cu:9)
         refresh(L);
         if (n_ids[L] <= 0) return;
-        const T tau_ratio = tau_vec[L + 1] / tau_vec[L];
+        const T tau_ratio = plant == 1 ? (T)1 : tau_vec[L + 1] / tau_vec[L];
+        const T off = plant == 3 ? (T)(0.25) : (T)(-0.25);
         const int nL = n_ids[L];
         const int *ids = idset(L);
 #pragma omp parallel for schedule(static)
@@ -873,9 +881,9 @@ struct Oracle {
             for (int t = 0; t < M_CBLOCK; t++) {
                 int I, J, K;
                 cell_ijk(t, I, J, K);
-                Ixs[t] = (T)(-0.25) + (T)(0.5) * (T)I;
-                Iys[t] = (T)(-0.25) + (T)(0.5) * (T)J;
-                Izs[t] = (DIM == 3) ? (T)(-0.25) + (T)(0.5) * (T)K : (T)0;
+                Ixs[t] = off + (T)(0.5) * (T)I;
+                Iys[t] = off + (T)(0.5) * (T)J;
+                Izs[t] = (DIM == 3) ? off + (T)(0.5) * (T)K : (T)0;
                 long cell = (long)b * M_CBLOCK + t;
                 rho[t] = Fp(Q + 0)[cell];
                 uu[t] = Fp(Q + 1)[cell];
@@ -934,7 +942,7 @@ struct Oracle {
         if (L >= MAX_LEVELS - 1) return;  // M_Average guard (src/mesh/mesh_comm.cu:21)
         refresh(L + 1);
         if (n_ids[L] <= 0) return;
-        const T tau_ratio = tau_vec[L] / tau_vec[L + 1];
+        const T tau_ratio = plant == 2 ? (T)1 : tau_vec[L] / tau_vec[L + 1];
         const int nL = n_ids[L];
         const int *ids = idset(L);
 #pragma omp parallel for schedule(static)
@@ -980,6 +988,12 @@ struct Oracle {
                                     if (DIM == 2)
                                         Fp(V.pb[p])[pcell] = quant((T)(0.25) * (s_Fc[c0 + 0 + 4 * 0] + s_Fc[c0 + 1 + 4 * 0] +
                                                                                 s_Fc[c0 + 0 + 4 * 1] + s_Fc[c0 + 1 + 4 * 1]),
+                                                                   p);
+                                    else if (plant == 4)
+                                        Fp(V.pb[p])[pcell] = quant((T)(0.125) * (s_Fc[c0 + 0 + 4 * 0 + 16 * 0] + s_Fc[c0 + 1 + 4 * 0 + 16 * 0] +
+                                                                                 s_Fc[c0 + 0 + 4 * 1 + 16 * 0] + s_Fc[c0 + 1 + 4 * 1 + 16 * 0] +
+                                                                                 s_Fc[c0 + 0 + 4 * 0 + 16 * 1] + s_Fc[c0 + 1 + 4 * 0 + 16 * 1] +
+                                                                                 s_Fc[c0 + 0 + 4 * 1 + 16 * 1] + s_Fc[c0 + 0 + 4 * 1 + 16 * 1]),
                                                                    p);
                                     else
                                         Fp(V.pb[p])[pcell] = quant((T)(0.125) * (s_Fc[c0 + 0 + 4 * 0 + 16 * 0] + s_Fc[c0 + 1 + 4 * 0 + 16 * 0] +
@@ -1031,8 +1045,9 @@ struct Oracle {
                 global_Liter = 0;
                 interpolate(L, INTERP_INTERFACE);
                 collide_stream(L);
+                if (plant == 7) average(L, average_advance());
                 advance(L + 1);
-                average(L, average_advance());
+                if (plant != 7) average(L, average_advance());
             } else {
                 if (L < MAX_LEVELS - 1) interpolate(L, INTERP_INTERFACE);
                 collide_stream(L);
@@ -1040,13 +1055,13 @@ struct Oracle {
                 if (L < MAX_LEVELS - 1) {
                     advance(L + 1);
                     average(L, average_advance());
-                    interpolate(L, INTERP_INTERFACE);
+                    if (plant != 5) interpolate(L, INTERP_INTERFACE);
                 }
                 collide_stream(L);
                 if (L == MAX_LEVELS - 1) global_Liter++;
                 if (L < MAX_LEVELS - 1) {
                     advance(L + 1);
-                    average(L, average_advance());
+                    if (plant != 6) average(L, average_advance());
                 }
             }
         } else {

@@ -116,6 +116,9 @@ int oracle_get_sizes(void *h, agal_sizes *s)
 
 int oracle_set_ic(void *h, int L) { return dispatch(h, [&](auto &O) { O.set_ic(L); return AGAL_OK; }); }
 int oracle_advance(void *h, int L) { return dispatch(h, [&](auto &O) { O.advance(L); return AGAL_OK; }); }
+// Planted mistake of the restatement (0 = none); only the tests that show the pins catch
+// mistakes set it.
+int oracle_plant(void *h, int code) { return dispatch(h, [&](auto &O) { O.plant = code; return AGAL_OK; }); }
 int oracle_collide(void *h, int L) { return dispatch(h, [&](auto &O) { O.collide(L); return AGAL_OK; }); }
 int oracle_impose_bc(void *h, int L) { return dispatch(h, [&](auto &O) { O.impose_bc(L); return AGAL_OK; }); }
 int oracle_stream(void *h, int L) { return dispatch(h, [&](auto &O) { O.stream(L); return AGAL_OK; }); }

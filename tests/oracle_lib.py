@@ -27,6 +27,7 @@ def oracle_library():
         _lib = C.CDLL(ORACLE_PATH)
         for extra, (res, args) in {
             "set_threads": (C.c_int, [C.c_int]),
+            "plant": (C.c_int, [C.c_void_p, C.c_int]),
             "collide": (C.c_int, [C.c_void_p, C.c_int]),
             "impose_bc": (C.c_int, [C.c_void_p, C.c_int]),
             "stream": (C.c_int, [C.c_void_p, C.c_int]),
@@ -62,6 +63,10 @@ class OracleMesh(LbmMesh):
         lib = oracle_library()
         fns = capi.bind(lib, "oracle_", _NAMES)
         super().__init__(params, capacity, fns=fns, lib=lib, prefix="oracle_")
+
+    def plant(self, code: int):
+        """Plant a known mistake in the restatement (see Oracle::plant); 0 removes it."""
+        self._call_raw("plant", int(code))
 
     def collide(self, L):
         self._call_raw("collide", L)
