@@ -1,19 +1,30 @@
 """Benchmark of the octree-AMR lattice-Boltzmann step on one MI355X.
 
-Workload (one "step" = one coarse time step, S_Advance of the root level with recursive
-sub-cycling): D3Q19 BGK lid-driven cavity, root grid 64^3 in 4x4x4 cell blocks, 3 refinement
-levels marked by the velocity-gradient-magnitude criterion (|grad u| above 2^-2 on the root level,
-halved per level) and re-evaluated every 100 coarse steps, Re = 1000, lid
-velocity 0.05, FP32 populations in structure-of-arrays layout.  The warm-up runs the real driver
-(M_AdvanceLoop: steps plus solution-based rebuilds) from rest so the mesh is the one the flow
-produces; the timed window then runs K coarse steps.
+Headline workload (one "step" = one coarse time step, S_Advance of the root level with recursive
+sub-cycling): D3Q19 BGK lid-driven cavity, root grid 64^3 in 4x4x4 cell blocks, 3 refinement levels
+marked by the velocity-gradient-magnitude criterion (|grad u| above 2^-2 on the root level, halved
+per level) and re-evaluated every 100 coarse steps, Re 1000, lid velocity 0.05, FP32 populations in
+structure-of-arrays layout.
 
-value: MLUPS counting every cell update of every active block (2^L sub-steps on level L) divided
-by the step time of the window, rebuild time excluded (reported separately, and folded in at one
-measured rebuild per 100 coarse steps as mlups_rebuilds_amortized).
-roofline: the fused collide+boundary+stream kernel, algorithmic bytes 2*Q*4 per cell update
-divided by its average launch time from HIP events on the context stream.
-cpu_baseline: the CPU restatement (OpenMP) stepping a copy of the same mesh for a bounded sample.
+Phases (the mesh is the one the flow produces, independent of --warmup):
+  1. development: `--develop` coarse steps of the real driver (M_AdvanceLoop: steps plus the
+     solution-based rebuilds every 100 steps) from rest; the run stops with an error if the finer
+     levels stay empty;
+  2. warm-up: `--warmup` coarse steps on the developed mesh, untimed;
+  3. roofline pass: `--roofline-steps` coarse steps with HIP events around every launch of the
+     fused collide + boundary + stream kernel (direct launches on the context stream);
+  4. timed window: `--steps` coarse steps of the driver (recorded HIP graphs), HIP events on the
+     context stream; a rebuild that falls inside the window is timed separately and the cell
+     updates are counted on the mesh of each stretch between rebuilds.
+
+value: actual cell updates per second (every active block, 2^L sub-steps on level L) in MLUPS.
+roofline: the fused kernel, algorithmic bytes 2*Q*4 per cell update / its average launch time.
+cpu_baseline: the CPU restatement (the oracle, OpenMP over blocks) stepping a copy of the
+developed mesh on all host cores (up to 16), plus a serial leg.
+
+A second leg runs the large cavity (root 256^3, 5 levels, Re 5000, rebuild every 50 coarse
+steps) in FP32 and FP16 storage; studies of the storage precision, the learned refinement
+predictor and the D3Q27 MRT sphere flow follow.
 """
 from __future__ import annotations
 
@@ -28,22 +39,29 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MLUPS + %HBM roofline, D3Q19 AMR; FP16-store vs FP32 at ≤1e-3 vel err"
 T_START = time.time()
+HBM_PEAK_GBS = 8000.0
+U_LID = 0.05
 
 
 def log(msg: str):
     """Progress line on stderr (the JSON result is the only stdout line)."""
     print(f"[bench {time.time() - T_START:7.1f}s] {msg}", file=sys.stderr, flush=True)
-HBM_PEAK_GBS = 8000.0
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=99)
-    ap.add_argument("--warmup", type=int, default=1000)
-    ap.add_argument("--capacity", type=int, default=400000, help="block pool size")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--develop", type=int, default=1000,
+                    help="coarse steps from rest with rebuilds before the warm-up (builds the adaptive mesh)")
+    ap.add_argument("--capacity", type=int, default=400000, help="block pool size of the 64^3 cavity")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample (both legs)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the 256^3 five-level cavity leg")
+    ap.add_argument("--large-develop", type=int, default=300)
+    ap.add_argument("--large-steps", type=int, default=10)
+    ap.add_argument("--large-capacity", type=int, default=6000000)
     ap.add_argument("--no-storage-study", action="store_true",
                     help="skip the FP16/BF16/FP32/FP64 comparison on one mesh sequence")
     ap.add_argument("--no-sphere-study", action="store_true",
@@ -55,10 +73,186 @@ def parse():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly instead of replaying recorded step graphs (same kernels "
                          "and results; for runs under a kernel tracer)")
+    ap.add_argument("--only-headline", action="store_true", help="headline leg only (no CPU, large leg or studies)")
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (torch) to calibrate HBM byte counters")
     return ap.parse_args()
+
+
+def cavity_params(N, levels, Re, P_REFINE, precision=None):
+    from agal_amd import capi
+    return capi.default_params(velocity_set=capi.D3Q19, precision=capi.FP32 if precision is None else precision,
+                               Nx=N, MAX_LEVELS_WALL=1, MAX_LEVELS_INTERIOR=levels, v0=U_LID / Re,
+                               bc_preset=capi.BC_LDC3D, bc_velocity=U_LID, P_REFINE=P_REFINE, N_REFINE_START=-2.0,
+                               N_REFINE_INC=1.0, N_REFINE_MAX=1.0, ic_rho=1.0, S_CRITERION=2)
+
+
+def blocks(m):
+    s = m.refresh_sizes()
+    return [int(s.n_ids[L]) for L in range(s.max_levels)]
+
+
+def updates_per_step(n_ids):
+    """Actual cell updates of one coarse step: every active block, 2^L sub-steps on level L."""
+    return sum(n * 64 * (1 << L) for L, n in enumerate(n_ids))
+
+
+def leaf_updates_per_step(n_ids):
+    """The reference's MLUPS count (solver_lbm_advance.cu:81-87): leaf blocks weighted 2^L."""
+    ML = len(n_ids)
+    return sum((n_ids[L] - (n_ids[L + 1] // 8 if L + 1 < ML else 0)) * 64 * (1 << L) for L in range(ML))
+
+
+def develop(m, n, what):
+    """n coarse steps of the driver from rest (rebuilds included); returns the next step index."""
+    t0 = time.time()
+    done = 0
+    while done < n:
+        k = min(n - done, 250)
+        m.advance_loop(done, k)
+        done += k
+        log(f"{what}: developed {done}/{n} coarse steps, blocks per level {blocks(m)}")
+    m.sync()
+    return n, time.time() - t0
+
+
+def timed_window(m, i0, K):
+    """K coarse steps of the driver from step index i0, split at rebuilds; returns step and rebuild
+    times and the cell updates counted per stretch on its own mesh."""
+    P = m.params.P_REFINE
+    i, left = i0, K
+    step_ms = rebuild_ms = 0.0
+    upd = upd_ref = 0
+    while left > 0:
+        chunk = min(left, P - (i % P))  # the last step of a chunk may trigger a rebuild
+        nid = blocks(m)
+        st, s_ms, r_ms = m.advance_loop(i, chunk)
+        step_ms += s_ms
+        rebuild_ms += r_ms
+        upd += updates_per_step(nid) * chunk
+        upd_ref += leaf_updates_per_step(nid) * chunk
+        i += chunk
+        left -= chunk
+    m.sync()
+    return {"step_ms": step_ms, "rebuild_ms": rebuild_ms, "updates": upd, "updates_reference": upd_ref, "next": i}
+
+
+def roofline_pass(m, steps):
+    """Average launch time of the fused kernel over `steps` coarse steps (direct launches, HIP
+    events recorded by the dispatches themselves) and the cells those launches updated."""
+    import ctypes as C
+    m._f["timing_reset"](m.h, 1)
+    for _ in range(steps):
+        m.S_Advance(0)
+    m.sync()
+    fms, nl, fcells = C.c_double(), C.c_longlong(), C.c_double()
+    m._f["timing_get"](m.h, C.byref(fms), C.byref(nl), C.byref(fcells))
+    m._f["timing_reset"](m.h, 0)
+    return fms.value, int(nl.value), fcells.value
+
+
+def profiled_traffic(kernel_file, n_ids):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, when it
+    was measured on this exact mesh (same blocks per level); otherwise None."""
+    tp = os.path.join(ROOT, "profiles", kernel_file)
+    try:
+        d = json.load(open(tp))
+    except Exception:
+        return None, None
+    if list(d.get("blocks_per_level", [])) != list(n_ids):
+        return None, d.get("blocks_per_level")
+    return d.get("bytes_per_launch"), d.get("blocks_per_level")
+
+
+def cavity_leg(args, N, levels, Re, P_REFINE, capacity, develop_steps, steps, what, precision=None):
+    from agal_amd import capi
+    from agal_amd.mesh import LbmMesh
+
+    p = cavity_params(N, levels, Re, P_REFINE, precision)
+    m = LbmMesh(p, capacity)
+    if args.no_graphs:
+        m.set_option(capi.OPT_GRAPHS, 0)
+    m.S_SetIC(0)
+    m.M_FreezeRefinedCells(0)
+    i, dev_s = develop(m, develop_steps, what)
+    nid = blocks(m)
+    if any(n == 0 for n in nid[1:]):
+        raise SystemExit(f"{what}: the mesh did not develop its finer levels after {develop_steps} coarse steps "
+                         f"({nid}); nothing adaptive to time")
+    log(f"{what}: warm-up {args.warmup} coarse steps")
+    if args.warmup > 0:
+        w = timed_window(m, i, args.warmup)
+        i = w["next"]
+    nid_roof = blocks(m)
+    log(f"{what}: roofline pass ({args.roofline_steps} coarse steps, events per fused launch)")
+    fused_ms, launches, fcells = roofline_pass(m, args.roofline_steps)
+    log(f"{what}: timed window of {steps} coarse steps")
+    w = timed_window(m, i, steps)
+    nid_end = blocks(m)
+    step_s = w["step_ms"] * 1e-3
+    Q = 19
+    b = 2 if p.precision in (capi.FP16, capi.BF16) else (8 if p.precision == capi.FP64 else 4)
+    alg_bytes = 2 * Q * b * fcells
+    achieved = alg_bytes / (fused_ms * 1e-3) / 1e9 if fused_ms > 0 else 0.0
+    fine_equiv = (N << (levels - 1)) ** 3 * (1 << (levels - 1))
+    out = {
+        "m": m,
+        "p": p,
+        "develop_steps": develop_steps,
+        "develop_seconds": round(dev_s, 2),
+        "blocks_per_level": nid_roof,
+        "blocks_per_level_end": nid_end,
+        "cell_updates_per_step": updates_per_step(nid_roof),
+        "mlups": w["updates"] / step_s / 1e6,
+        "mlups_reference_formula": w["updates_reference"] / step_s / 1e6,
+        "mlups_fine_equivalent": fine_equiv * steps / step_s / 1e6,
+        "mlups_end_to_end": w["updates"] / ((w["step_ms"] + w["rebuild_ms"]) * 1e-3) / 1e6,
+        "ms_per_step": w["step_ms"] / steps,
+        "rebuild_ms_in_window": w["rebuild_ms"],
+        "step_hbm_gbs_algorithmic": 2 * Q * b * w["updates"] / step_s / 1e9,
+        "fused": {"ms": fused_ms, "launches": launches, "cells": fcells, "alg_bytes": alg_bytes,
+                  "achieved": achieved, "bytes_per_cell_update": 2 * Q * b},
+        "next_step": w["next"],
+    }
+    return out
+
+
+def headline(args):
+    from agal_amd import capi
+
+    N, levels, Re = 64, 3, 1000.0
+    leg = cavity_leg(args, N, levels, Re, 100, args.capacity, args.develop, args.steps, "cavity 64^3")
+    m, p = leg["m"], leg["p"]
+    # Rebuild cost on this mesh (one solution-based rebuild after the window).
+    t1 = time.time()
+    m.M_Advance_RefineWithSolution()
+    m.sync()
+    rebuild_one_ms = (time.time() - t1) * 1e3
+    step_ms = leg["ms_per_step"]
+    # One rebuild per P_REFINE coarse steps at this cost (host wall clock, an upper bound):
+    mlups_amortized = leg["cell_updates_per_step"] / ((step_ms + rebuild_one_ms / p.P_REFINE) * 1e-3) / 1e6
+    fu = leg["fused"]
+    traffic, prof_mesh = profiled_traffic("k_collide_stream_d3q19_f32_ldc64_3lev_traffic.json",
+                                          leg["blocks_per_level"])
+    roof = {
+        "bound": "hbm",
+        "achieved": round(fu["achieved"], 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(fu["achieved"] / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "kernel": "k_collide_stream<float, float, 19, false>",
+        "launches": fu["launches"],
+        "avg_launch_ms": round(fu["ms"] / max(1, fu["launches"]), 5),
+        "bytes_per_launch_algorithmic": round(fu["alg_bytes"] / max(1, fu["launches"])),
+        "bytes_per_cell_update_algorithmic": fu["bytes_per_cell_update"],
+        "blocks_per_level": leg["blocks_per_level"],
+        "traffic_source": ("profiles/k_collide_stream_d3q19_f32_ldc64_3lev_traffic.json (rocprofv3 FETCH_SIZE / "
+                           "WRITE_SIZE on this mesh)") if traffic else
+                          f"null: the committed counter profile is for mesh {prof_mesh}",
+    }
+    return leg, roof, rebuild_one_ms, mlups_amortized
 
 
 def main():
@@ -78,98 +272,33 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU partitioning is "
               "a design, not built.", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
+    if args.only_headline:
+        args.no_cpu = args.no_large = args.no_storage_study = args.no_sphere_study = args.no_learned_study = True
 
-    from agal_amd import capi
-    from agal_amd.mesh import LbmMesh
-
-    Q, b = 19, 4
-    N, U, Re, levels = 64, 0.05, 1000.0, 3
-    p = capi.default_params(velocity_set=capi.D3Q19, precision=capi.FP32, Nx=N, MAX_LEVELS_WALL=1,
-                            MAX_LEVELS_INTERIOR=levels, v0=(U * N / Re) / N, bc_preset=capi.BC_LDC3D,
-                            bc_velocity=U, P_REFINE=100, N_REFINE_START=-2.0, N_REFINE_INC=1.0,
-                            N_REFINE_MAX=1.0, ic_rho=1.0, S_CRITERION=2)
-    m = LbmMesh(p, args.capacity)
-    if args.no_graphs:
-        m.set_option(capi.OPT_GRAPHS, 0)
-    m.S_SetIC(0)
-    m.M_FreezeRefinedCells(0)
-    log(f"cavity: {args.warmup} warm-up coarse steps with rebuilds")
-    t0 = time.time()
-    st_w, _, reb_w = m.advance_loop(0, args.warmup)
-    m.sync()
-    warm_s = time.time() - t0
-    log(f"cavity: warm-up done in {warm_s:.1f} s; roofline pass and timed window")
-    s = m.refresh_sizes()
-    n_ids = [int(s.n_ids[L]) for L in range(levels)]
-
-    # Roofline pass on the same mesh: HIP events around every fused launch (direct launches; the
-    # stepping state advances but no rebuild is triggered).
-    import ctypes as C
-    m._f["timing_reset"](m.h, 1)
-    for _ in range(args.roofline_steps):
-        m.S_Advance(0)
-    m.sync()
-    fms, nl, fcells = C.c_double(), C.c_longlong(), C.c_double()
-    m._f["timing_get"](m.h, C.byref(fms), C.byref(nl), C.byref(fcells))
-    m._f["timing_reset"](m.h, 0)
-
-    # Timed window: K coarse steps of the driver (each step replayed from a recorded HIP graph),
-    # bracketed by HIP events on the context stream.
-    m.sync()
-    st, step_ms, rebuild_ms = m.advance_loop(args.warmup, args.steps)
-    m.sync()
-    s = m.refresh_sizes()
-    n_ids_end = [int(s.n_ids[L]) for L in range(levels)]
-
-    # Cell updates per coarse step (the mesh is fixed between rebuilds).
-    cells = 64
-    upd_actual = sum(n_ids_end[L] * cells * (1 << L) for L in range(levels))
-    leaves = [n_ids_end[L] - (n_ids_end[L + 1] // 8 if L + 1 < levels else 0) for L in range(levels)]
-    upd_reference = sum(leaves[L] * cells * (1 << L) for L in range(levels))
-    upd_fine_equiv = (N << (levels - 1)) ** 3 * (1 << (levels - 1))
-    K = args.steps
-    step_s = step_ms * 1e-3
-    mlups = upd_actual * K / step_s / 1e6
-    mlups_ref = upd_reference * K / step_s / 1e6
-    mlups_fine = upd_fine_equiv * K / step_s / 1e6
-    mlups_e2e = upd_actual * K / ((step_ms + rebuild_ms) * 1e-3) / 1e6
-    fused_ms = fms.value
-    launches = nl.value
-    alg_bytes = 2 * Q * b * fcells.value
-    achieved = alg_bytes / (fused_ms * 1e-3) / 1e9 if fused_ms > 0 else 0.0
-
-    # Rebuild cost on this mesh (one solution-based rebuild after the window).
-    t1 = time.time()
-    m.M_Advance_RefineWithSolution()
-    m.sync()
-    rebuild_one_ms = (time.time() - t1) * 1e3
-    # The window holds no rebuild; with one rebuild per P_REFINE coarse steps at this cost (host
-    # wall clock, so an upper bound) the rate over a long run is:
-    mlups_amortized = upd_actual * K / ((step_ms + rebuild_one_ms * K / p.P_REFINE) * 1e-3) / 1e6
-
-    traffic = None
-    tp = os.path.join(ROOT, "profiles", "k_collide_stream_d3q19_f32_ldc64_3lev_traffic.json")
-    if os.path.exists(tp):
-        try:
-            traffic = json.load(open(tp)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
-
+    leg, roof, rebuild_one_ms, mlups_amortized = headline(args)
+    log(f"cavity 64^3: {leg['mlups']:.0f} MLUPS, fused kernel {roof['achieved']:.0f} GB/s ({roof['frac']:.3f})")
     cpu = None
     if not args.no_cpu:
-        log("CPU baseline sample")
-        cpu = cpu_baseline(m, p, args.capacity, upd_actual, args.cpu_seconds)
-    del m
+        log("CPU baseline sample (serial and all host cores)")
+        cpu = cpu_baseline(leg["m"], leg["p"], args.capacity, leg["cell_updates_per_step"], leg["blocks_per_level"],
+                           args.cpu_seconds)
+    leg["m"].close()
+    large = None
+    if not args.no_large:
+        try:
+            large = large_leg(args)
+        except Exception as e:  # reported in the line; the headline stands on its own
+            log(f"large cavity failed: {e}")
+            large = {"error": str(e)}
     storage = None
     if not args.no_storage_study:
         log("storage study: FP64 / FP32 / FP16 / BF16 on one mesh sequence")
-        storage = storage_study(p, args)
+        storage = storage_study(leg["p"], args)
     learned = None
     if not args.no_learned_study:
         log("learned refinement study: 2000 training + 2000 inference coarse steps, with and without predictions")
         try:
-            learned = learned_study(p, args)
+            learned = learned_study(leg["p"], args)
         except Exception as e:
             log(f"learned study failed: {e}")
             learned = {"error": str(e)}
@@ -178,18 +307,19 @@ def main():
         log("sphere study: D3Q27 MRT flow past the noisy icosphere")
         try:
             sphere = sphere_study(args)
-        except Exception as e:  # reported in the line; the headline measurement stands on its own
+        except Exception as e:
             log(f"sphere study failed: {e}")
             sphere = {"error": str(e)}
 
+    K = args.steps
     line = {
         "metric": METRIC,
-        "value": round(mlups, 1),
+        "value": round(leg["mlups"], 1),
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": round(step_ms / K, 4),
+        "ms_per_step": round(leg["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -197,33 +327,26 @@ def main():
         "data": "synthetic: lid-driven cavity from rest, mesh produced by the solver's own refinement",
         "config": {
             "workload": "D3Q19 BGK lid-driven cavity, root 64^3 (4x4x4 blocks), 3 levels by the |grad u| "
-                        "criterion, Re 1000, U 0.05, rebuild every 100 coarse steps, FP32 SoA",
-            "blocks_per_level": n_ids_end,
-            "cell_updates_per_step": upd_actual,
+                        "criterion, Re 1000, U 0.05, rebuild every 100 coarse steps, FP32 SoA; "
+                        f"mesh developed over {leg['develop_steps']} coarse steps from rest",
+            "blocks_per_level": leg["blocks_per_level"],
+            "blocks_per_level_end": leg["blocks_per_level_end"],
+            "cell_updates_per_step": leg["cell_updates_per_step"],
             "parallelism": "single GPU",
-            "step_launch": "direct launches" if args.no_graphs else "one recorded HIP graph per coarse step",
+            "step_launch": "direct launches" if args.no_graphs else "recorded HIP graphs, up to 4 coarse steps each",
         },
-        "mlups_reference_formula": round(mlups_ref, 1),
-        "mlups_fine_equivalent": round(mlups_fine, 1),
-        "mlups_end_to_end": round(mlups_e2e, 1),
-        "rebuild_ms_in_window": round(rebuild_ms, 3),
+        "mlups_reference_formula": round(leg["mlups_reference_formula"], 1),
+        "mlups_fine_equivalent": round(leg["mlups_fine_equivalent"], 1),
+        "mlups_end_to_end": round(leg["mlups_end_to_end"], 1),
+        "rebuild_ms_in_window": round(leg["rebuild_ms_in_window"], 3),
         "rebuild_ms_one": round(rebuild_one_ms, 3),
         "mlups_rebuilds_amortized": round(mlups_amortized, 1),
-        "warmup_seconds": round(warm_s, 3),
-        "step_hbm_gbs_algorithmic": round(2 * Q * b * upd_actual * K / step_s / 1e9, 1),
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "kernel": "k_collide_stream<float, float, 19, false>",
-            "launches": launches,
-            "avg_launch_ms": round(fused_ms / max(1, launches), 5),
-            "bytes_per_launch_algorithmic": round(alg_bytes / max(1, launches)),
-        },
+        "develop_seconds": leg["develop_seconds"],
+        "step_hbm_gbs_algorithmic": round(leg["step_hbm_gbs_algorithmic"], 1),
+        "step_roofline_frac": round(leg["step_hbm_gbs_algorithmic"] / HBM_PEAK_GBS, 4),
+        "roofline": roof,
         "cpu_baseline": cpu,
+        "large_cavity": large,
         "storage16": storage,
         "learned_refinement": learned,
         "sphere": sphere,
@@ -235,11 +358,46 @@ def main():
             f.write(txt + "\n")
 
 
+def large_leg(args):
+    """The large cavity: root 256^3 (262,144 blocks), 5 levels, Re 5000, rebuild every 50 coarse
+    steps, timed in FP32 and in FP16 storage of f - w_q (each developed from rest)."""
+    from agal_amd import capi
+
+    out = {"workload": "D3Q19 BGK lid-driven cavity, root 256^3 (4x4x4 blocks), 5 levels by the |grad u| criterion "
+                       "(2^-2 on the root level, halved per level), Re 5000, U 0.05, rebuild every 50 coarse steps",
+           "develop_steps": args.large_develop, "steps": args.large_steps}
+    for key, prec in (("fp32", capi.FP32), ("fp16", capi.FP16)):
+        log(f"large cavity {key}: develop {args.large_develop} coarse steps")
+        leg = cavity_leg(args, 256, 5, 5000.0, 50, args.large_capacity, args.large_develop, args.large_steps,
+                         f"large cavity {key}", precision=prec)
+        fu = leg["fused"]
+        out[key] = {
+            "mlups": round(leg["mlups"], 1),
+            "mlups_reference_formula": round(leg["mlups_reference_formula"], 1),
+            "mlups_fine_equivalent": round(leg["mlups_fine_equivalent"], 1),
+            "mlups_end_to_end": round(leg["mlups_end_to_end"], 1),
+            "ms_per_step": round(leg["ms_per_step"], 3),
+            "rebuild_ms_in_window": round(leg["rebuild_ms_in_window"], 2),
+            "develop_seconds": leg["develop_seconds"],
+            "blocks_per_level": leg["blocks_per_level"],
+            "blocks_per_level_end": leg["blocks_per_level_end"],
+            "cell_updates_per_step": leg["cell_updates_per_step"],
+            "step_roofline_frac": round(leg["step_hbm_gbs_algorithmic"] / HBM_PEAK_GBS, 4),
+            "fused_kernel": {"avg_launch_ms": round(fu["ms"] / max(1, fu["launches"]), 4),
+                             "achieved_gbs": round(fu["achieved"], 1), "frac": round(fu["achieved"] / HBM_PEAK_GBS, 4),
+                             "bytes_per_cell_update": fu["bytes_per_cell_update"]},
+        }
+        leg["m"].close()
+        log(f"large cavity {key}: {out[key]['mlups']:.0f} MLUPS on {leg['blocks_per_level']}")
+    out["speedup_fp16_vs_fp32"] = round(out["fp16"]["mlups"] / out["fp32"]["mlups"], 3)
+    return out
+
+
 def storage_study(p, args):
     """FP16 / BF16 pools of f - w_q against FP32 and FP64 on one mesh sequence.
 
     An FP64 run from rest leads; FP32, FP16 and BF16 runs follow it, taking its refinement marks
-    at every rebuild, so all four share every mesh.  After the warm-up the velocity of the leaf
+    at every rebuild, so all four share every mesh.  After the development the velocity of the leaf
     cells is compared with FP64, then each follower times K coarse steps (graph replays) on the
     shared mesh, and the fused kernel of each is timed per launch."""
     import ctypes as C
@@ -259,40 +417,36 @@ def storage_study(p, args):
         ctx[k].S_SetIC(0)
         ctx[k].M_FreezeRefinedCells(0)
     t0 = time.time()
-    advance_twins(ctx["fp64"], [ctx["fp32"], ctx["fp16"], ctx["bf16"]], 0, args.warmup)
+    advance_twins(ctx["fp64"], [ctx["fp32"], ctx["fp16"], ctx["bf16"]], 0, args.develop)
     warm_s = time.time() - t0
     errs = {k: velocity_error(ctx["fp64"], ctx[k]) for k in ("fp32", "fp16", "bf16")}
-    s = ctx["fp64"].refresh_sizes()
-    levels = s.max_levels
-    n_ids = [int(s.n_ids[L]) for L in range(levels)]
-    upd = sum(n_ids[L] * 64 * (1 << L) for L in range(levels))
-    out = {"warmup_steps": args.warmup, "warmup_seconds": round(warm_s, 2), "blocks_per_level": n_ids,
+    n_ids = blocks(ctx["fp64"])
+    upd = updates_per_step(n_ids)
+    out = {"develop_steps": args.develop, "develop_seconds": round(warm_s, 2), "blocks_per_level": n_ids,
            "cell_updates_per_step": upd, "vel_err_vs_fp64": {}, "mlups": {}, "fused_kernel": {}}
     bpp = {"fp64": 8, "fp32": 4, "fp16": 2, "bf16": 2}
+    K = min(max(args.steps, 20), p.P_REFINE - 1)  # no rebuild inside the window
     for k in ("fp64", "fp32", "fp16", "bf16"):
         m = ctx[k]
         log(f"storage study: timing {k}")
         m.sync()
-        _, step_ms, _ = m.advance_loop(args.warmup, args.steps)
+        _, step_ms, _ = m.advance_loop(args.develop, K)
         m.sync()
-        out["mlups"][k] = round(upd * args.steps / (step_ms * 1e-3) / 1e6, 1)
-        m._f["timing_reset"](m.h, 1)
-        for _ in range(max(1, args.roofline_steps // 2)):
-            m.S_Advance(0)
-        m.sync()
-        fms, nl, fc = C.c_double(), C.c_longlong(), C.c_double()
-        m._f["timing_get"](m.h, C.byref(fms), C.byref(nl), C.byref(fc))
-        m._f["timing_reset"](m.h, 0)
-        alg = 2 * 19 * bpp[k] * fc.value
-        out["fused_kernel"][k] = {"avg_launch_ms": round(fms.value / max(1, nl.value), 5),
-                                  "achieved_gbs": round(alg / (fms.value * 1e-3) / 1e9, 1) if fms.value > 0 else None,
+        out["mlups"][k] = round(upd * K / (step_ms * 1e-3) / 1e6, 1)
+        fms, nl, fc = roofline_pass(m, max(1, args.roofline_steps // 2))
+        alg = 2 * 19 * bpp[k] * fc
+        out["fused_kernel"][k] = {"avg_launch_ms": round(fms / max(1, nl), 5),
+                                  "achieved_gbs": round(alg / (fms * 1e-3) / 1e9, 1) if fms > 0 else None,
                                   "bytes_per_cell_update": 2 * 19 * bpp[k]}
         if k != "fp64":
             out["vel_err_vs_fp64"][k] = {"rel_l2": float(f"{errs[k]['rel_l2_velocity']:.4g}"),
                                          "max_abs": float(f"{errs[k]['max_abs_velocity']:.4g}")}
+    out["timed_steps"] = K
     out["leaf_cells_compared"] = errs["fp16"]["leaf_cells"]
     out["speedup_fp16_vs_fp32"] = round(out["mlups"]["fp16"] / out["mlups"]["fp32"], 3)
     out["fp16_within_1e-3"] = out["vel_err_vs_fp64"]["fp16"]["rel_l2"] <= 1e-3
+    for m in ctx.values():
+        m.close()
     return out
 
 
@@ -316,8 +470,7 @@ def learned_study(p, args):
         t0 = time.time()
         res[use] = m.advance_loop_learned(0, 4000, opts)
         res[use]["wall_s"] = round(time.time() - t0, 3)
-        s = m.refresh_sizes()
-        res[use]["blocks_per_level"] = [int(s.n_ids[L]) for L in range(s.max_levels)]
+        res[use]["blocks_per_level"] = blocks(m)
         m.close()
         log(f"learned study: use_predictions={use} done in {res[use]['wall_s']} s")
     for key, r in (("plain", res[0]), ("learned", res[1])):
@@ -341,7 +494,6 @@ def sphere_study(args):
     (matrix-core MRT and vector MRT) and FP16 storage of f - w (matrix-core MRT) run the same static
     mesh from the same uniform start in lockstep; leaf velocities are compared with FP64 every 250
     coarse steps and the drag (momentum exchange, FP64 reduction) is averaged over the last 500."""
-    import ctypes as C
     import math
 
     from agal_amd import capi
@@ -370,10 +522,9 @@ def sphere_study(args):
         m.sync()
         out.setdefault("near_wall_setup_s", {})[k] = round(time.time() - t0, 3)
         ctx[k] = m
-        log(f"sphere {k}: near-wall hierarchy {[int(m.sizes.n_ids[L]) for L in range(levels)]}")
-    s = ctx["fp64"].refresh_sizes()
-    n_ids = [int(s.n_ids[L]) for L in range(levels)]
-    upd = sum(n_ids[L] * 64 * (1 << L) for L in range(levels))
+        log(f"sphere {k}: near-wall hierarchy {blocks(m)}")
+    n_ids = blocks(ctx["fp64"])
+    upd = updates_per_step(n_ids)
     out["blocks_per_level"] = n_ids
     out["cell_updates_per_step"] = upd
     K = args.sphere_steps
@@ -404,17 +555,11 @@ def sphere_study(args):
     for k, m in ctx.items():
         out["mlups"][k] = round(upd * K / (ms[k] * 1e-3) / 1e6, 1)
         out["cd"][k] = round(sum(cds[k]) / len(cds[k]), 5)
-        m._f["timing_reset"](m.h, 1)
-        for _ in range(5):
-            m.S_Advance(0)
-        m.sync()
-        fms, nl, fc = C.c_double(), C.c_longlong(), C.c_double()
-        m._f["timing_get"](m.h, C.byref(fms), C.byref(nl), C.byref(fc))
-        m._f["timing_reset"](m.h, 0)
+        fms, nl, fc = roofline_pass(m, 5)
         b = {"fp64": 8, "fp32": 4, "fp32_vector": 4, "fp16": 2}[k]
-        out["fused_kernel"][k] = {"avg_launch_ms": round(fms.value / max(1, nl.value), 5),
-                                  "achieved_gbs": round(2 * 27 * b * fc.value / (fms.value * 1e-3) / 1e9, 1)
-                                  if fms.value > 0 else None}
+        out["fused_kernel"][k] = {"avg_launch_ms": round(fms / max(1, nl), 5),
+                                  "achieved_gbs": round(2 * 27 * b * fc / (fms * 1e-3) / 1e9, 1)
+                                  if fms > 0 else None}
     out["cd_window"] = f"mean over the last {K - avg_from} coarse steps, sampled every 10"
     out["vel_err_vs_fp64_series"] = series
     out["cd_rel_err_vs_fp64"] = {k: float(f"{abs(out['cd'][k] / out['cd']['fp64'] - 1):.3g}") for k in ctx if k != "fp64"}
@@ -424,31 +569,42 @@ def sphere_study(args):
     return out
 
 
-def cpu_baseline(m, p, capacity, upd_per_step, budget_s):
-    """The CPU restatement stepping the same mesh (state copied from the GPU) on the host cores."""
+def cpu_baseline(m, p, capacity, upd_per_step, n_ids, budget_s):
+    """The CPU restatement stepping the same developed mesh (state copied from the GPU): a serial
+    leg and a leg on all host cores (capped at 16, the GPU's share of the box)."""
     try:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_lib import OracleMesh, set_threads
     except Exception as e:  # the checker is not built
         return {"value": None, "unit": "MLUPS", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
-    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = set_threads(max(1, min(ncpu, 16)))
+    nproc = os.cpu_count()
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
     o = OracleMesh(p, capacity)
     o.import_state(m.state())
-    o.S_Advance(0)  # first touch
-    steps, t = 0, 0.0
-    while t < budget_s and steps < 200:
-        t0 = time.time()
-        o.S_Advance(0)
-        t += time.time() - t0
-        steps += 1
+
+    def leg(threads, budget, max_steps):
+        used = set_threads(threads)
+        o.S_Advance(0)  # first touch
+        steps, t = 0, 0.0
+        while t < budget and steps < max_steps:
+            t0 = time.time()
+            o.S_Advance(0)
+            t += time.time() - t0
+            steps += 1
+        return used, steps, t
+
+    threads, steps, t = leg(max(1, min(ncpu, 16)), 0.6 * budget_s, 400)
+    _, s_steps, s_t = leg(1, 0.4 * budget_s, 100)
     o.close()
     return {
         "value": round(upd_per_step * steps / t / 1e6, 2),
         "unit": "MLUPS",
         "cores": threads,
         "kind": "port",
-        "sample": f"{steps} coarse steps of the same 3-level mesh (FP32, OpenMP over blocks), {t:.1f} s",
+        "sample": f"{steps} coarse steps of the developed {len(n_ids)}-level mesh {n_ids} (FP32, OpenMP over "
+                  f"blocks, {threads} threads), {t:.1f} s; host nproc {nproc}, affinity {ncpu}",
+        "serial": {"value": round(upd_per_step * s_steps / s_t / 1e6, 3), "cores": 1,
+                   "sample": f"{s_steps} coarse steps of the same mesh on one thread, {s_t:.1f} s"},
     }
 
 
