@@ -117,6 +117,23 @@ class LearnStats(C.Structure):
 MLP_NFEAT, MLP_NPARAM = 12, 5057
 
 
+class Pool(C.Structure):
+    """agal_pool: a whole block pool for agal_import_pool (M_LoadToGPU)."""
+    _fields_ = [
+        ("n_blocks", C.c_longlong),
+        ("nbr", C.POINTER(C.c_int)), ("nbr_child", C.POINTER(C.c_int)),
+        ("ref", C.POINTER(C.c_int)), ("level", C.POINTER(C.c_int)), ("onb", C.POINTER(C.c_int)),
+        ("blk_mask", C.POINTER(C.c_int)),
+        ("coords", C.POINTER(C.c_float)),
+        ("cell_mask", C.POINTER(C.c_int)),
+        ("n_ids", C.POINTER(C.c_int)),
+        ("id_sets", C.POINTER(C.c_int)),
+        ("n_gaps", C.c_int),
+        ("gap_set", C.POINTER(C.c_int)),
+        ("populations", C.c_void_p),
+    ]
+
+
 class Sizes(C.Structure):
     _fields_ = [
         ("n_dim", C.c_int),
@@ -202,6 +219,7 @@ PROTOTYPES = {
     "download_coords": (_I, [_P, C.POINTER(C.c_float), _LL, _LL]),
     "download_id_set": (_I, [_P, _I, C.POINTER(_I), _I]),
     "download_gap_set": (_I, [_P, C.POINTER(_I), _I]),
+    "import_pool": (_I, [_P, C.POINTER(Pool)]),
     "set_option": (_I, [_P, _I, _LL]),
     "timing_reset": (_I, [_P, _I]),
     "timing_get": (_I, [_P, C.POINTER(C.c_double), C.POINTER(_LL), C.POINTER(C.c_double)]),

@@ -1434,6 +1434,88 @@ int agal_download_gap_set(agal_ctx *ctx, int *dst, int n)
     return AGAL_OK;
 }
 
+int agal_import_pool(agal_ctx *ctx, const agal_pool *pl)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    const int ML = c->MAX_LEVELS, QM = c->QMAX;
+    const long long nmax = c->n_maxcblocks;
+    if (!pl || !pl->nbr || !pl->nbr_child || !pl->ref || !pl->level || !pl->onb || !pl->blk_mask || !pl->coords ||
+        !pl->cell_mask || !pl->n_ids || (pl->n_gaps > 0 && !pl->gap_set))
+        return AGAL_ERR_ARG;
+    const long long nb = pl->n_blocks;
+    if (nb < c->n_coarsecblocks || nb > nmax || pl->n_gaps < 0 || pl->n_gaps > nmax) return AGAL_ERR_ARG;
+    long long total = 0;
+    for (int L = 0; L < ML; L++) {
+        if (pl->n_ids[L] < 0 || pl->n_ids[L] > nb) return AGAL_ERR_ARG;
+        total += pl->n_ids[L];
+    }
+    if (total > 0 && !pl->id_sets) return AGAL_ERR_ARG;
+    for (long long k = 0; k < total; k++)
+        if (pl->id_sets[k] < 0 || pl->id_sets[k] >= nb) return AGAL_ERR_ARG;
+    AGAL_HIP(c, hipSetDevice(c->device));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    int st;
+    // IDs the context used beyond the imported range become inactive blocks without neighbours.
+    const long long old = c->id_max[ML];
+    if (old > nb) {
+        if ((st = launch_fill(c, c->ref + nb, old - nb, REF_INACTIVE)) || (st = launch_fill(c, c->level + nb, old - nb, 0)) ||
+            (st = launch_fill(c, c->onb + nb, old - nb, 0)) || (st = launch_fill(c, c->blk_mask + nb, old - nb, BLK_REGULAR)))
+            return st;
+        for (int p = 0; p < QM; p++)
+            if ((st = launch_fill(c, c->nbr + p * nmax + nb, old - nb, N_SKIPID)) ||
+                (st = launch_fill(c, c->nbr_child + p * nmax + nb, old - nb, N_SKIPID)))
+                return st;
+    }
+    for (int p = 0; p < QM; p++) {
+        AGAL_HIP(c, hipMemcpyAsync(c->nbr + p * nmax, pl->nbr + p * nb, sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
+        AGAL_HIP(c, hipMemcpyAsync(c->nbr_child + p * nmax, pl->nbr_child + p * nb, sizeof(int) * nb, hipMemcpyHostToDevice,
+                                   c->stream));
+    }
+    AGAL_HIP(c, hipMemcpyAsync(c->ref, pl->ref, sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
+    AGAL_HIP(c, hipMemcpyAsync(c->level, pl->level, sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
+    AGAL_HIP(c, hipMemcpyAsync(c->onb, pl->onb, sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
+    AGAL_HIP(c, hipMemcpyAsync(c->blk_mask, pl->blk_mask, sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
+    for (int d = 0; d < c->DIM; d++)
+        AGAL_HIP(c, hipMemcpyAsync(c->X + d * nmax, pl->coords + d * nb, sizeof(float) * nb, hipMemcpyHostToDevice, c->stream));
+    AGAL_HIP(c, hipMemcpyAsync(c->cells_mask, pl->cell_mask, sizeof(int) * nb * c->CELLS, hipMemcpyHostToDevice, c->stream));
+    if (pl->n_gaps > 0)
+        AGAL_HIP(c, hipMemcpyAsync(c->gap_set, pl->gap_set, sizeof(int) * pl->n_gaps, hipMemcpyHostToDevice, c->stream));
+    AGAL_HIP(c, hipStreamSynchronize(c->stream));
+    c->n_gaps = pl->n_gaps;
+    for (int L = 1; L < ML; L++) c->id_max[L] = 0;
+    c->id_max[0] = c->n_coarsecblocks;
+    c->id_max[ML] = (int)nb;
+    std::fill(c->cur.begin(), c->cur.end(), 0);
+    c->pending_fix.clear();
+    c->deferred.clear();
+    if ((st = launch_fill(c, c->exc_count, 2 * (ML + 1), 0))) return st;
+    if ((st = rebuild_level_lists(c, nb))) return st;
+    if (pl->populations) {
+        c->ic_done = true;
+        if ((st = stage_pops(c, const_cast<void *>(pl->populations), nb, nb * c->CELLS, 1))) return st;
+    }
+    // The rebuilt lists must be the given ID sets.
+    long long off = 0;
+    for (int L = 0; L < ML; L++) {
+        if (c->n_ids[L] != pl->n_ids[L]) {
+            c->last_error = "agal_import_pool: level " + std::to_string(L) + " holds " + std::to_string(c->n_ids[L]) +
+                            " active blocks by ref / level, the ID set " + std::to_string(pl->n_ids[L]);
+            return AGAL_ERR_ARG;
+        }
+        std::vector<int> ids(c->n_ids[L]);
+        if (!ids.empty())
+            AGAL_HIP(c, hipMemcpy(ids.data(), c->id_set_L(L), sizeof(int) * ids.size(), hipMemcpyDeviceToHost));
+        if (!std::equal(ids.begin(), ids.end(), pl->id_sets + off)) {
+            c->last_error = "agal_import_pool: the ID set of level " + std::to_string(L) + " is not the ascending list of "
+                            "its active blocks";
+            return AGAL_ERR_ARG;
+        }
+        off += c->n_ids[L];
+    }
+    return AGAL_OK;
+}
+
 int agal_set_option(agal_ctx *ctx, int option, long long value)
 {
     CTX_OR_ARG(ctx);

@@ -257,6 +257,33 @@ class LbmMesh:
         self._call("download_gap_set", np_ptr(out, C.c_int), n)
         return out
 
+    def import_pool(self, st: dict):
+        """M_LoadToGPU of a whole pool (the dict state() returns) through agal_import_pool."""
+        keep = []
+
+        def arr(a, dt=np.int32):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_float if dt == np.float32 else C.c_int))
+
+        p = capi.Pool()
+        p.n_blocks = int(st["id_max"])
+        p.nbr, p.nbr_child = arr(st["nbr"]), arr(st["nbr_child"])
+        p.ref, p.level, p.onb, p.blk_mask = arr(st["ref"]), arr(st["level"]), arr(st["onb"]), arr(st["blk_mask"])
+        p.coords = arr(st["coords"], np.float32)
+        p.cell_mask = arr(st["cell_mask"])
+        p.n_ids = arr(np.array(st["n_ids"]))
+        sets = [np.asarray(x, dtype=np.int32) for x in st["id_sets"]]
+        p.id_sets = arr(np.concatenate(sets) if sets else np.zeros(0))
+        p.n_gaps = len(st["gap_set"])
+        p.gap_set = arr(st["gap_set"])
+        if st.get("f") is not None:
+            f = np.ascontiguousarray(st["f"], dtype=self.dtype)
+            keep.append(f)
+            p.populations = f.ctypes.data_as(C.c_void_p)
+        self._call("import_pool", C.byref(p))
+        self.refresh_sizes()
+
     def state(self) -> dict:
         """Everything the parity tests compare, restricted to active blocks where it matters."""
         s = self.refresh_sizes()

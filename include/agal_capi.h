@@ -311,6 +311,27 @@ int  agal_download_block_ints(agal_ctx *ctx, int which, int *dst, long long n_bl
 int  agal_download_coords(agal_ctx *ctx, float *dst, long long n_blocks, long long dst_stride);
 int  agal_download_id_set(agal_ctx *ctx, int L, int *dst, int n);
 int  agal_download_gap_set(agal_ctx *ctx, int *dst, int n);
+
+/* M_LoadToGPU (src/mesh/mesh_output.cu:4-42) for the whole pool at once, as a restart load
+ * (mesh_restart.cu:110-220) fills it: every block table for IDs [0, n_blocks), the cell masks,
+ * the per-level ID sets (concatenated, each ascending), the gap stack and, when `populations` is
+ * not NULL, the populations (physical order, the transfer type of agal_download_populations).
+ * Blocks from n_blocks on become inactive.  The per-level lists are rebuilt from ref / level and
+ * must reproduce the given ID sets, else AGAL_ERR_ARG and the context keeps the imported tables
+ * with the rebuilt lists. */
+typedef struct agal_pool {
+    long long  n_blocks;
+    const int *nbr, *nbr_child;            /* n_q_max rows of n_blocks (velocity order)       */
+    const int *ref, *level, *onb, *blk_mask;
+    const float *coords;                   /* n_dim rows of n_blocks                           */
+    const int *cell_mask;                  /* n_blocks * cells_per_block                       */
+    const int *n_ids;                      /* max_levels counts                                */
+    const int *id_sets;                    /* sum(n_ids) IDs, level after level                */
+    int        n_gaps;
+    const int *gap_set;
+    const void *populations;               /* n_q rows of n_blocks * cells_per_block, or NULL  */
+} agal_pool;
+int  agal_import_pool(agal_ctx *ctx, const agal_pool *pool);
 /* `which` values for agal_download_block_ints. */
 #define AGAL_BLK_NBR        0
 #define AGAL_BLK_NBR_CHILD  1

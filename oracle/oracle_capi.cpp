@@ -362,7 +362,7 @@ int oracle_download_gap_set(void *h, int *dst, int n)
 /* Replace the whole block-pool state (tables for blocks [0, n_blocks), per-level ID sets given
  * concatenated in level order, the gap stack and the ID maxima) -- used to time the restatement on
  * a mesh produced elsewhere. */
-int oracle_import_pool(void *h, long long n_blocks, const int *nbr, const int *nbr_child, const int *ref,
+int oracle_import_pool_arrays(void *h, long long n_blocks, const int *nbr, const int *nbr_child, const int *ref,
                        const int *level, const int *onb, const int *blk_mask, const float *coords,
                        const int *cell_mask, const int *n_ids, const int *id_max, const int *id_sets,
                        int n_gaps, const int *gap_set)
@@ -389,6 +389,66 @@ int oracle_import_pool(void *h, long long n_blocks, const int *nbr, const int *n
         for (int L = 0; L <= O.MAX_LEVELS; L++) O.id_max[L] = id_max[L];
         O.n_gaps = n_gaps;
         std::memcpy(O.gap_set.data(), gap_set, sizeof(int) * n_gaps);
+        return AGAL_OK;
+    });
+}
+
+/* agal_import_pool for the restatement: the tables of blocks [0, n_blocks), later IDs inactive,
+ * the ID sets as given (checked against ref / level), populations in physical order. */
+int oracle_import_pool(void *h, const agal_pool *pl)
+{
+    if (!pl) return AGAL_ERR_ARG;
+    return dispatch(h, [&](auto &O) {
+        using T = typename std::decay<decltype(O.F[0])>::type;
+        const long long nb = pl->n_blocks;
+        if (nb < O.n_coarsecblocks || nb > O.n_maxcblocks || pl->n_gaps < 0 || pl->n_gaps > O.n_maxcblocks) return AGAL_ERR_ARG;
+        const long long old = O.id_max[O.MAX_LEVELS];
+        for (long long b = nb; b < old; b++) {
+            O.ref[b] = REF_INACTIVE;
+            O.level[b] = 0;
+            O.onb[b] = 0;
+            O.blk_mask[b] = BLK_REGULAR;
+            for (int p = 0; p < O.Q_max; p++) {
+                O.nbr[(size_t)p * O.n_maxcblocks + b] = N_SKIPID;
+                O.nbr_child[(size_t)p * O.n_maxcblocks + b] = N_SKIPID;
+            }
+        }
+        for (int p = 0; p < O.Q_max; p++) {
+            std::memcpy(&O.nbr[(size_t)p * O.n_maxcblocks], pl->nbr + p * nb, sizeof(int) * nb);
+            std::memcpy(&O.nbr_child[(size_t)p * O.n_maxcblocks], pl->nbr_child + p * nb, sizeof(int) * nb);
+        }
+        std::memcpy(O.ref.data(), pl->ref, sizeof(int) * nb);
+        std::memcpy(O.level.data(), pl->level, sizeof(int) * nb);
+        std::memcpy(O.onb.data(), pl->onb, sizeof(int) * nb);
+        std::memcpy(O.blk_mask.data(), pl->blk_mask, sizeof(int) * nb);
+        for (int d = 0; d < O.DIM; d++)
+            std::memcpy(&O.X[(size_t)d * O.n_maxcblocks], pl->coords + d * nb, sizeof(float) * nb);
+        std::memcpy(O.cells_mask.data(), pl->cell_mask, sizeof(int) * nb * O.M_CBLOCK);
+        long long off = 0;
+        for (int L = 0; L < O.MAX_LEVELS; L++) {
+            std::vector<int> expect;
+            for (long long b = 0; b < nb; b++)
+                if (O.ref[b] != REF_INACTIVE && O.level[b] == L) expect.push_back((int)b);
+            if ((long long)expect.size() != pl->n_ids[L] || !std::equal(expect.begin(), expect.end(), pl->id_sets + off))
+                return AGAL_ERR_ARG;
+            O.n_ids[L] = pl->n_ids[L];
+            std::memcpy(&O.id_set[(size_t)L * O.n_maxcblocks], pl->id_sets + off, sizeof(int) * pl->n_ids[L]);
+            off += pl->n_ids[L];
+            O.id_max[L] = (L == 0) ? O.n_coarsecblocks : (pl->n_ids[L] > 0 ? pl->id_sets[off - 1] + 1 : 0);
+        }
+        int m = 0;
+        for (int L = 0; L < O.MAX_LEVELS; L++) m = std::max(m, O.id_max[L]);
+        O.id_max[O.MAX_LEVELS] = m;
+        O.n_gaps = pl->n_gaps;
+        if (pl->n_gaps > 0) std::memcpy(O.gap_set.data(), pl->gap_set, sizeof(int) * pl->n_gaps);
+        if (pl->populations) {
+            const T *in = static_cast<const T *>(pl->populations);
+            const long long nc = nb * O.M_CBLOCK;
+            for (int q = 0; q < O.Q; q++) {
+                T *F = O.Fp(O.V.pb[q]);
+                for (long long i = 0; i < nc; i++) F[i] = O.quant(in[q * nc + i], q);
+            }
+        }
         return AGAL_OK;
     });
 }

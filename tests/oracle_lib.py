@@ -134,14 +134,14 @@ class OracleMesh(LbmMesh):
             ids = st["id_sets"][L]
             id_max[L] = int(ids.max()) + 1 if ids.size else 0
         id_max[0] = s.n_coarsecblocks
-        f = lib.oracle_import_pool
+        f = lib.oracle_import_pool_arrays
         f.restype = C.c_int
         st_ = f(self.h, C.c_longlong(nb), arr(st["nbr"]), arr(st["nbr_child"]), arr(st["ref"]), arr(st["level"]),
                 arr(st["onb"]), arr(st["blk_mask"]), arr(st["coords"], np.float32), arr(st["cell_mask"]),
                 arr(np.array(st["n_ids"])), arr(np.array(id_max)), arr(np.concatenate(st["id_sets"])),
                 C.c_int(len(st["gap_set"])), arr(st["gap_set"]))
         if st_ != capi.AGAL_OK:
-            raise capi.AgalError(st_, "oracle_import_pool")
+            raise capi.AgalError(st_, "oracle_import_pool_arrays")
         self.refresh_sizes()
         self.set_populations(st["f"])
 
