@@ -1,8 +1,9 @@
 """Parity at the benchmark workloads' own sizes.
 
-* D3Q19 lid-driven cavity, 64^3 root, 3 levels, Re 1000, rebuild every 100 coarse steps: the HIP
-  library and the CPU restatement run the same driver through two solution-based rebuilds; the
-  full state is compared bit for bit and the block pool checked against brute force.
+* D3Q19 lid-driven cavity, 64^3 root, 3 levels by the |grad u| criterion, Re 1000, rebuild every
+  100 coarse steps: the HIP library and the CPU restatement run the same driver through two
+  solution-based rebuilds; the full state is compared bit for bit and the block pool checked
+  against brute force.
 * D2Q9 cavity 128^2, Re 100, FP32 and FP64.
 * Size-independent property at 256^3 (16.7 M cells): on a periodic grid a state that repeats every
   8 cells evolves as the 8^3 periodic state does, so every cell of the large GPU run must equal the
@@ -23,7 +24,8 @@ pytestmark = pytest.mark.gpu
 
 
 def ldc64_params(**kw):
-    return ldc_params(vs=capi.D3Q19, N=64, Re=1000.0, levels=3, P_REFINE=100, **kw)
+    """The benchmark's cavity: |grad u| criterion (2^-2 on the root level, halved per level)."""
+    return ldc_params(vs=capi.D3Q19, N=64, Re=1000.0, levels=3, P_REFINE=100, S_CRITERION=2, **kw)
 
 
 def test_ldc64_three_levels_two_rebuilds():
