@@ -24,6 +24,8 @@ AGAL_ERR_POOL_EXHAUSTED = 5
 AGAL_ERR_STATE = 6
 AGAL_ERR_OVERFLOW = 7
 AGAL_ERR_NOT_BUILT = 8
+# Size of the fixed staging area of agal_download_populations_noalloc (AGAL_NOALLOC_SCRATCH_BYTES).
+AGAL_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 D2Q9, D3Q19, D3Q27 = 9, 19, 27
 FP32, FP64, FP16, BF16 = 0, 1, 2, 3
@@ -210,6 +212,7 @@ PROTOTYPES = {
     "freeze_refined": (_I, [_P, _I]),
     "advance_loop": (_I, [_P, _I, _I, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "download_populations": (_I, [_P, _P, _LL, _LL]),
+    "download_populations_noalloc": (_I, [_P, _P, _LL, _LL]),
     "upload_populations": (_I, [_P, _P, _LL, _LL]),
     "download_macros": (_I, [_P, _P, _LL, _LL]),
     "download_cell_masks": (_I, [_P, C.POINTER(_I), _LL]),

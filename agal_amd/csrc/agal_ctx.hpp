@@ -55,6 +55,7 @@ struct LevelTiming {
 struct Ctx {
     agal_params P{};
     int device = 0;
+    void *scratch = nullptr;  // this GPU's copy of the allocation-free staging area (not owned)
     hipStream_t stream = nullptr;
     std::string last_error;
 

@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -674,6 +675,34 @@ int collect_timing(Ctx *c)
 #define CTX_OR_ARG(ctx) \
     if (!(ctx)) return AGAL_ERR_ARG;
 
+// ---- allocation-free download: staging area -----------------------------------------------------
+// A file-scope device array, one copy per GPU that the library runs on, held until the process
+// ends.  On the host `scratch` is only a stand-in symbol, never a device address: host code uses
+// the address hipGetSymbolAddress returns on each GPU, looked up once per GPU (with that GPU
+// current) when the first context on it is created, and kept in scratch_dev[].
+static __device__ __align__(256) unsigned char scratch[AGAL_NOALLOC_SCRATCH_BYTES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex scratch_mu;
+static void *scratch_dev[SCRATCH_MAX_DEVICES];
+
+// Device address of `scratch` on the current GPU (c->device), looked up on first use there.
+static int scratch_lookup(Ctx *c)
+{
+    if (c->device < 0 || c->device >= SCRATCH_MAX_DEVICES) {
+        c->last_error = "device index beyond the staging table";
+        return AGAL_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[c->device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return hip_fail(c, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        scratch_dev[c->device] = ptr;
+    }
+    c->scratch = scratch_dev[c->device];
+    return AGAL_OK;
+}
+
 extern "C" {
 
 void agal_params_default(agal_params *p)
@@ -788,6 +817,10 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) {
         st = hip_fail(c, e, "hipSetDevice");
+        free_ctx(c);
+        return st;
+    }
+    if ((st = scratch_lookup(c))) {
         free_ctx(c);
         return st;
     }
@@ -1251,15 +1284,17 @@ int agal_advance_loop(agal_ctx *ctx, int i0, int n, double *out_step_ms, double 
 }
 
 // ---- data exchange ---------------------------------------------------------------------------
-static int stage_pops(Ctx *c, void *host, long long n_blocks, long long stride, int dir)
+// `fixed_stage`: a device staging area of at least Q * n_blocks * cells transfer values to use
+// instead of a fresh allocation (the allocation-free download), else nullptr.
+static int stage_pops(Ctx *c, void *host, long long n_blocks, long long stride, int dir, void *fixed_stage = nullptr)
 {
     if (n_blocks < 0 || n_blocks > c->n_maxcblocks) return AGAL_ERR_ARG;
     if (n_blocks == 0) return AGAL_OK;
     const long long nc = n_blocks * c->CELLS;
     if (stride < nc) return AGAL_ERR_ARG;
-    void *stage = nullptr;
+    void *stage = fixed_stage;
     const size_t cs = c->csize;
-    AGAL_HIP(c, hipMalloc(&stage, (size_t)c->Q * nc * cs));
+    if (!stage) AGAL_HIP(c, hipMalloc(&stage, (size_t)c->Q * nc * cs));
     int cl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int L = 0; L < c->MAX_LEVELS; L++) cl[L] = c->cur[L];
     int4 lo = make_int4(cl[0], cl[1], cl[2], cl[3]), hi = make_int4(cl[4], cl[5], cl[6], cl[7]);
@@ -1303,7 +1338,7 @@ static int stage_pops(Ctx *c, void *host, long long n_blocks, long long stride, 
         return AGAL_OK;
     };
     rc = body();
-    (void)hipFree(stage);
+    if (!fixed_stage) (void)hipFree(stage);
     return rc;
 }
 
@@ -1312,6 +1347,25 @@ int agal_download_populations(agal_ctx *ctx, void *dst, long long n_blocks, long
     CTX_OR_ARG(ctx);
     if (!dst) return AGAL_ERR_ARG;
     return stage_pops(reinterpret_cast<Ctx *>(ctx), dst, n_blocks, dst_stride, 0);
+}
+
+int agal_download_populations_noalloc(agal_ctx *ctx, void *dst, long long n_blocks, long long dst_stride)
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!dst || n_blocks < 0 || n_blocks > c->n_maxcblocks) return AGAL_ERR_ARG;
+    const unsigned long long need = (unsigned long long)c->Q * (unsigned long long)n_blocks * c->CELLS * c->csize;
+    if (need > AGAL_NOALLOC_SCRATCH_BYTES) {
+        c->last_error = "agal_download_populations_noalloc: " + std::to_string(n_blocks) + " blocks need " +
+                        std::to_string(need) + " staging bytes, above the fixed " +
+                        std::to_string((unsigned long long)AGAL_NOALLOC_SCRATCH_BYTES);
+        return AGAL_ERR_ARG;
+    }
+    if (!c->scratch) {
+        c->last_error = "agal_download_populations_noalloc: no staging area on this device";
+        return AGAL_ERR_STATE;
+    }
+    return stage_pops(c, dst, n_blocks, dst_stride, 0, c->scratch);
 }
 
 int agal_upload_populations(agal_ctx *ctx, const void *src, long long n_blocks, long long src_stride)

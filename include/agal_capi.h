@@ -301,6 +301,15 @@ long long agal_load_stl(const char *path, float *out, long long capacity);
  * `dst_stride` / `src_stride` is the distance in elements between consecutive arrays. */
 int  agal_download_populations(agal_ctx *ctx, void *dst, long long n_blocks, long long dst_stride);
 int  agal_upload_populations(agal_ctx *ctx, const void *src, long long n_blocks, long long src_stride);
+/* No counterpart in the reference.  agal_download_populations for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the same output, staged through a fixed device array of AGAL_NOALLOC_SCRATCH_BYTES
+ * that the library holds on each GPU until the process ends, instead of a buffer allocated per
+ * call.  Calls no hipMalloc / hipFree.  Returns AGAL_ERR_ARG (message in agal_last_error) when
+ * Q * n_blocks * cells_per_block transfer values exceed the array.  All calls on one GPU share
+ * its array: calls that overlap in time on the same GPU race. */
+#define AGAL_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int  agal_download_populations_noalloc(agal_ctx *ctx, void *dst, long long n_blocks, long long dst_stride);
 int  agal_download_macros(agal_ctx *ctx, void *dst, long long n_blocks, long long dst_stride);
 int  agal_download_cell_masks(agal_ctx *ctx, int *dst, long long n_blocks);
 /* M_LoadToGPU (src/mesh/mesh_output.cu:4-42) for one array: overwrite cell masks / a block table

@@ -15,7 +15,8 @@ ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
 _NAMES = [k for k in capi.PROTOTYPES
           if not k.startswith("timing") and not k.startswith("predictor") and
-          k not in ("set_option", "upload_cell_masks", "upload_block_ints", "criterion_labels", "advance_loop_learned")]
+          k not in ("set_option", "upload_cell_masks", "upload_block_ints", "criterion_labels", "advance_loop_learned",
+                        "download_populations_noalloc")]
 _lib = None
 
 
