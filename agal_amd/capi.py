@@ -41,7 +41,7 @@ N_SKIPID = -999
 CELL_INTERIOR, CELL_INTERFACE, CELL_GHOST, CELL_SOLID, CELL_BOUNDARY = 0, 1, 2, -1, -6
 BLOCKMASK_REGULAR, BLOCKMASK_INTERFACE, BLOCKMASK_SOLIDB, BLOCKMASK_SOLIDA, BLOCKMASK_SOLID = 0, 1, -1, -2, -3
 BLK_NBR, BLK_NBR_CHILD, BLK_REF, BLK_LEVEL, BLK_ONB, BLK_MASK = 0, 1, 2, 3, 4, 5
-OPT_GRAPHS, OPT_MRT_MFMA = 1, 2
+OPT_GRAPHS, OPT_MRT_MFMA, OPT_EXC_CAP = 1, 2, 3
 
 
 class Params(C.Structure):
@@ -79,6 +79,13 @@ class Params(C.Structure):
         ("s5", C.c_double),
         ("s6", C.c_double),
         ("G_NEAR_WALL_DISTANCE", C.c_double),
+        ("G_BIN_DENSITY", C.c_int),
+        ("G_BIN_LEVELS", C.c_int),
+        ("G_BIN_SPEC", C.c_int),
+        ("G_BIN_MAKE_LINKS", C.c_int),
+        ("S_BC_TYPE", C.c_int),
+        ("S_FORCE_TYPE", C.c_int),
+        ("S_FORCE_ORDER", C.c_int),
     ]
 
 
@@ -175,6 +182,8 @@ def default_params(**kw) -> Params:
     p.ic_rho, p.ic_u, p.ic_v, p.ic_w = 1.0, 0.0, 0.0, 0.0
     p.s1, p.s2, p.s3, p.s4, p.s5, p.s6 = 1.19, 1.4, 1.2, 1.2, 1.2, 1.98
     p.G_NEAR_WALL_DISTANCE = 0.05
+    p.G_BIN_DENSITY, p.G_BIN_LEVELS, p.G_BIN_SPEC, p.G_BIN_MAKE_LINKS = 1, 1, 1, 0
+    p.S_BC_TYPE, p.S_FORCE_TYPE, p.S_FORCE_ORDER = 0, 0, 0
     for k, v in kw.items():
         if not hasattr(p, k):
             raise KeyError(k)
@@ -212,6 +221,8 @@ PROTOTYPES = {
     "freeze_refined": (_I, [_P, _I]),
     "advance_loop": (_I, [_P, _I, _I, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "download_populations": (_I, [_P, _P, _LL, _LL]),
+    "download_links": (_I, [_P, _P, _LL, _P, _LL, _P]),
+    "download_link_bins": (_I, [_P, _P, _LL, _P, _LL, _P]),
     "download_populations_noalloc": (_I, [_P, _P, _LL, _LL]),
     "upload_populations": (_I, [_P, _P, _LL, _LL]),
     "download_macros": (_I, [_P, _P, _LL, _LL]),

@@ -16,6 +16,9 @@ constexpr int REF_UNREFINED = 0, REF_UNREFINED_VIO = 9, REF_REFINED = 1, REF_REF
               REF_MARK_COARSEN = 5, REF_NEW = 6, REF_REMOVE = 7, REF_INACTIVE = 8, REF_INDET_E = 11,
               REF_INDET_O = 12;
 constexpr int CELL_INTERIOR = 0, CELL_INTERFACE = 1, CELL_GHOST = 2, CELL_SOLID = -1, CELL_BOUNDARY = -6;
+// Block flag bit (beside the fused step's simple / interior kind in bits 0-1): the block holds
+// boundary cells with face-cell links and takes the linked sub-step instead of the fused one.
+constexpr int LINK_FLAG = 8;
 constexpr int BLK_REGULAR = 0, BLK_INTERFACE = 1, BLK_SOLID = -3, BLK_SOLIDA = -2, BLK_SOLIDB = -1;
 
 // D3Q27 order; D3Q19 is its first 19 entries (inc/util.h:95).

@@ -98,7 +98,8 @@ struct Ctx {
     int *hscal = nullptr;     // pinned host mirror
     // Exceptions of the stream guard: (initiator cell, partner cell, direction) + value, one list
     // of exc_cap entries per level.
-    long long exc_cap = 0;
+    long long exc_cap = 0, exc_cap_alloc = 0;
+    bool exc_overflowed = false;  // an overflow was reported: steps are refused
     int *exc_idx = nullptr;   // 3 ints per entry
     void *exc_val = nullptr;  // one value per entry
     int *exc_count = nullptr; // per level (and the extra slot MAX_LEVELS): [2L] count, [2L+1] overflow flag
@@ -110,8 +111,21 @@ struct Ctx {
     long long geo_n_tri = 0;
     int geo_G = 0;
     double geo_y0 = 0, geo_z0 = 0, geo_inv_dy = 0, geo_inv_dz = 0;
-    double *geo_part = nullptr;   // 3 doubles per finest-level block: force partials
-    double *geo_force = nullptr;  // 3 doubles, result of the force reduction
+
+    // Face-cell links (G_BIN_MAKE_LINKS 1): bins of the link search, ranks of the blocks holding
+    // boundary cells, the fractions [rank*64 + cell + q*lk_ncb] (27 directions), the linked blocks of
+    // each level (lk_ids + lk_off[L], lk_n[L] of them) and the per-block momentum-exchange sums
+    // (3 x n_maxcblocks values of the compute type).
+    bool links_on = false;
+    int *lk_bcount = nullptr, *lk_onb_solid = nullptr, *lk_ids = nullptr, *lk_counter = nullptr;
+    float *lk_X = nullptr;
+    void *lk_blkF = nullptr;
+    long long lk_n_solidb = 0, lk_ncb = 0, lk_corrections = 0;
+    long long lk_off[8] = {};
+    int lk_n[8] = {};
+    int *lb_start = nullptr, *lb_faces = nullptr;
+    int lb_G = 0, lb_level = 0;
+    long long lb_n_bins = 0, lb_n_ent = 0, lb_n_kept = 0;
 
     Predictor pred;
 
@@ -207,6 +221,11 @@ int near_wall_passes(double near_wall_distance, int L, double dx_L);
 int mark_near_wall(Ctx *c, int L);
 int check_masks_level(Ctx *c, int L);
 int compute_forces(Ctx *c, double out[3]);
+void free_links(Ctx *c);
+int links_reset_counts(Ctx *c);
+int compute_links(Ctx *c);
+int links_rebuild_lists(Ctx *c);
+int launch_mea_post(Ctx *c);
 long long icosphere(double cx, double cy, double cz, double r, int subdiv, double noise, unsigned long long seed,
                     float *out, long long cap);
 long long load_stl_ascii(const char *path, float *out, long long cap);

@@ -735,7 +735,8 @@ int rebuild_level_lists(Ctx *c, long long range_hint)
                            c->simple, n, 2);
     }
     AGAL_HIP(c, hipGetLastError());
-    return AGAL_OK;
+    // The linked blocks of the new level lists (their ranks and fractions stay with the block IDs).
+    return links_rebuild_lists(c);
 }
 
 // Mesh::M_RefineAndCoarsenBlocks (src/mesh/mesh_amr.cu:2161-2744).

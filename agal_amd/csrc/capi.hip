@@ -175,6 +175,7 @@ void free_ctx(Ctx *c)
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_geometry(c);
+    free_links(c);
     free_predictor(c);
     if (c->hscal) (void)hipHostFree(c->hscal);
     for (auto &g : c->graphs)
@@ -346,7 +347,9 @@ int advance_step(Ctx *c, int L0)
     int st = advance_rec(c, L0);
     c->defer_coarse = false;
     const int st2 = launch_flush_deferred(c);
-    return st ? st : st2;
+    if (st || st2) return st ? st : st2;
+    // The after-stream half of the momentum exchange, on the finished finest level.
+    return launch_mea_post(c);
 }
 
 unsigned cur_mask(const Ctx *c)
@@ -390,6 +393,9 @@ int steps_once(Ctx *c, int L0, int k)
             break;
         }
     if (slot->exec) {
+        // The evicted graph may still be executing (it was launched asynchronously, possibly by
+        // the previous coarse step): wait for the stream before destroying its executable.
+        AGAL_HIP(c, hipStreamSynchronize(c->stream));
         (void)hipGraphExecDestroy(slot->exec);
         slot->exec = nullptr;
     }
@@ -625,10 +631,14 @@ int refine_near_wall_impl(Ctx *c)
         c->ic_done = true;
     }
     if (geom) {
+        if ((st = links_reset_counts(c))) return st;
         for (int L = c->N_LEVEL_START; L < c->MAX_LEVELS_WALL; L++)
             if ((st = check_masks_level(c, L))) return st;
         // Solid cells and masks changed: participation and fast-path flags of the blocks.
         if ((st = rebuild_level_lists(c, 0))) return st;
+        // Ranks of the boundary blocks and, with G_BIN_MAKE_LINKS, their link fractions
+        // (M_Geometry_Voxelize_S3, M_LinkLengthComputation; mesh_advance.cu:99-111).
+        if (c->P.G_BIN_MAKE_LINKS == 1 && (st = compute_links(c))) return st;
     }
     if (ML > 1) {
         const int n = c->id_max[ML];
@@ -641,17 +651,31 @@ int refine_near_wall_impl(Ctx *c)
     return status;
 }
 
+// An overflowed exception list dropped links: the populations are wrong from then on, so the
+// context refuses further steps (AGAL_ERR_STATE) once the overflow has been reported.
 int check_exceptions(Ctx *c)
 {
+    if (c->exc_overflowed) return AGAL_ERR_OVERFLOW;
     std::vector<int> v(2 * (c->MAX_LEVELS + 1), 0);
     AGAL_HIP(c, hipMemcpyAsync(v.data(), c->exc_count, sizeof(int) * v.size(), hipMemcpyDeviceToHost, c->stream));
     AGAL_HIP(c, hipStreamSynchronize(c->stream));
     for (int L = 0; L <= c->MAX_LEVELS; L++)
         if (v[2 * L + 1]) {
-            c->last_error = "stream-guard exception list of level " + std::to_string(L) + " overflowed";
+            c->last_error = "stream-guard exception list of level " + std::to_string(L) + " overflowed (" +
+                            std::to_string(c->exc_cap) + " entries): populations are invalid, the context refuses "
+                            "further steps";
+            c->exc_overflowed = true;
             return AGAL_ERR_OVERFLOW;
         }
     return AGAL_OK;
+}
+
+// Steps refuse to run on a context whose exception list overflowed.
+static int usable(Ctx *c)
+{
+    if (!c->exc_overflowed) return AGAL_OK;
+    c->last_error = "a stream-guard exception list overflowed earlier: the populations are invalid";
+    return AGAL_ERR_STATE;
 }
 
 int collect_timing(Ctx *c)
@@ -734,6 +758,13 @@ void agal_params_default(agal_params *p)
     p->s5 = 1.2;
     p->s6 = 1.98;
     p->G_NEAR_WALL_DISTANCE = 0.05;
+    p->G_BIN_DENSITY = 1;
+    p->G_BIN_LEVELS = 1;
+    p->G_BIN_SPEC = 1;
+    p->G_BIN_MAKE_LINKS = 0;
+    p->S_BC_TYPE = 0;
+    p->S_FORCE_TYPE = 0;
+    p->S_FORCE_ORDER = 0;
 }
 
 const char *agal_status_string(int s)
@@ -835,7 +866,7 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     }
     const long long nmax = c->n_maxcblocks, ncm = c->n_maxcells;
     const size_t ts = c->tsize, cs = c->csize;
-    c->exc_cap = std::min<long long>(ncm, 1LL << 22);
+    c->exc_cap = c->exc_cap_alloc = std::min<long long>(ncm, 1LL << 22);
 #define ALLOC(ptr, bytes)                                        \
     if ((st = dalloc(c, &(ptr), (bytes))) != AGAL_OK) {          \
         free_ctx(c);                                             \
@@ -964,6 +995,7 @@ int agal_advance(agal_ctx *ctx, int L0)
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (L0 < 0 || L0 >= c->MAX_LEVELS) return AGAL_ERR_ARG;
     if (!c->ic_done) return AGAL_ERR_STATE;
+    if (int st = usable(c)) return st;
     return step_once(c, L0);
 }
 
@@ -972,6 +1004,7 @@ int agal_collide_stream(agal_ctx *ctx, int L)
     CTX_OR_ARG(ctx);
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (L < 0 || L >= c->MAX_LEVELS) return AGAL_ERR_ARG;
+    if (int st = usable(c)) return st;
     return launch_collide_stream(c, L);
 }
 
@@ -1063,6 +1096,52 @@ int agal_forces(agal_ctx *ctx, double out[3])
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c->ic_done) return AGAL_ERR_STATE;
     return compute_forces(c, out);
+}
+
+int agal_download_links(agal_ctx *ctx, int *onb_solid, long long n_blocks, float *X_b, long long n_values,
+                        long long info[3])
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (n_blocks < 0 || n_blocks > c->n_maxcblocks || n_values < 0) return AGAL_ERR_ARG;
+    if (info) {
+        info[0] = c->lk_n_solidb;
+        info[1] = c->lk_corrections;
+        info[2] = c->links_on ? 1 : 0;
+    }
+    if (onb_solid && n_blocks > 0) {
+        if (c->lk_onb_solid)
+            AGAL_HIP(c, hipMemcpy(onb_solid, c->lk_onb_solid, sizeof(int) * n_blocks, hipMemcpyDeviceToHost));
+        else
+            for (long long k = 0; k < n_blocks; k++) onb_solid[k] = -1;
+    }
+    if (X_b && n_values > 0) {
+        if (!c->lk_X || n_values > 27 * c->lk_ncb) return AGAL_ERR_ARG;
+        AGAL_HIP(c, hipMemcpy(X_b, c->lk_X, sizeof(float) * n_values, hipMemcpyDeviceToHost));
+    }
+    return AGAL_OK;
+}
+
+int agal_download_link_bins(agal_ctx *ctx, int *bin_start, long long n_start, int *faces, long long n_faces,
+                            long long info[4])
+{
+    CTX_OR_ARG(ctx);
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (info) {
+        info[0] = c->lb_n_bins;
+        info[1] = c->lb_n_ent;
+        info[2] = c->lb_G;
+        info[3] = c->lb_n_kept;
+    }
+    if (bin_start && n_start > 0) {
+        if (!c->lb_start || n_start > c->lb_n_bins + 1) return AGAL_ERR_ARG;
+        AGAL_HIP(c, hipMemcpy(bin_start, c->lb_start, sizeof(int) * n_start, hipMemcpyDeviceToHost));
+    }
+    if (faces && n_faces > 0) {
+        if (!c->lb_faces || n_faces > c->lb_n_ent) return AGAL_ERR_ARG;
+        AGAL_HIP(c, hipMemcpy(faces, c->lb_faces, sizeof(int) * n_faces, hipMemcpyDeviceToHost));
+    }
+    return AGAL_OK;
 }
 
 long long agal_icosphere(double cx, double cy, double cz, double radius, int subdivisions, double noise,
@@ -1183,6 +1262,7 @@ int agal_advance_loop_learned(agal_ctx *ctx, int i0, int n, const agal_learn_opt
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!opts || !stats || n < 0 || opts->batch <= 0 || opts->infer_every <= 0 || opts->adam_steps < 0) return AGAL_ERR_ARG;
     if (!c->ic_done) return AGAL_ERR_STATE;
+    if (int st = usable(c)) return st;
     return advance_loop_learned_impl(c, i0, n, opts, stats);
 }
 
@@ -1230,9 +1310,19 @@ int agal_advance_loop(agal_ctx *ctx, int i0, int n, double *out_step_ms, double 
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c->ic_done) return AGAL_ERR_STATE;
     if (n < 0) return AGAL_ERR_ARG;
-    hipEvent_t e0, e1;
-    AGAL_HIP(c, hipEventCreate(&e0));
-    AGAL_HIP(c, hipEventCreate(&e1));
+    if (int st = usable(c)) return st;
+    // The two timing events live until the function returns, on every path.
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    AGAL_HIP(c, hipEventCreate(&ev.e[0]));
+    AGAL_HIP(c, hipEventCreate(&ev.e[1]));
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     double ts = 0, tr = 0;
     int status = AGAL_OK, st;
     const int iter_mult = 1 << c->N_LEVEL_START;
@@ -1275,8 +1365,6 @@ int agal_advance_loop(agal_ctx *ctx, int i0, int n, double *out_step_ms, double 
         AGAL_HIP(c, hipEventElapsedTime(&ms, e0, e1));
         ts += ms;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if ((st = check_exceptions(c))) return st;
     if (out_step_ms) *out_step_ms = ts;
     if (out_rebuild_ms) *out_rebuild_ms = tr;
@@ -1581,6 +1669,13 @@ int agal_set_option(agal_ctx *ctx, int option, long long value)
     if (option == AGAL_OPT_MRT_MFMA) {
         c->mrt_mfma = value != 0;
         c->mesh_epoch++;  // recorded step graphs hold the other kernel
+        return AGAL_OK;
+    }
+    if (option == AGAL_OPT_EXC_CAP) {
+        // Entries per exception list, at most the allocated capacity.
+        if (value < 1 || value > c->exc_cap_alloc) return AGAL_ERR_ARG;
+        c->exc_cap = value;
+        c->mesh_epoch++;  // recorded step graphs hold the old bound
         return AGAL_OK;
     }
     return AGAL_ERR_ARG;

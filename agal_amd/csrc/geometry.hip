@@ -180,7 +180,8 @@ __global__ void k_mark_finalize(int idm, int *ref, int *tmp)
 // Only SOLID masks are read and only non-solid cells are written, so one pass is race-free.
 __global__ __launch_bounds__(GWG) void k_check_masks(const int *__restrict__ ids, int n, const int *__restrict__ nbr_slot,
                                                      const int *__restrict__ nbr_child, const int *__restrict__ blk_mask,
-                                                     int *__restrict__ cells_mask, int *__restrict__ onb)
+                                                     int *__restrict__ cells_mask, int *__restrict__ onb,
+                                                     int *__restrict__ bcount)
 {
     const int t = threadIdx.x & 63;
     const int bi = blockIdx.x * (GWG / 64) + (threadIdx.x >> 6);
@@ -203,78 +204,18 @@ __global__ __launch_bounds__(GWG) void k_check_masks(const int *__restrict__ ids
     }
     const bool bnd = near && cells_mask[cell] != CELL_SOLID;
     if (bnd) cells_mask[cell] = CELL_BOUNDARY;
-    if (__any(bnd) && t == 0) onb[b] = 1;
+    // The block's count of boundary cells (tmp_1, mesh_voxelizer_masks.cu:151-175).
+    const int nb = __popcll(__ballot(bnd));
+    if (nb > 0 && t == 0) {
+        onb[b] = 1;
+        if (bcount) bcount[b] = nb;
+    }
 }
 
 __global__ void k_fill_int(int *a, long long n, int v)
 {
     const long long i = (long long)blockIdx.x * GWG + threadIdx.x;
     if (i < n) a[i] = v;
-}
-
-// Momentum exchange on the finest level, first order (Cu_ComputeForcesMEA, order 1): every link
-// from a BOUNDARY cell x to a solid cell x + c_p carries f*_p c_p into the body when the post-
-// collision value is sent and the same value again when it comes back, 2 f*_p c_p in all (the
-// reference's pre- and post-stream calls on the last fine sub-step).  Called after the step, when
-// the bounced value sits in the cell's population of direction pb(p).  One wavefront per block;
-// FP64 per-block partials.
-template <typename T, typename S, int Q>
-__global__ __launch_bounds__(GWG) void k_forces_mea(const int *__restrict__ ids, int n, const S *__restrict__ F,
-                                                    long long ncm, const int *__restrict__ nbr_slot,
-                                                    const int *__restrict__ cells_mask, double *__restrict__ part)
-{
-    using VS = VelSet<Q>;
-    const int t = threadIdx.x & 63;
-    const int bi = blockIdx.x * (GWG / 64) + (threadIdx.x >> 6);
-    if (bi >= n) return;
-    const int b = ids[bi];
-    const int my_nb = t < 27 ? nbr_slot[(long long)b * 27 + t] : 0;
-    const long long cell = (long long)b * 64 + t;
-    const bool fluid = cells_mask[cell] == CELL_BOUNDARY;
-    const int I = t & 3, J = (t >> 2) & 3, K = t >> 4;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-#pragma unroll
-    for (int p = 1; p < Q; p++) {
-        const int Ip = I + VS::cx(p), Jp = J + VS::cy(p), Kp = K + VS::cz(p);
-        const int slot = (Ip < 0 ? 0 : (Ip >= 4 ? 2 : 1)) + 3 * (Jp < 0 ? 0 : (Jp >= 4 ? 2 : 1)) +
-                         9 * (Kp < 0 ? 0 : (Kp >= 4 ? 2 : 1));
-        const int nbk = __shfl(my_nb, slot, 64);
-        if (!fluid || nbk < 0) continue;
-        const long long tgt = (long long)nbk * 64 + ((Ip & 3) + 4 * (Jp & 3) + 16 * (Kp & 3));
-        if (cells_mask[tgt] != CELL_SOLID) continue;
-        const double f = (double)Store<T, S>::dec(F[(long long)VS::pb(p) * ncm + cell], (T)VS::w(p));
-        fx += 2.0 * f * (double)VS::cx(p);
-        fy += 2.0 * f * (double)VS::cy(p);
-        fz += 2.0 * f * (double)VS::cz(p);
-    }
-    for (int d = 32; d > 0; d >>= 1) {
-        fx += __shfl_down(fx, d, 64);
-        fy += __shfl_down(fy, d, 64);
-        fz += __shfl_down(fz, d, 64);
-    }
-    if (t == 0) {
-        part[3LL * bi + 0] = fx;
-        part[3LL * bi + 1] = fy;
-        part[3LL * bi + 2] = fz;
-    }
-}
-
-// Sum of the per-block partials in block order (deterministic), one workgroup.
-__global__ void k_sum_partials(const double *part, int n, double *out)
-{
-    __shared__ double s[3][GWG];
-    double a[3] = {0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < n; i += GWG)
-        for (int d = 0; d < 3; d++) a[d] += part[3LL * i + d];
-    for (int d = 0; d < 3; d++) s[d][threadIdx.x] = a[d];
-    __syncthreads();
-    for (int w = GWG / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int d = 0; d < 3; d++) s[d][threadIdx.x] += s[d][threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int d = 0; d < 3; d++) out[d] = s[d][0];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -287,11 +228,8 @@ void free_geometry(Ctx *c)
     if (c->geo_tri) (void)hipFree(c->geo_tri);
     if (c->geo_bin_start) (void)hipFree(c->geo_bin_start);
     if (c->geo_bin_tri) (void)hipFree(c->geo_bin_tri);
-    if (c->geo_part) (void)hipFree(c->geo_part);
-    if (c->geo_force) (void)hipFree(c->geo_force);
     c->geo_tri = nullptr;
     c->geo_bin_start = c->geo_bin_tri = nullptr;
-    c->geo_part = c->geo_force = nullptr;
     c->geo_n_tri = 0;
 }
 
@@ -350,10 +288,6 @@ int set_geometry(Ctx *c, const float *tri, long long n_tri)
         list.insert(list.end(), bins[i].begin(), bins[i].end());
     }
     start[bins.size()] = (int)list.size();
-    if (!c->geo_part) {
-        AGAL_HIP(c, hipMalloc(&c->geo_part, sizeof(double) * 3 * c->n_maxcblocks));
-        AGAL_HIP(c, hipMalloc(&c->geo_force, sizeof(double) * 3));
-    }
     AGAL_HIP(c, hipMalloc(&c->geo_tri, sizeof(float) * 9 * n_tri));
     AGAL_HIP(c, hipMalloc(&c->geo_bin_start, sizeof(int) * start.size()));
     AGAL_HIP(c, hipMalloc(&c->geo_bin_tri, sizeof(int) * std::max<size_t>(1, list.size())));
@@ -418,46 +352,9 @@ int check_masks_level(Ctx *c, int L)
     const int n = c->n_ids[L];
     if (n <= 0) return AGAL_OK;
     hipLaunchKernelGGL(k_check_masks, dim3((unsigned)((n + 3) / 4)), dim3(GWG), 0, c->stream, c->id_set_L(L), n,
-                       c->nbr_slot, c->nbr_child, c->blk_mask, c->cells_mask, c->onb);
+                       c->nbr_slot, c->nbr_child, c->blk_mask, c->cells_mask, c->onb, c->lk_bcount);
     AGAL_HIP(c, hipGetLastError());
     return AGAL_OK;
-}
-
-template <typename T, typename S, int Q>
-static int forces_t(Ctx *c, double out[3])
-{
-    const int L = c->MAX_LEVELS - 1;
-    const int n = c->n_ids[L];
-    out[0] = out[1] = out[2] = 0.0;
-    if (n <= 0) return AGAL_OK;
-    hipLaunchKernelGGL((k_forces_mea<T, S, Q>), dim3((unsigned)((n + 3) / 4)), dim3(GWG), 0, c->stream, c->id_set_L(L),
-                       n, static_cast<const S *>(c->F[c->cur[L]]), c->n_maxcells, c->nbr_slot, c->cells_mask,
-                       c->geo_part);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(GWG), 0, c->stream, c->geo_part, n, c->geo_force);
-    AGAL_HIP(c, hipGetLastError());
-    double h[3];
-    AGAL_HIP(c, hipMemcpyAsync(h, c->geo_force, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
-    AGAL_HIP(c, hipStreamSynchronize(c->stream));
-    // Cell volume dx_L^3 of the finest level times the factor 1/dx of the finest level
-    // (solver_lbm_init.cu:19-27, solver_lbm_compute.cu:171-173).
-    const double dx = c->dxf_vec[L];
-    for (int d = 0; d < 3; d++) out[d] = h[d] * (dx * dx * dx) / dx;
-    return AGAL_OK;
-}
-
-int compute_forces(Ctx *c, double out[3])
-{
-    if (c->DIM != 3) return AGAL_ERR_NOT_BUILT;
-    if (!c->geo_part) {
-        c->last_error = "no geometry set";
-        return AGAL_ERR_STATE;
-    }
-    switch (c->precision) {
-        case AGAL_FP64: return c->Q == 19 ? forces_t<double, double, 19>(c, out) : forces_t<double, double, 27>(c, out);
-        case AGAL_FP16: return c->Q == 19 ? forces_t<float, _Float16, 19>(c, out) : forces_t<float, _Float16, 27>(c, out);
-        case AGAL_BF16: return c->Q == 19 ? forces_t<float, bf16s, 19>(c, out) : forces_t<float, bf16s, 27>(c, out);
-        default: return c->Q == 19 ? forces_t<float, float, 19>(c, out) : forces_t<float, float, 27>(c, out);
-    }
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -639,6 +639,7 @@ __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, int w
     // the block index out of divergent control flow, so it stays a scalar.
     const int my_nb = a.nbr_slot[(long long)b * 27 + (t < 27 ? t : 26)];
     const int flags_u = __builtin_amdgcn_readfirstlane(a.simple[b]);
+    if (flags_u & LINK_FLAG) return;  // a linked block: its sub-step is k_link_step's
     const int simple_u = flags_u & 3;
     const int part_u = __builtin_amdgcn_readfirstlane(a.part[b]);
     const int onb_u = __builtin_amdgcn_readfirstlane(a.onb[b]);
@@ -1393,6 +1394,304 @@ __global__ __launch_bounds__(WG) void k_collide_stream(MultiStep<T, S> m, StepEx
 }
 
 // ---------------------------------------------------------------------------------------------
+// Linked blocks (3D, face-cell links of a geometry): the sub-step of blocks holding boundary cells
+// with cut-link fractions.  Cu_Collide, then Cu_ImposeBC with the interpolated bounce-back of
+// Bouzidi, Firdaouss and Lallemand (2001) (solver_lbm_kernels.cu:796-875), then the push of the
+// fused step (push_link), one wavefront per block; the fused step skips these blocks (LINK_FLAG).
+// Both read the level's input buffer and write disjoint slots of its output buffer, and record
+// guard exceptions into the same list, so they can run in either order.
+//
+// For a link of fraction q along c_p from boundary cell x:
+//   q < 1/2:  f_p <- 2q f_p + (1 - 2q) f*_p(x - c_p);   q >= 1/2:  f_p <- f_p / 2q + (2q - 1)/2q f_pb,
+// per half-set pair in the order hook(p), link(p), hook(pb), link(pb), so the pb link's second
+// form sees the new f_p.  f*_p(x - c_p) is the post-collision value behind the cell: a lane of the
+// same wavefront, or, across a block face, the collision of that cell recomputed here from the
+// same inputs with the same code (so the same bits).  A missing block behind the cell leaves f_p
+// unchanged (the reference reads out of bounds there).
+//
+// With `mea` the block's momentum-exchange sums before the boundary rules (Cu_ComputeForcesMEA
+// post_step 0, solver_lbm_kernels_forces.cu:322-540) are formed here from the post-collision
+// values and stored (assigned) per block; k_mea_post adds the after-stream part.
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+struct LinkArgs {
+    const int *ids;       // the level's linked blocks
+    int n;
+    const int *onb_solid;  // block -> rank in the link arrays
+    const float *X_b;      // fractions [rank*64 + cell + q*ncb], 27 directions
+    long long ncb;
+    int bc_type;          // S_BC_TYPE: 1 = interpolated bounce-back
+    int mea;              // form the pre-boundary momentum-exchange sums (post_step 0)
+    int order;            // S_FORCE_ORDER
+    T *blkF;              // 3 x nmax block sums
+    long long nmax;
+    T dv;                 // cell volume dx^3 of the level
+};
+
+// Post-collision value f*_q of cell `tc` of block nbk (which exists), from the launch's input
+// buffer: collide_cell on the same loads and flags as that cell's own wavefront.
+template <typename T, typename S, int Q>
+__device__ __forceinline__ T post_collision_at(const StepArgs<T, S> &a, int nbk, int tc, int q)
+{
+    using VS = VelSet<Q>;
+    using ST = Store<T, S>;
+    const long long cell = (long long)nbk * 64 + tc;
+    T f[Q], fs[Q];
+#pragma unroll
+    for (int p = 0; p < Q; p++) f[p] = ST::dec(a.A[(long long)p * a.ncm + cell], (T)VS::w(p));
+    const bool inert = a.part[nbk] == 0 || a.cells_mask[cell] == CELL_SOLID;
+    T rho, u, v, w;
+    moments<T, Q>(f, rho, u, v, w);
+    collide_cell<T, S, Q>(a, f, fs, rho, u, v, w, inert);
+    T r = (T)0;
+#pragma unroll
+    for (int p = 0; p < Q; p++)
+        if (p == q) r = fs[p];
+    return r;
+}
+
+// Reduction of one value per lane over the wavefront in the reference's shared-memory tree order
+// (s = 32, 16, ..., 1: v[t] = v[t] + v[t + s]); lane 0 holds the result.
+template <typename T>
+__device__ __forceinline__ T tree_sum64(T v)
+{
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) v = v + __shfl_down(v, s, 64);
+    return v;
+}
+
+// One momentum-exchange contribution (side s of a half-set pair) added to acc:
+// order 1: f c_s; order 2: ((1/2 + q) f + (1/2 - q) f_m) c_s, the literal 1/2 a double, so the term
+// and the running sum are formed in double and rounded by the reference's +=.
+template <typename T>
+__device__ __forceinline__ void mea_add(T *acc, int order, T f, T fm, float dist, int cx, int cy, int cz)
+{
+    const int c[3] = {cx, cy, cz};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (order == 1) {
+            acc[d] += f * (T)c[d];
+        } else {
+            const double qd = (double)(T)dist;
+            acc[d] = (T)((double)acc[d] + ((0.5 + qd) * (double)f + (0.5 - qd) * (double)fm) * (double)c[d]);
+        }
+    }
+}
+
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(WG) void k_link_step(StepArgs<T, S> a, LinkArgs<T> la)
+{
+    using VS = VelSet<Q>;
+    using ST = Store<T, S>;
+    const int t = threadIdx.x & 63;
+    const int bi = blockIdx.x * (WG / 64) + (int)(threadIdx.x >> 6);
+    if (bi >= la.n) return;  // whole wavefronts
+    const int b = __builtin_amdgcn_readfirstlane(la.ids[bi]);
+    const unsigned cell = (unsigned)b * 64u + (unsigned)t;
+    const int my_nb = a.nbr_slot[(long long)b * 27 + (t < 27 ? t : 26)];
+    const int part_u = __builtin_amdgcn_readfirstlane(a.part[b]);
+    const int onb_u = __builtin_amdgcn_readfirstlane(a.onb[b]);
+    const int rank = __builtin_amdgcn_readfirstlane(la.onb_solid[b]);
+    T f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = ST::dec(a.A[(long long)q * a.ncm + cell], (T)VS::w(q));
+    const int mask = a.cells_mask[cell];
+    const bool inert = part_u == 0 || mask == CELL_SOLID;
+    T fs[Q];
+    T rho, u, v, w;
+    moments<T, Q>(f, rho, u, v, w);
+    collide_cell<T, S, Q>(a, f, fs, rho, u, v, w, inert);
+
+    int nbs[27];
+#pragma unroll
+    for (int s = 0; s < 27; s++) nbs[s] = __builtin_amdgcn_readlane(my_nb, s);
+    const int I = t & 3, J = (t >> 2) & 3, K = t >> 4;
+    // Block of the cell x + c_q (slot table) and of x - c_q.
+    auto slot_of = [&](int dx, int dy, int dz) {
+        const int Ip = I + dx, Jp = J + dy, Kp = K + dz;
+        return (Ip < 0 ? 0 : (Ip >= 4 ? 2 : 1)) + 3 * (Jp < 0 ? 0 : (Jp >= 4 ? 2 : 1)) + 9 * (Kp < 0 ? 0 : (Kp >= 4 ? 2 : 1));
+    };
+    auto cell_of = [&](int dx, int dy, int dz) { return ((I + dx) & 3) + 4 * ((J + dy) & 3) + 16 * ((K + dz) & 3); };
+    auto block_at = [&](int sl) {
+        int r = nbs[0];
+#pragma unroll
+        for (int k = 1; k < 27; k++)
+            if (sl == k) r = nbs[k];
+        return r;
+    };
+
+    const bool linked = rank >= 0 && mask == CELL_BOUNDARY;
+    const long long cb = (long long)rank * 64 + t;
+    float xq[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) xq[q] = (linked && q > 0) ? la.X_b[cb + (long long)q * la.ncb] : -1.0f;
+    const bool curved = la.bc_type == 1 && linked;
+    const bool mea2 = la.mea && la.order != 1 && linked;
+
+    // f*_q(x - c_q): within the block from lane (I,J,K) - c_q (every lane takes part), else recomputed.
+    T fm[Q];
+    unsigned long long need = 0;  // bit q: needed and across a block face
+    fm[0] = (T)0;
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+        const int cx = VS::cx(q), cy = VS::cy(q), cz = VS::cz(q);
+        const int src = cell_of(-cx, -cy, -cz);
+        if constexpr (sizeof(T) == 8) {
+            const long long bits = __builtin_bit_cast(long long, fs[q]);
+            const int lo = __shfl((int)(bits & 0xffffffffLL), src, 64), hi = __shfl((int)(bits >> 32), src, 64);
+            fm[q] = __builtin_bit_cast(T, ((long long)hi << 32) | (unsigned)lo);
+        } else {
+            fm[q] = __shfl(fs[q], src, 64);
+        }
+        const bool inside = (unsigned)(I - cx) < 4u && (unsigned)(J - cy) < 4u && (unsigned)(K - cz) < 4u;
+        const bool want = (curved && xq[q] > 0 && xq[q] < 0.5f) || (mea2 && xq[q] > 0);
+        if (want && !inside) need |= 1ull << q;
+    }
+    unsigned long long missing = 0;  // bit q: the block behind is missing
+    while (need) {
+        const int q = __builtin_ctzll(need);
+        need &= need - 1;
+        int cx = 0, cy = 0, cz = 0;
+#pragma unroll
+        for (int p = 1; p < Q; p++)
+            if (p == q) {
+                cx = VS::cx(p);
+                cy = VS::cy(p);
+                cz = VS::cz(p);
+            }
+        const int nbk = block_at(slot_of(-cx, -cy, -cz));
+        T val = (T)0;
+        if (nbk >= 0)
+            val = post_collision_at<T, S, Q>(a, nbk, cell_of(-cx, -cy, -cz), q);
+        else
+            missing |= 1ull << q;
+#pragma unroll
+        for (int p = 1; p < Q; p++)
+            if (p == q) fm[p] = val;
+    }
+
+    // Momentum exchange before the boundary rules (post_step 0).
+    if (la.mea) {
+        T acc[3] = {(T)0, (T)0, (T)0};
+        if (linked) {
+#pragma unroll
+            for (int p = 1; p < Q; p++) {
+                if (!VS::half(p)) continue;
+                const int pb = VS::pb(p);
+                if (xq[p] > 0) mea_add<T>(acc, la.order, fs[p], fm[p], xq[p], VS::cx(p), VS::cy(p), VS::cz(p));
+                if (xq[pb] > 0) mea_add<T>(acc, la.order, fs[pb], fm[pb], xq[pb], VS::cx(pb), VS::cy(pb), VS::cz(pb));
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const T s = tree_sum64<T>(acc[d]);
+            if (t == 0) la.blkF[(long long)d * la.nmax + b] = s * la.dv;
+        }
+    }
+
+    // Targets of the pushes (blocks in direction +c_q).
+    int nbq[Q];
+#pragma unroll
+    for (int q = 1; q < Q; q++) nbq[q] = block_at(slot_of(VS::cx(q), VS::cy(q), VS::cz(q)));
+    nbq[0] = b;
+
+    // Cu_ImposeBC: face hooks and the interpolated bounce-back, pair by pair.
+    const bool onb = onb_u == 1 && mask != CELL_SOLID;
+    if (onb) {
+#pragma unroll
+        for (int p = 1; p < Q; p++) {
+            if (!VS::half(p)) continue;
+            const int pb = VS::pb(p);
+            T f_p = fs[p], f_q = fs[pb];
+            if (nbq[p] < 0 && nbq[p] != N_SKIPID)
+                f_p = bc_hook<T>(a.bc_preset, a.U, nbq[p], f_p, u, v, w, (T)VS::w(p), (T)VS::cx(p), (T)VS::cy(p), (T)VS::cz(p));
+            if (curved) {
+                const T dQ = (T)xq[p];
+                if (xq[p] > 0 && xq[p] < 0.5f) {
+                    if (!((missing >> p) & 1)) f_p = (T)(2.0) * dQ * f_p + ((T)(1.0) - (T)(2.0) * dQ) * fm[p];
+                } else if (xq[p] >= 0.5f) {
+                    f_p = ((T)(1.0) / ((T)(2.0) * dQ)) * f_p + (((T)(2.0) * dQ - (T)(1.0)) / ((T)(2.0) * dQ)) * f_q;
+                }
+            }
+            if (nbq[pb] < 0 && nbq[pb] != N_SKIPID)
+                f_q = bc_hook<T>(a.bc_preset, a.U, nbq[pb], f_q, u, v, w, (T)VS::w(pb), (T)VS::cx(pb), (T)VS::cy(pb),
+                                 (T)VS::cz(pb));
+            if (curved) {
+                const T dQ = (T)xq[pb];
+                if (xq[pb] > 0 && xq[pb] < 0.5f) {
+                    if (!((missing >> pb) & 1)) f_q = (T)(2.0) * dQ * f_q + ((T)(1.0) - (T)(2.0) * dQ) * fm[pb];
+                } else if (xq[pb] >= 0.5f) {
+                    f_q = ((T)(1.0) / ((T)(2.0) * dQ)) * f_q + (((T)(2.0) * dQ - (T)(1.0)) / ((T)(2.0) * dQ)) * f_p;
+                }
+            }
+            fs[p] = f_p;
+            fs[pb] = f_q;
+        }
+    }
+
+    // The push (hooks already applied).
+    store_pop(a.B, cell, ST::enc(fs[0], (T)VS::w(0)));
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+        const int nbk = nbq[q];
+        const long long tgt = (long long)nbk * 64 + cell_of(VS::cx(q), VS::cy(q), VS::cz(q));
+        push_link<T, S, Q>(a, q, fs[q], nbk, tgt, (long long)cell, inert, false, false, u, v, w);
+    }
+}
+
+// Cu_ComputeForcesMEA post_step 1 on the finest level after its last sub-step of the coarse step
+// (and its exception fixup): the same sums from the values that arrived -- the reference's slot p
+// after the stream is this build's row pb(p) -- added to the block's entry.
+template <typename T, typename S, int Q>
+__global__ __launch_bounds__(WG) void k_mea_post(const int *__restrict__ ids, int n, const S *__restrict__ F,
+                                                 long long ncm, const int *__restrict__ nbr_slot,
+                                                 const int *__restrict__ cells_mask, LinkArgs<T> la)
+{
+    using VS = VelSet<Q>;
+    using ST = Store<T, S>;
+    const int t = threadIdx.x & 63;
+    const int bi = blockIdx.x * (WG / 64) + (int)(threadIdx.x >> 6);
+    if (bi >= n) return;
+    const int b = __builtin_amdgcn_readfirstlane(ids[bi]);
+    const int rank = __builtin_amdgcn_readfirstlane(la.onb_solid[b]);
+    const long long cell = (long long)b * 64 + t;
+    const int my_nb = nbr_slot[(long long)b * 27 + (t < 27 ? t : 26)];
+    const bool linked = rank >= 0 && cells_mask[cell] == CELL_BOUNDARY;
+    const int I = t & 3, J = (t >> 2) & 3, K = t >> 4;
+    const long long cb = (long long)rank * 64 + t;
+    T acc[3] = {(T)0, (T)0, (T)0};
+    // Slot s of cell (I,J,K) - c_s: row pb(s) there.
+    auto behind = [&](int s_dir) -> T {
+        const int cx = VS::cx(s_dir), cy = VS::cy(s_dir), cz = VS::cz(s_dir);
+        const int Ip = I - cx, Jp = J - cy, Kp = K - cz;
+        const int slot = (Ip < 0 ? 0 : (Ip >= 4 ? 2 : 1)) + 3 * (Jp < 0 ? 0 : (Jp >= 4 ? 2 : 1)) +
+                         9 * (Kp < 0 ? 0 : (Kp >= 4 ? 2 : 1));
+        const int nbk = __shfl(my_nb, slot, 64);
+        if (nbk < 0) return (T)0;
+        const long long o = (long long)nbk * 64 + ((Ip & 3) + 4 * (Jp & 3) + 16 * (Kp & 3));
+        return ST::dec(F[(long long)VS::pb(s_dir) * ncm + o], (T)VS::w(s_dir));
+    };
+#pragma unroll
+    for (int p = 1; p < Q; p++) {
+        if (!VS::half(p)) continue;
+        const int pb = VS::pb(p);
+        // every lane evaluates its shuffles; only linked lanes add
+        const T fmp = behind(p), fmq = behind(pb);
+        if (!linked) continue;
+        const float xp = la.X_b[cb + (long long)p * la.ncb], xb = la.X_b[cb + (long long)pb * la.ncb];
+        const T f_p = ST::dec(F[(long long)pb * ncm + cell], (T)VS::w(p));
+        const T f_q = ST::dec(F[(long long)p * ncm + cell], (T)VS::w(pb));
+        if (xp > 0) mea_add<T>(acc, la.order, f_p, fmp, xp, VS::cx(p), VS::cy(p), VS::cz(p));
+        if (xb > 0) mea_add<T>(acc, la.order, f_q, fmq, xb, VS::cx(pb), VS::cy(pb), VS::cz(pb));
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const T s = tree_sum64<T>(acc[d]);
+        if (t == 0) la.blkF[(long long)d * la.nmax + b] += s * la.dv;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------------------------
 template <typename T>
@@ -1495,7 +1794,52 @@ static StepArgs<T, S> step_args(Ctx *c, int L, int rd, int vmode = 0)
 template <int Q>
 static int pair_mode(const Ctx *c, int L)
 {
-    return (VelSet<Q>::DIM == 3 && c->defer_coarse && L == c->MAX_LEVELS - 1) ? c->fine_pair : 0;
+    // Linked blocks recompute neighbours' collisions from the input buffer as it is stored, so
+    // with links every sub-step resolves its exceptions by a fixup instead.
+    return (VelSet<Q>::DIM == 3 && c->defer_coarse && L == c->MAX_LEVELS - 1 && !c->links_on) ? c->fine_pair : 0;
+}
+
+// The momentum-exchange force (force type 0) is evaluated on the finest level's last sub-step of a
+// coarse step, every step on a single level (solver_lbm_advance.cu:124-131, :144-150).
+static bool mea_active(const Ctx *c) { return c->links_on && c->P.S_FORCE_TYPE == 0 && c->DIM == 3; }
+static bool mea_due(const Ctx *c, int L)
+{
+    if (!mea_active(c) || L != c->MAX_LEVELS - 1) return false;
+    const bool multi = c->MAX_LEVELS > 1 && c->MAX_LEVELS != c->N_LEVEL_START + 1;
+    return !multi || c->global_Liter == (1 << (c->MAX_LEVELS - 1)) - 1;
+}
+
+template <typename T>
+static LinkArgs<T> link_args(Ctx *c, int L, bool mea)
+{
+    LinkArgs<T> la;
+    la.ids = c->lk_ids + c->lk_off[L];
+    la.n = c->lk_n[L];
+    la.onb_solid = c->lk_onb_solid;
+    la.X_b = c->lk_X;
+    la.ncb = c->lk_ncb;
+    la.bc_type = c->P.S_BC_TYPE;
+    la.mea = mea ? 1 : 0;
+    la.order = c->P.S_FORCE_ORDER;
+    la.blkF = static_cast<T *>(c->lk_blkF);
+    la.nmax = c->n_maxcblocks;
+    const T dx = (T)c->dxf_vec[L];
+    la.dv = (T)((T)((T)(T)1 * dx) * dx) * dx;  // dvf_vec (solver_lbm_init.cu:19-27)
+    return la;
+}
+
+// The linked blocks' sub-step of level L reading buffer rd, on stream s.
+template <typename T, typename S, int Q>
+static int launch_link_step(Ctx *c, int L, int rd, hipStream_t s)
+{
+    if constexpr (VelSet<Q>::DIM == 3) {
+        if (!c->links_on || c->lk_n[L] <= 0) return AGAL_OK;
+        const LinkArgs<T> la = link_args<T>(c, L, mea_due(c, L));
+        hipLaunchKernelGGL((k_link_step<T, S, Q>), dim3((unsigned)((la.n + WG / 64 - 1) / (WG / 64))), dim3(WG), 0, s,
+                           step_args<T, S, Q>(c, L, rd, 0), la);
+        AGAL_HIP(c, hipGetLastError());
+    }
+    return AGAL_OK;
 }
 
 template <typename T, typename S>
@@ -1636,6 +1980,8 @@ static int launch_fused(Ctx *c, const int *levels, const int *rd, int nl, const 
             hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, s, m, x);
             AGAL_HIP(c, hipGetLastError());
         }
+        for (int j = base; j < nl && j < base + MAXLV; j++)
+            if ((st = launch_link_step<T, S, Q>(c, levels[j], rd[j], s))) return st;
         if ((st = sched_done(c, s, slot, rmask, wmask))) return st;
         // A list left for the next finest sub-step (pair mode 1) is not pending a fixup.
         for (int j = base; j < nl && j < base + MAXLV; j++)
@@ -1815,6 +2161,24 @@ static int avg_interp_t(Ctx *c, int L)
         return AGAL_ERR_ARG;                                                               \
     } while (0)
 
+template <typename T, typename S, int Q>
+static int mea_post_t(Ctx *c)
+{
+    if constexpr (VelSet<Q>::DIM == 3) {
+        const int L = c->MAX_LEVELS - 1;
+        if (!mea_active(c) || c->lk_n[L] <= 0) return AGAL_OK;
+        const LinkArgs<T> la = link_args<T>(c, L, true);
+        const unsigned rm = res_bit(L, c->cur[L]);
+        hipStream_t s = sched_begin(c, 0, rm, 0);
+        hipLaunchKernelGGL((k_mea_post<T, S, Q>), dim3((unsigned)((la.n + WG / 64 - 1) / (WG / 64))), dim3(WG), 0, s, la.ids,
+                           la.n, Fbuf<S>(c, c->cur[L]), c->n_maxcells, c->nbr_slot, c->cells_mask, la);
+        AGAL_HIP(c, hipGetLastError());
+        return sched_done(c, s, 0, rm, 0);
+    }
+    return AGAL_OK;
+}
+
+int launch_mea_post(Ctx *c) { AGAL_DISPATCH(c, mea_post_t, c); }
 int launch_set_ic(Ctx *c, int L) { AGAL_DISPATCH(c, set_ic_t, c, L); }
 int launch_refresh(Ctx *c, int L) { AGAL_DISPATCH(c, refresh_t, c, L); }
 int launch_collide_stream(Ctx *c, int L) { AGAL_DISPATCH(c, collide_stream_t, c, L); }

@@ -129,10 +129,40 @@ class LbmMesh:
         self._call("check_masks", int(L))
 
     def S_ComputeForces(self) -> np.ndarray:
-        """Momentum-exchange force on the geometry after the last coarse step (x, y, z)."""
+        """S_ReportForces: the momentum-exchange force (x, y, z) accumulated over the last coarse step;
+        the accumulators are reset (needs G_BIN_MAKE_LINKS 1, S_FORCE_TYPE 0)."""
         out = (C.c_double * 3)()
         self._call("forces", out)
         return np.array(list(out))
+
+    def links(self) -> dict:
+        """Face-cell links: onb_solid (block -> rank), X_b (27, 64 * n_solidb) fractions, n_solidb,
+        corrections of the validation."""
+        nb = self.n_blocks()
+        info = (C.c_longlong * 3)()
+        onb = np.empty(nb, dtype=np.int32)
+        self._call("download_links", np_ptr(onb), nb, None, 0, info)
+        ns = int(info[0])
+        xb = np.empty(27 * 64 * ns, dtype=np.float32)
+        if ns > 0 and info[2]:
+            self._call("download_links", None, 0, np_ptr(xb), xb.size, info)
+        else:
+            xb[:] = -1.0
+        return {"onb_solid": onb, "X_b": xb.reshape(27, 64 * ns), "n_solidb": ns, "corrections": int(info[1]),
+                "present": bool(info[2])}
+
+    def link_bins(self) -> dict:
+        """The face bins of the link search: start (n_bins + 1), faces, bins per unit length, kept faces."""
+        info = (C.c_longlong * 4)()
+        self._call("download_link_bins", None, 0, None, 0, info)
+        nbins, nent = int(info[0]), int(info[1])
+        start = np.zeros(max(nbins + 1, 0), dtype=np.int32)
+        faces = np.zeros(nent, dtype=np.int32)
+        if nbins > 0:
+            self._call("download_link_bins", np_ptr(start), start.size, None, 0, info)
+        if nent > 0:
+            self._call("download_link_bins", None, 0, np_ptr(faces), nent, info)
+        return {"start": start, "faces": faces, "G": int(info[2]), "kept": int(info[3])}
 
     # -- learned refinement predictor (no reference counterpart) -----------------------------------
     def predictor_init(self, seed=1, batch_capacity=4096):
@@ -284,8 +314,10 @@ class LbmMesh:
         self._call("import_pool", C.byref(p))
         self.refresh_sizes()
 
-    def state(self) -> dict:
-        """Everything the parity tests compare, restricted to active blocks where it matters."""
+    def state(self, with_f: bool = True, with_cells: bool = True) -> dict:
+        """Everything the parity tests compare, restricted to active blocks where it matters
+        (without the populations when with_f is False, without cell masks and the per-block masks
+        when with_cells is False)."""
         s = self.refresh_sizes()
         ML = s.max_levels
         st = {
@@ -300,9 +332,11 @@ class LbmMesh:
         st["level"] = self.block_ints(capi.BLK_LEVEL, nb)
         st["nbr"] = self.block_ints(capi.BLK_NBR, nb)
         st["nbr_child"] = self.block_ints(capi.BLK_NBR_CHILD, nb)
-        st["onb"] = self.block_ints(capi.BLK_ONB, nb)
-        st["blk_mask"] = self.block_ints(capi.BLK_MASK, nb)
         st["coords"] = self.coords(nb)
-        st["cell_mask"] = self.cell_masks(nb)
-        st["f"] = self.populations(nb)
+        if with_cells:
+            st["onb"] = self.block_ints(capi.BLK_ONB, nb)
+            st["blk_mask"] = self.block_ints(capi.BLK_MASK, nb)
+            st["cell_mask"] = self.cell_masks(nb)
+        if with_f:
+            st["f"] = self.populations(nb)
         return st

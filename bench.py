@@ -61,7 +61,7 @@ def parse():
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 five-level cavity leg")
     ap.add_argument("--large-develop", type=int, default=300)
     ap.add_argument("--large-steps", type=int, default=10)
-    ap.add_argument("--large-capacity", type=int, default=6000000)
+    ap.add_argument("--large-capacity", type=int, default=12000000)
     ap.add_argument("--no-storage-study", action="store_true",
                     help="skip the FP16/BF16/FP32/FP64 comparison on one mesh sequence")
     ap.add_argument("--no-sphere-study", action="store_true",

@@ -103,6 +103,13 @@ public:
         base.s5 = Dbl("s5");
         base.s6 = Dbl("s6");
         base.G_NEAR_WALL_DISTANCE = Dbl("G_NEAR_WALL_DISTANCE");
+        base.G_BIN_DENSITY = Int("G_BIN_DENSITY");
+        base.G_BIN_LEVELS = Int("G_BIN_LEVELS");
+        base.G_BIN_SPEC = Int("G_BIN_SPEC");
+        base.G_BIN_MAKE_LINKS = Int("G_BIN_MAKE_LINKS");
+        base.S_BC_TYPE = Int("S_BC_TYPE");
+        base.S_FORCE_TYPE = Int("S_FORCE_TYPE");
+        base.S_FORCE_ORDER = Int("S_FORCE_ORDER");
         const int coll = Int("S_COLLISION");
         if (coll == 0) base.collision = AGAL_BGK;
         else if (coll == 2) base.collision = AGAL_MRT;

@@ -125,6 +125,13 @@ typedef struct agal_params {
                                         (input.txt keys s1..s6, solver_lbm_init.cu:55-61)       */
     double G_NEAR_WALL_DISTANCE;  /* width of the refined band around a geometry on level 0
                                      (parser.h:121; halved per level, mesh_voxelizer.cu:39-41) */
+    /* Face-cell links of a geometry (parser.h:73-89): G_BIN_* size the face bins
+     * (geometry_bins.h:118-123), G_BIN_MAKE_LINKS 1 computes the cut-link fractions
+     * (mesh_advance.cu:106-111); S_BC_TYPE 1 applies the linear interpolated bounce-back on them
+     * (solver_lbm_kernels.cu:796-875); S_FORCE_TYPE 0 with S_FORCE_ORDER 1 / 2 is the
+     * momentum-exchange force on the linked cells (solver_lbm_kernels_forces.cu:322-540). */
+    int    G_BIN_DENSITY, G_BIN_LEVELS, G_BIN_SPEC, G_BIN_MAKE_LINKS;
+    int    S_BC_TYPE, S_FORCE_TYPE, S_FORCE_ORDER;
 } agal_params;
 
 typedef struct agal_ctx agal_ctx;
@@ -275,11 +282,27 @@ int  agal_voxelize(agal_ctx *ctx, int L);
 /* Mesh::M_Geometry_Voxelize_S2 (mesh_voxelizer.cu:139-174) on level L: BOUNDARY cell masks (non-
  * solid cells with a solid neighbour) in leaf SOLIDB / SOLIDA blocks. */
 int  agal_check_masks(agal_ctx *ctx, int L);
-/* Solver_LBM::S_ComputeForcesMEA (order 1) + S_ReportForces (solver_lbm_kernels_forces.cu:322-540,
- * solver_lbm_compute.cu:165-218), evaluated after a coarse step on the finest level: out = force on
- * the body, (sum over BOUNDARY cells x and links p into a solid cell of 2 f*_p c_p) dx^3 / dx with
- * dx the finest spacing, accumulated in FP64.  Drag / lift coefficients are left to the caller. */
+/* Solver_LBM::S_ReportForces (solver_lbm_compute.cu:165-218), force type 0: the momentum-exchange
+ * sums of Cu_ComputeForcesMEA (solver_lbm_kernels_forces.cu:322-540), which every coarse step forms
+ * around the finest level's last sub-step (after the collision and after the stream) on the cut
+ * links of boundary cells -- order 1: f c_p; order 2 (S_FORCE_ORDER 2): ((1/2 + q) f + (1/2 - q)
+ * f_m) c_p -- per block times dx^3, summed over blocks in FP64 and times 1/dx of the finest level
+ * into out (x, y, z); the block sums are then reset.  Needs G_BIN_MAKE_LINKS 1 and S_FORCE_TYPE 0
+ * (else AGAL_ERR_STATE).  Drag / lift coefficients are left to the caller. */
 int  agal_forces(agal_ctx *ctx, double out[3]);
+/* Face-cell links after agal_refine_near_wall with G_BIN_MAKE_LINKS 1 (Mesh::M_Geometry_Voxelize_S3
+ * and M_LinkLengthComputation, mesh_voxelizer.cu:177-296): onb_solid[k] = rank of block k among the
+ * blocks holding boundary cells (cblock_ID_onb_solid, -1 otherwise) for k < n_blocks; X_b = the
+ * first n_values cut-link fractions, [rank*64 + cell + q*64*n_solidb] over the 27 directions
+ * (cells_f_X_b, -1 = no cut); info = {n_solidb (ranks rounded up to a multiple of 128), corrections
+ * counted by the validation, links present}.  NULL pointers skip a part. */
+int  agal_download_links(agal_ctx *ctx, int *onb_solid, long long n_blocks, float *X_b, long long n_values,
+                         long long info[3]);
+/* The face bins of the link search (Geometry::Bins::G_MakeBinsGPU_MD, geometry_bins_gpu.cu:1043-
+ * 1336): bin_start (n_bins + 1 offsets) and faces (per bin, ordered by overlap rank then face);
+ * info = {n_bins, entries, bins per unit length, faces kept by the ray-indicator filter}. */
+int  agal_download_link_bins(agal_ctx *ctx, int *bin_start, long long n_start, int *faces, long long n_faces,
+                             long long info[4]);
 /* Host helpers, no context.  Icosphere: the icosahedron subdivided `subdivisions` times (20*4^s
  * triangles), each vertex at radius r (1 + noise u), u uniform in [-1, 1) from a splitmix64
  * stream seeded with `seed`, in vertex order.  ASCII STL: the facets of the file (the format of
@@ -360,6 +383,11 @@ int  agal_import_pool(agal_ctx *ctx, const agal_pool *pool);
  * order).  0 (default) keeps the vector path, whose sums follow a fixed order shared with the CPU
  * restatement. */
 #define AGAL_OPT_MRT_MFMA 2
+/* AGAL_OPT_EXC_CAP: entries per stream-guard exception list (default and maximum:
+ * min(cells of the pool, 2^22) per level).  A sub-step whose list overflows drops links; the next
+ * agal_sync / agal_advance_loop reports AGAL_ERR_OVERFLOW and the context refuses further steps
+ * (AGAL_ERR_STATE). */
+#define AGAL_OPT_EXC_CAP 3
 int  agal_set_option(agal_ctx *ctx, int option, long long value);
 
 /* ---- measurement ----------------------------------------------------------------------------- */

@@ -246,6 +246,197 @@ inline T floor_log2_exact(T x)
     return T(std::ilogb(x));
 }
 
+// ---------------------------------------------------------------------------------------------
+// Float geometry of the face-cell links, in the reference's operation order (ufloat_g_t = float,
+// src/main.cu:61): vec3 arithmetic and face helpers (inc/util.h:273-560), triangle-box overlap by
+// separating axes (inc/util_tribin.h:81-160).  Compiled without FMA contraction.
+// ---------------------------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+inline V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+inline float dotv(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 crossv(V3 a, V3 b) { return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline V3 unitv(V3 a)
+{
+    const float onorm = 1.0f / std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    return a * onorm;
+}
+inline V3 face_vertex(const float *tri, long f, int k) { return V3{tri[9 * f + 3 * k], tri[9 * f + 3 * k + 1], tri[9 * f + 3 * k + 2]}; }
+// FaceNormal / FaceNormalUnit (util.h:419-432), 3D.
+inline V3 face_normal(V3 v1, V3 v2, V3 v3) { return crossv(v2 - v1, v3 - v1); }
+inline V3 face_normal_unit(V3 v1, V3 v2, V3 v3) { return unitv(face_normal(v1, v2, v3)); }
+// CheckPointInTriangleExtended (util.h:522-545): inside or within eps of every edge line.
+inline bool point_in_triangle_ext(V3 vp, V3 v1, V3 v2, V3 v3, V3 n, float eps = std::numeric_limits<float>::epsilon())
+{
+    V3 ven = unitv(crossv(v2 - v1, n));
+    if (dotv(v1 - vp, ven) <= -eps) return false;
+    ven = unitv(crossv(v3 - v2, n));
+    if (dotv(v2 - vp, ven) <= -eps) return false;
+    ven = unitv(crossv(v1 - v3, n));
+    if (dotv(v3 - vp, ven) <= -eps) return false;
+    return true;
+}
+// TrianglePlaneCutsBox (util_tribin.h:81-94).
+inline bool plane_cuts_box(V3 vm, V3 vM, V3 v1, V3 v2, V3 v3)
+{
+    const V3 n = face_normal(v1, v2, v3);
+    const V3 c{n.x > 0 ? (vM.x - vm.x) : 0.0f, n.y > 0 ? (vM.y - vm.y) : 0.0f, n.z > 0 ? (vM.z - vm.z) : 0.0f};
+    const float d = dotv(n, vm);
+    const float d1 = dotv(n, c - v1);
+    const float d2 = dotv(n, (vM - vm - c) - v1);
+    return (d + d1) * (d + d2) <= 0;
+}
+// TriangleBinOverlap2D_AxisGap (util_tribin.h:96-128): true when `axis` separates the projected
+// triangle from the projected box.
+inline bool axis_gap_2d(int axis, float v1x, float v1y, float v2x, float v2y, float v3x, float v3y, float rmx, float rmy,
+                        float rMx, float rMy)
+{
+    float ex = 0.0f, ey = 0.0f;
+    if (axis == 0) { ex = 1.0f; ey = 0.0f; }
+    if (axis == 1) { ex = 0.0f; ey = 1.0f; }
+    if (axis == 2) { ex = v2y - v1y; ey = v1x - v2x; }
+    if (axis == 3) { ex = v3y - v2y; ey = v2x - v3x; }
+    if (axis == 4) { ex = v1y - v3y; ey = v3x - v1x; }
+    float t = std::fmax(std::fmax(v1x * ex + v1y * ey, v2x * ex + v2y * ey), v3x * ex + v3y * ey);
+    float r = std::fmin(std::fmin(std::fmin(rmx * ex + rmy * ey, rMx * ex + rmy * ey), rmx * ex + rMy * ey), rMx * ex + rMy * ey);
+    bool b = t < r;
+    t = std::fmin(std::fmin(v1x * ex + v1y * ey, v2x * ex + v2y * ey), v3x * ex + v3y * ey);
+    r = std::fmax(std::fmax(std::fmax(rmx * ex + rmy * ey, rMx * ex + rmy * ey), rmx * ex + rMy * ey), rMx * ex + rMy * ey);
+    return b || (r < t);
+}
+// TriangleBinOverlap3D (util_tribin.h:131-160): the plane test, then five axes in each of the XY,
+// YZ and ZX projections.
+inline bool tri_box_overlap(V3 vm, V3 vM, V3 v1, V3 v2, V3 v3)
+{
+    if (!plane_cuts_box(vm, vM, v1, v2, v3)) return false;
+    for (int a = 0; a < 5; a++)
+        if (axis_gap_2d(a, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y, vm.x, vm.y, vM.x, vM.y)) return false;
+    for (int a = 0; a < 5; a++)
+        if (axis_gap_2d(a, v1.y, v1.z, v2.y, v2.z, v3.y, v3.z, vm.y, vm.z, vM.y, vM.z)) return false;
+    for (int a = 0; a < 5; a++)
+        if (axis_gap_2d(a, v1.z, v1.x, v2.z, v2.x, v3.z, v3.x, vm.z, vm.x, vM.z, vM.x)) return false;
+    return true;
+}
+
+// Face bins of the link computation: Geometry::Bins with G_MakeBinsGPU_MD (geometry_bins_gpu.cu:
+// 1043-1336), for the 3D velocity sets.  bin_start has n_bins + 1 offsets into faces.
+struct LinkBins {
+    int level = 0;         // n_bin_level_MD = min(MAX_LEVELS_WALL, G_BIN_LEVELS) - 1 (geometry_bins.h:201)
+    int G = 0;             // bins per unit length: G_BIN_DENSITY * 2^level
+    bool use_ray = false;  // the ray-indicator filter (geometry_bins_gpu.cu:1060-1065)
+    std::vector<int> filtered;   // face ids that pass the filter, ascending (all when unused)
+    std::vector<int> bin_start;  // CSR offsets, n_bins + 1
+    std::vector<int> faces;      // per bin: (overlap rank, face) ascending
+};
+
+// Cu_ComputeVoxelRayIndicators_MD (geometry_bins_gpu.cu:22-157), 3D: the face is kept when the line
+// through some level cell centre of its bounding box, along some half-set direction, meets it.
+inline bool face_ray_indicator(const float *tri, long f, float dx_L, float dx_Lo2, float Nx, float Ny, float Nz,
+                               const int conn[3][27])
+{
+    const V3 v1 = face_vertex(tri, f, 0), v2 = face_vertex(tri, f, 1), v3 = face_vertex(tri, f, 2);
+    const V3 n = face_normal_unit(v1, v2, v3);
+    const int ixmin = (int)(std::fmin(std::fmin(v1.x, v2.x), v3.x) * Nx);
+    const int iymin = (int)(std::fmin(std::fmin(v1.y, v2.y), v3.y) * Ny);
+    const int izmin = (int)(std::fmin(std::fmin(v1.z, v2.z), v3.z) * Nz);
+    const int ixmax = (int)(std::fmax(std::fmax(v1.x, v2.x), v3.x) * Nx);
+    const int iymax = (int)(std::fmax(std::fmax(v1.y, v2.y), v3.y) * Ny);
+    const int izmax = (int)(std::fmax(std::fmax(v1.z, v2.z), v3.z) * Nz);
+    for (int p = 1; p < 27; p++) {
+        if (!(p == 26 || ((p - 1) % 2 == 0 && p < 25))) continue;
+        const V3 ray{(float)conn[0][p], (float)conn[1][p], (float)conn[2][p]};
+        for (int iz = izmin; iz <= izmax; iz++)
+            for (int iy = iymin; iy <= iymax; iy++)
+                for (int ix = ixmin; ix <= ixmax; ix++) {
+                    const V3 vp{dx_Lo2 + dx_L * (float)ix, dx_Lo2 + dx_L * (float)iy, dx_Lo2 + dx_L * (float)iz};
+                    const float d = dotv(v1 - vp, n) / dotv(ray, n);
+                    const V3 vi = vp + ray * d;
+                    if (point_in_triangle_ext(vi, v1, v2, v3, n)) return true;
+                }
+    }
+    return false;
+}
+
+// The bins.  Cu_ComputeBoundingBoxLimits3D (geometry_bins_gpu.cu:421-561) gives each face its
+// first (2 + G_BIN_SPEC)^3 overlapping bins, boxes grown by dx_L + FLT_EPSILON, scanning K, J, I
+// from one bin below to one above the face's bounding box; the (bin, face) pairs sit at
+// position rank * n_filtered + filtered index and are stably sorted by bin (thrust's radix sort).
+// Pairs whose bin index falls outside [0, n_bins) are dropped (the reference scatters them out of
+// bounds).
+inline void make_link_bins(LinkBins &B, const float *tri, long n_faces, const agal_params &P, const int conn[3][27])
+{
+    const int n_bin_levels = std::min(P.G_BIN_LEVELS, P.MAX_LEVELS_WALL);
+    B.level = std::min(P.MAX_LEVELS_WALL, n_bin_levels) - 1;
+    B.G = P.G_BIN_DENSITY << B.level;
+    B.use_ray = P.MAX_LEVELS_WALL == B.level + 1;
+    const float Lx = (float)P.L_c, Ly = (float)(P.L_fy * P.L_c), Lz = (float)(P.L_fz * P.L_c);
+    const int Nx = P.Nx, Ny = (int)(Nx * (Ly / Lx)), Nz = (int)(Nx * (Lz / Lx));
+    float dxg = Lx / (float)Nx, Lx0g = Lx / (float)P.G_BIN_DENSITY, Ly0g = Ly / (float)P.G_BIN_DENSITY,
+          Lz0g = Lz / (float)P.G_BIN_DENSITY;
+    int NxL = Nx, NyL = Ny, NzL = Nz;
+    for (int L = 1; L <= B.level; L++) {
+        dxg *= 0.5f;
+        Lx0g *= 0.5f;
+        Ly0g *= 0.5f;
+        Lz0g *= 0.5f;
+        NxL *= 2;
+        NyL *= 2;
+        NzL *= 2;
+    }
+    B.filtered.clear();
+    for (long f = 0; f < n_faces; f++)
+        if (!B.use_ray || face_ray_indicator(tri, f, dxg, 0.5f * dxg, (float)NxL, (float)NyL, (float)NzL, conn))
+            B.filtered.push_back((int)f);
+    const long nf = (long)B.filtered.size();
+    const long n_bins = (long)B.G * B.G * B.G;
+    const int n_limits = (2 + P.G_BIN_SPEC) * (2 + P.G_BIN_SPEC) * (2 + P.G_BIN_SPEC);
+    const float dx = dxg + std::numeric_limits<float>::epsilon();
+    struct E {
+        long bin, pos;
+        int face;
+    };
+    std::vector<E> ents;
+    for (long m = 0; m < nf; m++) {
+        const int f = B.filtered[m];
+        const V3 v1 = face_vertex(tri, f, 0), v2 = face_vertex(tri, f, 1), v3 = face_vertex(tri, f, 2);
+        const V3 vBm{std::fmin(std::fmin(v1.x, v2.x), v3.x), std::fmin(std::fmin(v1.y, v2.y), v3.y),
+                     std::fmin(std::fmin(v1.z, v2.z), v3.z)};
+        const V3 vBM{std::fmax(std::fmax(v1.x, v2.x), v3.x), std::fmax(std::fmax(v1.y, v2.y), v3.y),
+                     std::fmax(std::fmax(v1.z, v2.z), v3.z)};
+        bool C = true;
+        if ((vBm.x < -dx && vBM.x < -dx) || (vBm.x > Lx + dx && vBM.x > Lx + dx)) C = false;
+        if ((vBm.y < -dx && vBM.y < -dx) || (vBm.y > Ly + dx && vBM.y > Ly + dx)) C = false;
+        if ((vBm.z < -dx && vBM.z < -dx) || (vBm.z > Lz + dx && vBM.z > Lz + dx)) C = false;
+        if (!C) continue;
+        const int xl = (int)(vBm.x * (float)B.G) - 1, yl = (int)(vBm.y * (float)B.G) - 1, zl = (int)(vBm.z * (float)B.G) - 1;
+        const int xL = (int)(vBM.x * (float)B.G) + 1, yL = (int)(vBM.y * (float)B.G) + 1, zL = (int)(vBM.z * (float)B.G) + 1;
+        int counter = 0;
+        for (int K = zl; K < zL + 1; K++)
+            for (int J = yl; J < yL + 1; J++)
+                for (int I = xl; I < xL + 1; I++) {
+                    const V3 vm{(float)I * Lx0g - dx, (float)J * Ly0g - dx, (float)K * Lz0g - dx};
+                    const V3 vM{(float)(I + 1) * Lx0g + dx, (float)(J + 1) * Ly0g + dx, (float)(K + 1) * Lz0g + dx};
+                    if (tri_box_overlap(vm, vM, v1, v2, v3) && counter < n_limits) {
+                        const long gid = (long)I + (long)B.G * J + (long)B.G * B.G * K;
+                        if (gid >= 0 && gid < n_bins) ents.push_back(E{gid, (long)counter * nf + m, f});
+                        counter++;
+                    }
+                }
+    }
+    std::sort(ents.begin(), ents.end(), [](const E &a, const E &b) { return a.bin != b.bin ? a.bin < b.bin : a.pos < b.pos; });
+    B.bin_start.assign(n_bins + 1, 0);
+    B.faces.resize(ents.size());
+    for (size_t k = 0; k < ents.size(); k++) {
+        B.bin_start[ents[k].bin + 1]++;
+        B.faces[k] = ents[k].face;
+    }
+    for (long k = 0; k < n_bins; k++) B.bin_start[k + 1] += B.bin_start[k];
+}
+
 template <typename T>
 struct Oracle {
     agal_params P{};
@@ -338,6 +529,7 @@ struct Oracle {
         F.assign((size_t)N_U * n_maxcells, T(0));
         cells_mask.assign(n_maxcells, CELL_INTERIOR);
         X.assign((size_t)DIM * n_maxcblocks, -1.0f);
+        blk_F.assign((size_t)3 * n_maxcblocks, T(0));
         blk_mask.assign(n_maxcblocks, BLK_REGULAR);
         nbr.assign((size_t)Q_max * n_maxcblocks, N_SKIPID);
         nbr_child.assign((size_t)Q_max * n_maxcblocks, N_SKIPID);
@@ -683,12 +875,55 @@ struct Oracle {
         for (int S = 0; S < Q_max; S++) s_nbr[S] = nbr[b + (size_t)V.conn_map[S] * n_maxcblocks];
     }
 
-    // Cu_ImposeBC (solver_lbm_kernels.cu:691-887), curved-boundary variants excluded (S_BC_TYPE 0).
+    // Post-collision populations of level L before the boundary hooks: what the interpolated
+    // bounce-back reads "behind" a cell.  The reference's kernel reads them from the pool while
+    // other blocks of the same launch overwrite their boundary cells' slots; here every read sees
+    // the value the collision wrote.
+    std::vector<T> post;
+    T post_at(int p, long cell) const { return post[(size_t)p * n_maxcells + cell]; }
+
+    bool bouzidi_on() const { return links && P.S_BC_TYPE == 1; }
+
+    // Cu_ImposeBC (solver_lbm_kernels.cu:691-887): the preset hook on domain faces and, with
+    // S_BC_TYPE 1, the linear interpolated bounce-back of Bouzidi, Firdaouss and Lallemand (2001)
+    // on the cut links of BOUNDARY cells: for the fraction q of the link along c_p,
+    // q < 1/2: f_p <- 2q f_p + (1 - 2q) f*_p(x - c_p);  q >= 1/2: f_p <- f_p / 2q + (2q - 1)/2q f_pb,
+    // applied per half-set pair in the order hook(p), link(p), hook(pb), link(pb) -- so the pb
+    // link's second form already sees the new f_p.  A missing block behind the cell (the reference
+    // reads out of bounds there) leaves f_p unchanged.
     void impose_bc(int L)
     {
         if (n_ids[L] <= 0) return;
         const int nL = n_ids[L];
         const int *ids = idset(L);
+        const bool bz = bouzidi_on();
+        if (bz) {
+            post.assign((size_t)Q * n_maxcells, T(0));
+            for (int bi = 0; bi < nL; bi++) {
+                const int b = ids[bi];
+                for (int p = 0; p < Q; p++)
+                    for (int t = 0; t < M_CBLOCK; t++) {
+                        const long cell = (long)b * M_CBLOCK + t;
+                        post[(size_t)p * n_maxcells + cell] = Fp(p)[cell];
+                    }
+            }
+        }
+        // f*_p at x - c_p; `ok` false when that block is missing.
+        auto behind = [&](const int *s_nbr, int I, int J, int K, int p, bool &ok) -> T {
+            const int pbp = V.pb[p];
+            const int Ip = I + V.c[0][pbp], Jp = J + V.c[1][pbp], Kp = K + V.c[2][pbp];
+            const int nb = s_nbr[nbr_map(DIM, Ip, Jp, Kp)];
+            ok = nb >= 0;
+            return ok ? post_at(p, (long)nb * M_CBLOCK + nbr_cell_id(DIM, Ip, Jp, Kp)) : (T)0;
+        };
+        auto link = [&](T f, T fm, bool ok, T fo, float dQ) -> T {
+            if (dQ > 0 && dQ < 0.5f) {
+                if (ok) f = (T)(2.0) * (T)dQ * f + ((T)(1.0) - (T)(2.0) * (T)dQ) * fm;
+            } else if (dQ >= 0.5f) {
+                f = ((T)(1.0) / ((T)(2.0) * (T)dQ)) * f + (((T)(2.0) * (T)dQ - (T)(1.0)) / ((T)(2.0) * (T)dQ)) * fo;
+            }
+            return f;
+        };
 #pragma omp parallel for schedule(static)
         for (int bi = 0; bi < nL; bi++) {
             int b = ids[bi];
@@ -712,10 +947,22 @@ struct Oracle {
                         if (nb < 0 && nb != N_SKIPID)
                             f_p = bc_hook(nb, f_p, rho, u, v, w, (T)V.w[p], (T)V.c[0][p], (T)V.c[1][p], (T)V.c[2][p]);
                     }
+                    const bool curved = bz && valid_mask == CELL_BOUNDARY && onb_solid[b] > -1;
+                    const long cb = curved ? (long)onb_solid[b] * M_CBLOCK + t : 0;
+                    if (curved) {
+                        bool ok;
+                        const T fm = behind(s_nbr, I, J, K, p, ok);
+                        f_p = link(f_p, fm, ok, f_q, link_x(cb, p));
+                    }
                     {
                         int nb = s_nbr[nbr_map(DIM, I + V.c[0][pb], J + V.c[1][pb], (DIM == 3) ? K + V.c[2][pb] : 0)];
                         if (nb < 0 && nb != N_SKIPID)
                             f_q = bc_hook(nb, f_q, rho, u, v, w, (T)V.w[pb], (T)V.c[0][pb], (T)V.c[1][pb], (T)V.c[2][pb]);
+                    }
+                    if (curved) {
+                        bool ok;
+                        const T fm = behind(s_nbr, I, J, K, pb, ok);
+                        f_q = link(f_q, fm, ok, f_p, link_x(cb, pb));
                     }
                     if (valid_mask != CELL_SOLID) {
                         Fp(p)[cell] = f_p;
@@ -1008,12 +1255,14 @@ struct Oracle {
         }
     }
 
-    void collide_stream(int L)
+    void collide_stream(int L, bool mea = false)
     {
         collide(L);
+        if (mea) forces_mea(L, 0);
         impose_bc(L);
         if (store_mode) store_level(L);
         stream(L);
+        if (mea) forces_mea(L, 1);
     }
 
     // Writes every population of level L into the 16-bit pool (values already stored read back
@@ -1036,8 +1285,9 @@ struct Oracle {
         }
     }
 
-    // Solver_LBM::S_Advance (solver_lbm_advance.cu:30-174), forces excluded (they do not alter
-    // the populations).
+    // Solver_LBM::S_Advance (solver_lbm_advance.cu:30-174).  The momentum-exchange force (force
+    // type 0) brackets the finest level's last sub-step of a coarse step, and every step on a
+    // single level (:124-126, :144-150).
     void advance(int L)
     {
         if (MAX_LEVELS > 1 && (MAX_LEVELS != N_LEVEL_START + 1)) {
@@ -1057,7 +1307,7 @@ struct Oracle {
                     average(L, average_advance());
                     if (plant != 5) interpolate(L, INTERP_INTERFACE);
                 }
-                collide_stream(L);
+                collide_stream(L, mea_on() && L == MAX_LEVELS - 1 && global_Liter == (1 << (MAX_LEVELS - 1)) - 1);
                 if (L == MAX_LEVELS - 1) global_Liter++;
                 if (L < MAX_LEVELS - 1) {
                     advance(L + 1);
@@ -1065,7 +1315,7 @@ struct Oracle {
                 }
             }
         } else {
-            collide_stream(L);
+            collide_stream(L, mea_on());
         }
     }
 
@@ -1790,6 +2040,7 @@ struct Oracle {
     // blocks are flagged as on a boundary.
     void check_masks(int L)
     {
+        if ((long)bcount.size() != n_maxcblocks) bcount.assign(n_maxcblocks, 0);
         const int *ids = idset(L);
         for (int bi = 0; bi < n_ids[L]; bi++) {
             const int b = ids[bi];
@@ -1814,10 +2065,160 @@ struct Oracle {
                 if (near) {
                     cells_mask[cell] = CELL_BOUNDARY;
                     any = true;
+                    bcount[b]++;
                 }
             }
             if (any) onb[b] = 1;
         }
+    }
+
+    // ---------------------------------------------------------------------------------------
+    // Face-cell links.  Mesh::M_Geometry_Voxelize_S3 (mesh_voxelizer.cu:177-240): blocks holding
+    // boundary cells get consecutive ranks in ID order (cblock_ID_onb_solid), the link arrays hold
+    // (ranks rounded up to a multiple of 128) x 64 cells x 27 directions, reset to -1.
+    // ---------------------------------------------------------------------------------------
+    std::vector<int> bcount;      // boundary cells per block (tmp_1)
+    std::vector<int> onb_solid;   // block ID -> rank, -1 for blocks without boundary cells
+    long n_solidb = 0, n_maxcells_b = 0;
+    std::vector<float> X_b;       // link fraction of (rank, cell, direction), -1 = no cut
+    LinkBins lbins;
+    bool links = false;
+    long n_link_corrections = 0;
+    std::vector<T> blk_F;         // cblock_f_Ff: per block, the x / y / z momentum-exchange sums
+
+    void rank_boundary_blocks()
+    {
+        const int idm = id_max[MAX_LEVELS];
+        onb_solid.assign(n_maxcblocks, -1);
+        long n = 0;
+        for (int k = 0; k < idm; k++)
+            if (bcount[k] > 0) onb_solid[k] = (int)n++;
+        n_solidb = ((n + 128) / 128) * 128;
+        n_maxcells_b = n_solidb * M_CBLOCK;
+    }
+
+    float link_x(long cell_b, int q) const { return X_b[(size_t)cell_b + (size_t)q * n_maxcells_b]; }
+
+    // Cu_LinkLengthComputation (mesh_voxelizer_links.cu:25-199), 3D: for the cells of ranked
+    // blocks of level L whose bin holds faces, the smallest... in the reference's comparison: a
+    // candidate distance d in (0, dx + FLT_EPSILON) along c_q replaces the stored value when d is
+    // below the stored *fraction* d_old / dx or none is stored, and the fraction d / dx is stored;
+    // faces come in bin order.  Only BOUNDARY cells write their 26 fractions.
+    void link_lengths(int L)
+    {
+        const int *ids = idset(L);
+        const float dx_L = (float)dxf_vec[L];
+        const float eps = std::numeric_limits<float>::epsilon();
+        const long nt = (long)geo_tri.size() / 9;
+        (void)nt;
+        const int G = lbins.G;
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int bi = 0; bi < n_ids[L]; bi++) {
+            const int b = ids[bi];
+            const int rank = onb_solid[b];
+            if (rank < 0) continue;
+            for (int t = 0; t < M_CBLOCK; t++) {
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                // vec3<float>(X + I*dx + 0.5*dx): the 0.5*dx term is a double (mesh_voxelizer_links.cu:70-74).
+                const V3 vp{(float)((double)(X[b] + (float)I * dx_L) + 0.5 * (double)dx_L),
+                            (float)((double)(X[b + n_maxcblocks] + (float)J * dx_L) + 0.5 * (double)dx_L),
+                            (float)((double)(X[b + 2 * n_maxcblocks] + (float)K * dx_L) + 0.5 * (double)dx_L)};
+                const int bx = (int)(vp.x * (float)G), by = (int)(vp.y * (float)G), bz = (int)(vp.z * (float)G);
+                const long gbin = (long)bx + (long)G * by + (long)G * G * bz;
+                if (gbin < 0 || gbin >= (long)G * G * G) continue;
+                const int f0 = lbins.bin_start[gbin], n_f = lbins.bin_start[gbin + 1] - f0;
+                if (n_f <= 0) continue;
+                float dQ[27];
+                for (int q = 0; q < 27; q++) dQ[q] = -1.0f;
+                for (int k = 0; k < n_f; k++) {
+                    const int f = lbins.faces[f0 + k];
+                    const V3 v1 = face_vertex(geo_tri.data(), f, 0), v2 = face_vertex(geo_tri.data(), f, 1),
+                             v3 = face_vertex(geo_tri.data(), f, 2);
+                    const V3 n = face_normal_unit(v1, v2, v3);
+                    for (int q = 1; q < 27; q++) {
+                        const V3 ray{(float)V.c[0][q], (float)V.c[1][q], (float)V.c[2][q]};
+                        float d = dotv(ray, n);
+                        if (d != 0.0f) {
+                            d = dotv(v1 - vp, n) / d;
+                            if (d > 0.0f && d < dx_L + eps) {
+                                const V3 vi = vp + ray * d;
+                                if (point_in_triangle_ext(vi, v1, v2, v3, n)) {
+                                    const float dist_q = dQ[q];
+                                    if (d < dist_q || dist_q < 0) dQ[q] = d / dx_L;
+                                }
+                            }
+                        }
+                    }
+                }
+                const long cb = (long)rank * M_CBLOCK + t;
+                if (cells_mask[(long)b * M_CBLOCK + t] == CELL_BOUNDARY)
+                    for (int q = 1; q < 27; q++) X_b[(size_t)cb + (size_t)q * n_maxcells_b] = dQ[q];
+            }
+        }
+    }
+
+    // Cu_LinkLengthValidation (mesh_voxelizer_links.cu:200-357), 3D, over the half set: a BOUNDARY
+    // cell with no cut link towards a SOLID partner gets the fraction 1/2 (either side).  The
+    // partner's cell mask is read at (partner's rank) x 64 + cell, the rank standing where a block
+    // ID belongs, as the reference reads it (cells_ID_mask[nbr_kap_b*M_CBLOCK + nbr_kap_c] after
+    // nbr_kap_b = cblock_ID_onb_solid[nbr_kap_b], :268-279); the -1 "corrections" are never written
+    // (only values > 0 are stored).  Blocks run one after another here, in ID-set order.
+    void link_validation(int L)
+    {
+        const int *ids = idset(L);
+        for (int bi = 0; bi < n_ids[L]; bi++) {
+            const int b = ids[bi];
+            const int rank = onb_solid[b];
+            if (rank < 0) continue;
+            int s_nbr[27];
+            load_block_nbrs(b, s_nbr);
+            for (int t = 0; t < M_CBLOCK; t++) {
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                for (int p = 1; p < 27; p++) {
+                    if (!(p == 26 || ((p - 1) % 2 == 0 && p < 25))) continue;
+                    const int Ip = I + V.c[0][p], Jp = J + V.c[1][p], Kp = K + V.c[2][p];
+                    int nb = s_nbr[nbr_map(3, Ip, Jp, Kp)];
+                    const int nc = nbr_cell_id(3, Ip, Jp, Kp);
+                    if (nb > -1) nb = onb_solid[nb];
+                    if (nb <= -1) continue;
+                    const long a = (long)rank * M_CBLOCK + t, o = (long)nb * M_CBLOCK + nc;
+                    const float dQ_p = X_b[(size_t)a + (size_t)p * n_maxcells_b];
+                    const float dQ_pb = X_b[(size_t)o + (size_t)V.pb[p] * n_maxcells_b];
+                    const int mask_p = cells_mask[(long)b * M_CBLOCK + t];
+                    const int mask_pb = cells_mask[(long)nb * M_CBLOCK + nc];
+                    float c_p = -1.0f, c_pb = -1.0f;
+                    if (dQ_p > 0 && mask_pb == CELL_BOUNDARY && dQ_pb > 0) n_link_corrections++;
+                    if (dQ_pb > 0 && mask_p == CELL_BOUNDARY && dQ_p > 0) n_link_corrections++;
+                    if (mask_p == CELL_BOUNDARY && dQ_p < 0 && mask_pb == CELL_SOLID) {
+                        c_p = 0.5f;
+                        n_link_corrections++;
+                    }
+                    if (mask_pb == CELL_BOUNDARY && dQ_pb < 0 && mask_p == CELL_SOLID) {
+                        c_pb = 0.5f;
+                        n_link_corrections++;
+                    }
+                    if (c_p > 0) X_b[(size_t)a + (size_t)p * n_maxcells_b] = c_p;
+                    if (c_pb > 0) X_b[(size_t)o + (size_t)V.pb[p] * n_maxcells_b] = c_pb;
+                }
+            }
+        }
+    }
+
+    // Mesh::M_LinkLengthComputation (mesh_voxelizer.cu:262-296) on the wall levels, after the bins
+    // (Geometry::Bins, geometry_bins.h:197-203).
+    void compute_links()
+    {
+        make_link_bins(lbins, geo_tri.data(), (long)geo_tri.size() / 9, P, V.c);
+        X_b.assign((size_t)n_maxcells_b * 27, -1.0f);
+        n_link_corrections = 0;
+        for (int L = N_LEVEL_START; L < MAX_LEVELS_WALL; L++) {
+            if (n_ids[L] <= 0) continue;
+            link_lengths(L);
+            link_validation(L);
+        }
+        links = true;
     }
 
     // Mesh::M_Advance_RefineNearWall (src/mesh/mesh_advance.cu:53-121).  The benchmark-case hook
@@ -1843,43 +2244,96 @@ struct Oracle {
             if (MAX_LEVELS > 1 && L < MAX_LEVELS_WALL - 1) keep(refine_coarsen(nullptr));
             set_ic(L);
         }
-        if (geom)
+        if (geom) {
+            bcount.assign(n_maxcblocks, 0);
             for (int L = N_LEVEL_START; L < MAX_LEVELS_WALL; L++) check_masks(L);
+            rank_boundary_blocks();
+            if (P.G_BIN_MAKE_LINKS == 1 && DIM == 3 && n_solidb > 0) compute_links();
+        }
         if (MAX_LEVELS > 1) freeze_refined(0);
         return status;
     }
 
-    // Momentum-exchange force on the body (Cu_ComputeForcesMEA order 1, solver_lbm_kernels_forces.
-    // cu:322-540, reported as in S_ReportForces, solver_lbm_compute.cu:165-218), evaluated after a
-    // coarse step: each link p from a BOUNDARY cell of the finest level into a solid cell adds
-    // 2 f*_p c_p, f*_p being the value the link bounced back (now in slot p); the sum times the cell
-    // volume dx^3 and 1/dx of the finest level.  Accumulated in FP64.
-    void forces(double out[3])
+    // Cu_ComputeForcesMEA (solver_lbm_kernels_forces.cu:322-540) on level L: for each BOUNDARY
+    // cell of a ranked block and each half-set pair (p, pb), a link with fraction > 0 adds
+    // f c (order 1) or ((1/2 + q) f + (1/2 - q) f_m) c (order 2), f being the cell's slot p (pb)
+    // and f_m the same slot "behind" it, x - c_p (x + c_p).  Called after the collision
+    // (post_step 0: slot p holds f*_p) and after the stream (post_step 1: slot p holds what arrived,
+    // the reflected value on a wall link).  The per-cell sums (ufloat_t) are reduced over the block
+    // by the reference's shared-memory tree (s = 32, 16, ..., 1), times the cell volume dx^3; step
+    // 0 assigns the block's entry, step 1 adds to it.
+    void forces_mea(int L, int post_step)
     {
-        out[0] = out[1] = out[2] = 0.0;
-        const int L = MAX_LEVELS - 1;
         const int *ids = idset(L);
-        double acc[3] = {0.0, 0.0, 0.0};
+        const T dv = dxf_vec[L] * dxf_vec[L] * dxf_vec[L];
+        const int order = P.S_FORCE_ORDER;
         for (int bi = 0; bi < n_ids[L]; bi++) {
             const int b = ids[bi];
+            const int rank = onb_solid[b];
+            if (rank < 0) continue;
             int s_nbr[27];
             load_block_nbrs(b, s_nbr);
+            T acc[3][64];
             for (int t = 0; t < M_CBLOCK; t++) {
+                acc[0][t] = acc[1][t] = acc[2][t] = (T)0;
                 const long cell = (long)b * M_CBLOCK + t;
                 if (cells_mask[cell] != CELL_BOUNDARY) continue;
                 int I, J, K;
                 cell_ijk(t, I, J, K);
+                const long cb = (long)rank * M_CBLOCK + t;
                 for (int p = 1; p < Q; p++) {
-                    const int Ip = I + V.c[0][p], Jp = J + V.c[1][p], Kp = K + V.c[2][p];
-                    const int nb = s_nbr[nbr_map(DIM, Ip, Jp, Kp)];
-                    if (nb < 0 || cells_mask[(long)nb * M_CBLOCK + nbr_cell_id(DIM, Ip, Jp, Kp)] != CELL_SOLID) continue;
-                    const double f = (double)Fp(p)[cell];
-                    for (int d = 0; d < 3; d++) acc[d] += 2.0 * f * (double)V.c[d][p];
+                    if (!in_half_set(DIM, p)) continue;
+                    const int pb = V.pb[p];
+                    const T f_p = Fp(p)[cell], f_q = Fp(pb)[cell];
+                    for (int side = 0; side < 2; side++) {
+                        const int s_dir = side == 0 ? p : pb;    // the link's direction
+                        const float dist = link_x(cb, s_dir);
+                        if (!(dist > 0)) continue;
+                        const int bd = V.pb[s_dir];              // "behind": x + c_pb(s)
+                        const int Ip = I + V.c[0][bd], Jp = J + V.c[1][bd], Kp = K + V.c[2][bd];
+                        const int nb = s_nbr[nbr_map(DIM, Ip, Jp, Kp)];
+                        const T f = side == 0 ? f_p : f_q;
+                        T f_m = (T)0;
+                        if (nb >= 0) f_m = Fp(s_dir)[(long)nb * M_CBLOCK + nbr_cell_id(DIM, Ip, Jp, Kp)];
+                        for (int d = 0; d < 3; d++) {
+                            if (order == 1) {
+                                acc[d][t] += f * (T)V.c[d][s_dir];
+                            } else {
+                                // The literal 0.5 is a double: the term and the sum are formed in
+                                // double and rounded to ufloat_t by the +=.
+                                const double qd = (double)(T)dist;
+                                acc[d][t] = (T)((double)acc[d][t] +
+                                                ((0.5 + qd) * (double)f + (0.5 - qd) * (double)f_m) * (double)V.c[d][s_dir]);
+                            }
+                        }
+                    }
                 }
             }
+            for (int st = M_CBLOCK / 2; st > 0; st >>= 1)
+                for (int t = 0; t < st; t++)
+                    for (int d = 0; d < 3; d++) acc[d][t] = acc[d][t] + acc[d][t + st];
+            for (int d = 0; d < 3; d++) {
+                T &e = blk_F[(size_t)d * n_maxcblocks + b];
+                e = post_step == 0 ? acc[d][0] * dv : e + acc[d][0] * dv;
+            }
         }
-        const double dxf = (double)dxf_vec[L];
-        for (int d = 0; d < 3; d++) out[d] = acc[d] * (dxf * dxf * dxf) / dxf;
+    }
+
+    bool mea_on() const { return links && P.S_FORCE_TYPE == 0 && DIM == 3; }
+
+    // S_ReportForces (solver_lbm_compute.cu:165-218), force type 0: the block sums over IDs below
+    // id_max, summed in FP64 in ID order, times 1/dx of the finest level; the sums are then reset.
+    void report_forces(double out[3])
+    {
+        out[0] = out[1] = out[2] = 0.0;
+        const double factor = 1.0 / (double)dxf_vec[MAX_LEVELS - 1];
+        const int idm = id_max[MAX_LEVELS];
+        for (int d = 0; d < 3; d++) {
+            double acc = 0.0;
+            for (int k = 0; k < idm; k++) acc += (double)blk_F[(size_t)d * n_maxcblocks + k];
+            out[d] = factor * acc;
+        }
+        std::fill(blk_F.begin(), blk_F.end(), (T)0);
     }
 
     // The body of Mesh::M_AdvanceLoop (src/mesh/mesh_advance.cu:421-463) for coarse steps [i0,i0+n).

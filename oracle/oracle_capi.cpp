@@ -143,8 +143,51 @@ int oracle_voxelize(void *h, int L)
         return AGAL_OK;
     });
 }
+int oracle_download_links(void *h, int *onb_solid, long long n_blocks, float *X_b, long long n_values,
+                          long long *info)
+{
+    return dispatch(h, [&](auto &O) {
+        if (n_blocks < 0 || n_blocks > O.n_maxcblocks) return AGAL_ERR_ARG;
+        if (info) {
+            info[0] = O.n_solidb;
+            info[1] = O.n_link_corrections;
+            info[2] = O.links ? 1 : 0;
+        }
+        if (onb_solid)
+            for (long long k = 0; k < n_blocks; k++) onb_solid[k] = O.onb_solid.empty() ? -1 : O.onb_solid[k];
+        if (X_b) {
+            if (n_values > (long long)O.X_b.size()) return AGAL_ERR_ARG;
+            std::copy(O.X_b.begin(), O.X_b.begin() + n_values, X_b);
+        }
+        return AGAL_OK;
+    });
+}
+
+int oracle_download_link_bins(void *h, int *bin_start, long long n_start, int *faces, long long n_faces,
+                              long long *info)
+{
+    return dispatch(h, [&](auto &O) {
+        const auto &B = O.lbins;
+        if (info) {
+            info[0] = (long long)B.bin_start.size() - 1;
+            info[1] = (long long)B.faces.size();
+            info[2] = B.G;
+            info[3] = (long long)B.filtered.size();
+        }
+        if (bin_start) {
+            if (n_start > (long long)B.bin_start.size()) return AGAL_ERR_ARG;
+            std::copy(B.bin_start.begin(), B.bin_start.begin() + n_start, bin_start);
+        }
+        if (faces) {
+            if (n_faces > (long long)B.faces.size()) return AGAL_ERR_ARG;
+            std::copy(B.faces.begin(), B.faces.begin() + n_faces, faces);
+        }
+        return AGAL_OK;
+    });
+}
+
 int oracle_check_masks(void *h, int L) { return dispatch(h, [&](auto &O) { O.check_masks(L); return AGAL_OK; }); }
-int oracle_forces(void *h, double *out) { return dispatch(h, [&](auto &O) { O.forces(out); return AGAL_OK; }); }
+int oracle_forces(void *h, double *out) { return dispatch(h, [&](auto &O) { O.report_forces(out); return AGAL_OK; }); }
 /* Crossing count of the +x ray from (px, py, pz) with n_tri triangles (the voxelizer's predicate). */
 long long oracle_ray_count(const float *tri, long long n_tri, double px, double py, double pz)
 {

@@ -218,3 +218,113 @@ def check_state(st: dict, dim: int, n_root: tuple, periodic=(0, 0, 0), n_max: in
                 bad.append(f"2:1 balance violated next to block {b} (level {L})")
                 break
     return bad
+
+
+def check_state_fast(st: dict, n_root: tuple, n_max: int | None = None) -> list:
+    """check_state's invariants for a 3D, non-periodic pool, vectorized over blocks (for pools of
+    hundreds of thousands of blocks): ID sets and gap stack, unique lattice positions per level,
+    the neighbour table against a position lookup (domain faces by x, y, z precedence, missing
+    same-level neighbours N_SKIPID), the neighbour-child rule, children of refined blocks in place,
+    the reference's validators 1, 2 and 4, and 2:1 balance."""
+    bad = []
+    c = velocities(3)
+    ref, level, nbr, nbc = st["ref"], st["level"], st["nbr"], st["nbr_child"]
+    idm = st["id_max"]
+    active = ref != capi.REF_INACTIVE
+    ids = np.nonzero(active)[0]
+    # ID sets and gaps.
+    seen = np.zeros(idm, dtype=bool)
+    for L, s in enumerate(st["id_sets"]):
+        if s.size and not np.all(np.diff(s) > 0):
+            bad.append(f"id_set[{L}] not strictly ascending")
+        if s.size and (np.any(~active[s]) or np.any(level[s] != L)):
+            bad.append(f"id_set[{L}] holds inactive blocks or blocks of another level")
+        seen[s] = True
+    if not np.array_equal(seen, active):
+        bad.append("id sets do not cover exactly the active blocks")
+    if n_max is not None:
+        allids = np.concatenate([st["gap_set"], ids])
+        if allids.size != n_max or np.unique(allids).size != n_max or allids.min() < 0 or allids.max() >= n_max:
+            bad.append("gap stack and active set do not partition the pool")
+    # Integer positions: block coordinate / block width of its level.
+    X = st["coords"][:, :idm].astype(np.float64)
+    roots = ids[level[ids] == 0]
+    xs = np.unique(X[0, roots])
+    h0 = xs[1] - xs[0]
+    lev = level[ids].astype(np.int64)
+    ijk = np.rint(X[:, ids] / (h0 / (1 << lev))[None, :]).astype(np.int64)
+    span = np.int64(1) << 16  # positions + 1 per axis stay below 2^16 (256 root blocks x 2^7)
+
+    def keyof(L, a):
+        return ((L * span + (a[0] + 1)) * span + (a[1] + 1)) * span + (a[2] + 1)
+
+    keys = keyof(lev, ijk)
+    order = np.argsort(keys)
+    ks = keys[order]
+    if np.any(ks[1:] == ks[:-1]):
+        bad.append("two active blocks share a position")
+
+    def lookup(L, a):
+        k = keyof(L, a)
+        pos = np.searchsorted(ks, k)
+        pos = np.minimum(pos, ks.size - 1)
+        hit = ks[pos] == k
+        return np.where(hit, ids[order[pos]], N_SKIPID)
+
+    ext = np.stack([np.int64(n_root[a]) << lev for a in range(3)])
+    for p in range(27):
+        loc = ijk + c[:, p][:, None]
+        exp = lookup(lev, loc)
+        # Faces: x over y over z (mesh_init.cu:406-460).
+        for a, (lo, hi) in reversed(list(enumerate(((-1, -2), (-3, -4), (-5, -6))))):
+            exp = np.where(loc[a] < 0, lo, np.where(loc[a] >= ext[a], hi, exp))
+        got = nbr[p, ids]
+        if not np.array_equal(got, exp):
+            k = np.nonzero(got != exp)[0][:3]
+            bad.append(f"nbr[{p}] differs at blocks {ids[k].tolist()}: {got[k].tolist()} vs {exp[k].tolist()}")
+        # neighbour-child rule
+        ec = np.where(got >= 0, np.where(nbc[0, np.maximum(got, 0)] >= 0, nbc[0, np.maximum(got, 0)], N_SKIPID), got)
+        if p == 0:
+            ec = nbc[0, ids]
+        if not np.array_equal(nbc[p, ids], ec):
+            bad.append(f"nbr_child[{p}] breaks the first-child rule")
+        # validator 2
+        refined = np.isin(ref[np.maximum(got, 0)], [capi.REF_REFINED, capi.REF_REFINED_WCHILD, capi.REF_REFINED_PERM,
+                                                    capi.REF_REFINED_WCHILD_PERM, capi.REF_MARK_REFINE])
+        if np.any((got > -1) & refined & (nbc[p, ids] == N_SKIPID)):
+            bad.append(f"validator 2 fails in direction {p}")
+    # Children in place; validators 1 and 4.
+    fc = nbc[0, ids]
+    par = ids[fc >= 0]
+    if par.size:
+        f0 = nbc[0, par]
+        if f0.max() + 7 >= idm:
+            bad.append("child id beyond id_max")
+        else:
+            pij = ijk[:, np.searchsorted(ids, par)]
+            for k in range(8):
+                kid = f0 + k
+                off = np.array([k % 2, (k // 2) % 2, (k // 4) % 2])[:, None]
+                if np.any(~active[kid]) or np.any(level[kid] != level[par] + 1):
+                    bad.append("children not active one level down")
+                    break
+                kij = ijk[:, np.searchsorted(ids, kid)]
+                if not np.array_equal(kij, 2 * pij + off):
+                    bad.append(f"child {k} of some block out of place")
+                    break
+    r = ref[ids]
+    if np.any((r == capi.REF_REFINED) & (fc < 0)) or np.any((r == capi.REF_UNREFINED) & (fc > -1)):
+        bad.append("validator 1 fails")
+    # 2:1 balance: every in-domain neighbour position of a level-L block (L >= 2) lies in a refined
+    # block of level L-2.
+    sel = lev >= 2
+    if sel.any():
+        for p in range(1, 27):
+            loc = ijk[:, sel] + c[:, p][:, None]
+            inside = np.all((loc >= 0) & (loc < ext[:, sel]), axis=0)
+            g = lookup(lev[sel] - 2, loc >> 2)
+            ok = ~inside | ((g >= 0) & (nbc[0, np.maximum(g, 0)] >= 0))
+            if not np.all(ok):
+                bad.append(f"2:1 balance violated (direction {p})")
+                break
+    return bad

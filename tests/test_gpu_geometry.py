@@ -2,9 +2,9 @@
 boundary cells, the sphere flow's steps (inflow / outflow hook, bounce-back on the body) and the
 momentum-exchange force, against the CPU restatement (which tests every triangle per ray).
 
-Integer state and populations must be identical; the force is an FP64 sum whose order differs
-between the two (per-wavefront tree on the GPU, cell order on the CPU), so it is compared to a
-relative 1e-9."""
+Integer state and populations must be identical; the force's per-block sums are too, and their
+FP64 total is summed in another order on the GPU (strided lanes and a tree) than on the CPU (block
+order), so it is compared to a relative 1e-9."""
 from __future__ import annotations
 
 import os
@@ -53,7 +53,9 @@ def test_sphere_flow_near_wall_hierarchy_steps_and_forces(vs, prec, coll, N):
     """The body of the sphere-flow workload (radius 0.08 L at x = 0.3 L, +-5% noise) on a small
     root grid with three wall levels; three rounds of two coarse steps."""
     tri = body(center=(0.3, 0.5, 0.5), radius=0.08, subdivisions=3, noise=0.05, seed=1)
-    p = sphere_params(vs=vs, N=N, levels=3, prec=prec, collision=coll, D=0.16)
+    # Plain bounce-back on the body; face-cell links for the first-order momentum exchange.
+    p = sphere_params(vs=vs, N=N, levels=3, prec=prec, collision=coll, D=0.16, G_BIN_MAKE_LINKS=1, G_BIN_DENSITY=4,
+                      G_BIN_LEVELS=3, G_BIN_SPEC=1, S_BC_TYPE=0, S_FORCE_TYPE=0, S_FORCE_ORDER=1)
     g, o = pair(p, 20000, tri)
     sg = g.M_Advance_RefineNearWall()
     so = o.M_Advance_RefineNearWall()

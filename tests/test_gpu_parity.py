@@ -202,6 +202,29 @@ def test_stream_guard_exceptions():
     assert_same(g, o)
 
 
+def test_exception_list_overflow_is_reported_and_stops_the_context():
+    """With the exception list bounded to 4 entries (host option), the perturbed state of the guard
+    test overflows it: the list keeps its bound (kernels only skip the extra entries), the next sync
+    reports AGAL_ERR_OVERFLOW, and the context then refuses further steps."""
+    p = ldc_params(vs=capi.D3Q19, N=8, Re=10.0, levels=1)
+    g = LbmMesh(p, 64)
+    g.S_SetIC(0)
+    f = g.populations()
+    rng = np.random.default_rng(3)
+    g.set_populations(f * (1.0 + 1.5 * rng.standard_normal(f.shape).astype(f.dtype)))
+    g.set_option(capi.OPT_EXC_CAP, 4)
+    g.S_Advance(0)
+    with pytest.raises(capi.AgalError) as e:
+        g.sync()
+    assert e.value.status == capi.AGAL_ERR_OVERFLOW
+    with pytest.raises(capi.AgalError) as e:
+        g.S_Advance(0)
+    assert e.value.status == capi.AGAL_ERR_STATE
+    with pytest.raises(capi.AgalError):
+        g.set_option(capi.OPT_EXC_CAP, 1 << 40)  # above the allocated capacity
+    g.close()
+
+
 @pytest.mark.parametrize("vs,prec", [(capi.D3Q19, capi.FP32), (capi.D3Q19, capi.FP16), (capi.D3Q19, capi.BF16),
                                      (capi.D3Q27, capi.FP32)])
 def test_stream_guard_exceptions_across_levels(vs, prec):

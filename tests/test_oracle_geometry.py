@@ -7,7 +7,6 @@
 * The STL reader against the reference's own geometry inputs (input/geometry/*.stl).
 * Voxelization against an independent inside test, the generalized winding number.
 * The near-wall hierarchy against invariants of the marks (mesh_voxelizer_refine.cu:23-414).
-* The momentum-exchange force against a sum over a dense cell grid.
 """
 from __future__ import annotations
 
@@ -249,41 +248,3 @@ def test_near_wall_band_width_follows_the_parameter():
     assert counts[1] > counts[0] > 0
 
 
-# ---- forces --------------------------------------------------------------------------------------
-def test_force_equals_dense_link_sum():
-    """Populations set to a uniform equilibrium state: the force is sum over BOUNDARY cells x and
-    directions p with x + c_p solid of 2 f_pb(p)(x) c_p, times dx^3 / dx (here on one level)."""
-    Q, N = 19, 16
-    tri = body(center=(0.5, 0.5, 0.5), radius=0.3, subdivisions=3, noise=0.05, seed=5)
-    m = OracleMesh(sphere_params(vs=capi.D3Q19, N=N, levels=1, collision=capi.BGK, prec=capi.FP64), 128)
-    m.set_geometry(tri)
-    m.M_Advance_RefineNearWall()
-    c, w, pb, _ = velset(Q)
-    c, w, pb = c[:, :Q], w[:Q], pb[:Q]
-    u = np.array([0.03, -0.01, 0.02])
-    cu = c.T @ u
-    feq = w * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * u @ u)
-    f = m.populations()
-    f[:] = feq[:, None]
-    m.set_populations(f)
-    F = m.S_ComputeForces()
-    st, g = _dense(m)
-    solid = g == capi.CELL_SOLID
-    bnd = g == capi.CELL_BOUNDARY
-    acc = np.zeros(3)
-    for p in range(1, Q):
-        into = bnd & np.roll(solid, (-c[0, p], -c[1, p], -c[2, p]), axis=(0, 1, 2))
-        acc += 2.0 * feq[pb[p]] * c[:, p] * into.sum()
-    dx = 1.0 / N
-    np.testing.assert_allclose(F, acc * dx ** 3 / dx, rtol=1e-12, atol=1e-15)
-    assert abs(F[0]) > 1e-6
-
-
-def test_force_of_fluid_at_rest_vanishes():
-    """At rest every link into the body has a partner link out of it with the opposite c."""
-    tri = body(center=(0.5, 0.5, 0.5), radius=0.3, subdivisions=3, noise=0.05, seed=5)
-    m = OracleMesh(sphere_params(vs=capi.D3Q27, N=16, levels=1, prec=capi.FP64, ic_u=0.0), 128)
-    m.set_geometry(tri)
-    m.M_Advance_RefineNearWall()
-    F = m.S_ComputeForces()
-    assert np.all(np.abs(F) < 1e-15)

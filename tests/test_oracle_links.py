@@ -1,0 +1,285 @@
+"""The face-cell link part of the CPU restatement, pinned without a GPU.
+
+* Face bins (G_MakeBinsGPU_MD): every (bin, face) pair against an exact triangle / box overlap
+  test (Sutherland-Hodgman clipping of the triangle against the grown box, in double), and the
+  order inside a bin (overlap rank, then face).
+* The ray-indicator filter: a kept face is met by a cell-centred line, a dropped one by none,
+  decided in double away from edges.
+* Cut-link fractions against the analytic ray / sphere intersection for a fine icosphere.
+* The interpolated bounce-back and the momentum-exchange force on a plane Couette flow over an
+  off-lattice wall (a slab geometry): the wall sits where the geometry puts it (half-way for plain
+  bounce-back), the shear force equals rho nu U / H times the area, and the normal force equals
+  the lattice pressure rho / 3 times the area; at rest the two force orders agree.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from agal_amd import capi
+from oracle_lib import OracleMesh, set_threads, velset
+from parity import body, sphere_params
+
+
+def box_triangles(x0, x1, y0, y1, z0, z1):
+    v = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], np.float32)
+    f = [(0, 2, 1), (0, 3, 2), (4, 5, 6), (4, 6, 7), (0, 1, 5), (0, 5, 4),
+         (2, 3, 7), (2, 7, 6), (1, 2, 6), (1, 6, 5), (0, 4, 7), (0, 7, 3)]
+    return np.array([[v[a], v[b], v[c]] for a, b, c in f], np.float32)
+
+
+def clip_overlaps(tri, lo, hi) -> bool:
+    """Exact: does the closed triangle meet the closed box [lo, hi]?  Clip the polygon by the six
+    half-spaces in double; a non-empty remainder means overlap."""
+    poly = [np.asarray(v, dtype=np.float64) for v in tri]
+    for axis in range(3):
+        for side, bound in ((1, lo[axis]), (-1, hi[axis])):
+            out = []
+            n = len(poly)
+            for i in range(n):
+                a, b = poly[i], poly[(i + 1) % n]
+                da, db = side * (a[axis] - bound), side * (b[axis] - bound)
+                if da >= 0:
+                    out.append(a)
+                if (da >= 0) != (db >= 0):
+                    t = da / (da - db)
+                    out.append(a + t * (b - a))
+            poly = out
+            if not poly:
+                return False
+    return True
+
+
+def link_params(**kw):
+    base = dict(vs=capi.D3Q19, N=16, levels=1, collision=capi.BGK, G_BIN_MAKE_LINKS=1, G_BIN_DENSITY=4,
+                G_BIN_LEVELS=1, G_BIN_SPEC=1, S_BC_TYPE=1, S_FORCE_TYPE=0, S_FORCE_ORDER=2)
+    base.update(kw)
+    return sphere_params(**base)
+
+
+def test_link_bins_match_exact_overlap():
+    set_threads(8)
+    tri = body(center=(0.47, 0.52, 0.5), radius=0.28, subdivisions=2, noise=0.05, seed=3)
+    # G_BIN_LEVELS below MAX_LEVELS_WALL: bins on level 0, every face binned (no ray filter).
+    p = link_params(levels=2, G_BIN_LEVELS=1, G_BIN_DENSITY=4, G_BIN_SPEC=1)
+    m = OracleMesh(p, 8192)
+    m.set_geometry(tri)
+    m.M_Advance_RefineNearWall()
+    B = m.link_bins()
+    G = B["G"]
+    assert G == 4 and B["kept"] == tri.shape[0]
+    n_limits = 27
+    w = np.float32(1.0) / np.float32(G)
+    dx = np.float32(1.0 / 16) + np.float32(np.finfo(np.float32).eps)
+    expect = {}
+    for f, t in enumerate(tri):
+        lo, hi = t.min(0), t.max(0)
+        rng = [range(int(lo[a] * G) - 1, int(hi[a] * G) + 2) for a in range(3)]
+        counter = 0
+        for K in rng[2]:
+            for J in rng[1]:
+                for I in rng[0]:
+                    blo = np.array([I * w - dx, J * w - dx, K * w - dx], np.float64)
+                    bhi = np.array([(I + 1) * w + dx, (J + 1) * w + dx, (K + 1) * w + dx], np.float64)
+                    if counter < n_limits and clip_overlaps(t, blo, bhi):
+                        gid = I + G * J + G * G * K
+                        if 0 <= gid < G ** 3:
+                            expect.setdefault(gid, []).append((counter, f))
+                        counter += 1
+    start, faces = B["start"], B["faces"]
+    assert start.size == G ** 3 + 1 and start[-1] == faces.size
+    for gid in range(G ** 3):
+        got = faces[start[gid]:start[gid + 1]].tolist()
+        want = [f for _, f in sorted(expect.get(gid, []))]
+        assert got == want, (gid, got, want)
+    assert faces.size > 2 * tri.shape[0]
+
+
+def _line_hits_face(t, G_cells, dxL):
+    """Double-precision ray-indicator decision of one face (and whether it was close to an edge)."""
+    c, _, _, _ = velset(27)
+    half = [p for p in range(1, 27) if p == 26 or ((p - 1) % 2 == 0 and p < 25)]
+    v1, v2, v3 = (t.astype(np.float64))
+    n = np.cross(v2 - v1, v3 - v1)
+    n /= np.linalg.norm(n)
+    lo = (t.min(0) * G_cells).astype(int)
+    hi = (t.max(0) * G_cells).astype(int)
+    near = False
+    for p in half:
+        ray = c[:, p].astype(np.float64)
+        den = ray @ n
+        if abs(den) < 1e-9:
+            near = True
+            continue
+        for iz in range(lo[2], hi[2] + 1):
+            for iy in range(lo[1], hi[1] + 1):
+                for ix in range(lo[0], hi[0] + 1):
+                    vp = dxL * 0.5 + dxL * np.array([ix, iy, iz], np.float64)
+                    vi = vp + ray * ((v1 - vp) @ n / den)
+                    s = []
+                    for a, b in ((v1, v2), (v2, v3), (v3, v1)):
+                        e = np.cross(b - a, n)
+                        s.append((a - vi) @ (e / np.linalg.norm(e)))
+                    if min(s) > 1e-6:
+                        return True, near
+                    if min(s) > -1e-6:
+                        near = True
+    return False, near
+
+
+def test_ray_indicator_filter():
+    tri = body(center=(0.5, 0.5, 0.5), radius=0.2, subdivisions=3, noise=0.05, seed=11)
+    p = link_params(levels=1, G_BIN_LEVELS=1, N=16)
+    m = OracleMesh(p, 4096)
+    m.set_geometry(tri)
+    m.M_Advance_RefineNearWall()
+    kept = m.link_bins()["kept"]
+    hit, ambiguous = 0, 0
+    for t in tri:
+        h, near = _line_hits_face(t, 16, 1.0 / 16)
+        hit += h
+        ambiguous += near and not h
+    assert hit <= kept <= hit + ambiguous
+    assert 0 < kept < tri.shape[0]
+
+
+def test_link_fractions_match_analytic_sphere():
+    set_threads(8)
+    R, C0, N = 0.3, np.array([0.5, 0.5, 0.5]), 32
+    tri = capi.icosphere(tuple(C0), R, 5, 0.0, 0)  # facets within 6e-5 of the sphere: 0.002 dx
+    p = link_params(N=N, levels=1, G_BIN_DENSITY=8, G_BIN_LEVELS=1, G_BIN_SPEC=1)
+    m = OracleMesh(p, 4096)
+    m.set_geometry(tri)
+    m.M_Advance_RefineNearWall()
+    L = m.links()
+    st = m.state()
+    X = m.coords()
+    c, _, _, _ = velset(27)
+    dx = 1.0 / N
+    errs, missing, cut_exact = [], 0, 0
+    onb = L["onb_solid"]
+    for b in np.nonzero(onb >= 0)[0]:
+        for t in range(64):
+            if st["cell_mask"][b * 64 + t] != capi.CELL_BOUNDARY:
+                continue
+            I, J, K = t % 4, (t // 4) % 4, t // 16
+            x = X[:, b].astype(np.float64) + (np.array([I, J, K]) + 0.5) * dx
+            for q in range(1, 27):
+                cq = c[:, q].astype(np.float64)
+                # |x + s c - C| = R for the smallest s in (0, dx]
+                a2, b2, c2 = cq @ cq, 2 * cq @ (x - C0), (x - C0) @ (x - C0) - R * R
+                disc = b2 * b2 - 4 * a2 * c2
+                s_exact = None
+                if disc >= 0:
+                    r = sorted([(-b2 - np.sqrt(disc)) / (2 * a2), (-b2 + np.sqrt(disc)) / (2 * a2)])
+                    s_exact = next((s for s in r if 0 < s <= dx), None)
+                got = L["X_b"][q, onb[b] * 64 + t]
+                if s_exact is not None and s_exact / dx < 0.999:
+                    cut_exact += 1
+                    if got > 0 and got != 0.5:
+                        errs.append(abs(got - s_exact / dx))
+                    elif got <= 0:
+                        missing += 1
+    errs = np.array(errs)
+    assert cut_exact > 2000
+    assert errs.size > 0.95 * cut_exact
+    assert np.median(errs) < 3e-3 and np.quantile(errs, 0.99) < 0.02, (np.median(errs), np.quantile(errs, 0.99))
+    assert missing <= 0.01 * cut_exact
+
+
+def test_link_fractions_match_nearest_face_brute_force():
+    """Every stored fraction that the validation did not set against the nearest crossing of the
+    link with any face of the polyhedron (double, all faces, no bins)."""
+    set_threads(8)
+    N = 16
+    tri = body(center=(0.48, 0.51, 0.5), radius=0.3, subdivisions=3, noise=0.05, seed=21)
+    p = link_params(N=N, levels=1, G_BIN_DENSITY=4, G_BIN_LEVELS=1)
+    m = OracleMesh(p, 4096)
+    m.set_geometry(tri)
+    m.M_Advance_RefineNearWall()
+    L, st, X = m.links(), m.state(), m.coords()
+    c, _, _, _ = velset(27)
+    dx = 1.0 / N
+    T = tri.astype(np.float64)
+    v1, v2, v3 = T[:, 0], T[:, 1], T[:, 2]
+    nrm = np.cross(v2 - v1, v3 - v1)
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    onb = L["onb_solid"]
+    checked = 0
+    for b in np.nonzero(onb >= 0)[0]:
+        for t in range(64):
+            if st["cell_mask"][b * 64 + t] != capi.CELL_BOUNDARY:
+                continue
+            I, J, K = t % 4, (t // 4) % 4, t // 16
+            x = X[:, b].astype(np.float64) + (np.array([I, J, K]) + 0.5) * dx
+            for q in range(1, 27):
+                cq = c[:, q].astype(np.float64)
+                den = nrm @ cq
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    d = ((v1 - x) * nrm).sum(1) / den
+                    pi = x + d[:, None] * cq
+                    inside = np.ones(len(T), bool)
+                    for a, bb in ((v1, v2), (v2, v3), (v3, v1)):
+                        e = np.cross(bb - a, nrm)
+                        inside &= ((a - pi) * e).sum(1) >= -1e-7 * np.linalg.norm(e, axis=1)
+                ok = (den != 0) & (d > 0) & (d < dx * (1 + 1e-6)) & inside
+                got = L["X_b"][q, onb[b] * 64 + t]
+                if got == 0.5:
+                    continue
+                if ok.any():
+                    ds = np.sort(d[ok])
+                    # several faces met at (almost) one point: an edge or vertex -- either may win
+                    assert got > 0 and np.min(np.abs(ds - got * dx)) / dx < 1e-5, (b, t, q, got, ds / dx)
+                    checked += 1
+                else:
+                    assert got < 0, (b, t, q, got)
+    assert checked > 500
+
+
+def _slab_couette(zw, bc, order=2, N=16, steps=5000, U=0.01, tau=0.8):
+    dx = 1.0 / N
+    p = capi.default_params(velocity_set=capi.D3Q19, Nx=N, L_fy=0.25, MAX_LEVELS_WALL=1, MAX_LEVELS_INTERIOR=1,
+                            v0=(tau - 0.5) / 3 * dx, bc_preset=capi.BC_LDC3D, bc_velocity=U, PERIODIC_X=1,
+                            PERIODIC_Y=1, P_REFINE=1 << 30, G_BIN_MAKE_LINKS=1, G_BIN_DENSITY=4, G_BIN_LEVELS=1,
+                            G_BIN_SPEC=1, S_BC_TYPE=bc, S_FORCE_TYPE=0, S_FORCE_ORDER=order)
+    m = OracleMesh(p, 4096)
+    m.set_geometry(box_triangles(-0.1, 1.1, -0.1, 1.1, -0.1, zw))
+    m.M_Advance_RefineNearWall()
+    m.advance_loop(0, steps)
+    m.S_ComputeForces()
+    m.advance_loop(steps, 1)
+    F = m.S_ComputeForces()
+    m.S_RefreshVariables(0)
+    mac, X = m.macros(), m.coords()
+    z = (X[2][:, None] + (np.arange(64)[None, :] // 16 + 0.5) * dx).ravel()
+    zs = np.unique(np.round(z, 6))
+    prof = np.array([mac[1][np.abs(z - zz) < 1e-6].mean() for zz in zs])
+    sel = (zs > zw + 0.1) & (zs < 0.9)
+    a, b0 = np.polyfit(zs[sel], prof[sel], 1) if U != 0 else (1.0, 0.0)
+    nu = (tau - 0.5) / 3
+    area = 0.25
+    return -b0 / a, F, nu * U * dx * area / (1.0 - zw), area
+
+
+@pytest.mark.parametrize("zw", [0.19, 0.21, 0.17])  # fractions 0.46, 0.14 (linear form), 0.78 (second form)
+def test_interpolated_bounce_back_places_the_wall(zw):
+    set_threads(8)
+    N = 16
+    z0, F, Fx, area = _slab_couette(zw, bc=1)
+    assert abs(z0 - zw) * N < 0.01, (z0, zw)
+    assert abs(F[0] / Fx - 1) < 2e-3, (F[0], Fx)
+    assert abs(F[1]) < 1e-9
+    assert abs(F[2] / (-area / 3) - 1) < 1e-3, F[2]
+    # Plain bounce-back: the wall half-way between the last solid and the first fluid cell centre.
+    z0b, Fb, _, _ = _slab_couette(zw, bc=0)
+    k_fluid = int(np.ceil(zw * N - 0.5))
+    assert abs(z0b - k_fluid / N) * N < 0.01, (z0b, k_fluid / N)
+
+
+def test_force_orders_agree_at_rest():
+    set_threads(8)
+    _, F1, _, area = _slab_couette(0.19, bc=1, order=1, U=0.0, steps=10)
+    _, F2, _, _ = _slab_couette(0.19, bc=1, order=2, U=0.0, steps=10)
+    np.testing.assert_allclose(F1, F2, rtol=1e-6, atol=1e-12)
+    assert abs(F1[2] / (-area / 3) - 1) < 1e-6 and abs(F1[0]) < 1e-12 and abs(F1[1]) < 1e-12
