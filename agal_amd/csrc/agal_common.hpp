@@ -343,14 +343,37 @@ struct Store {  // S == T: the pool holds f
     static constexpr bool quantized = false;
 };
 
+// Binary16 bits of the largest half value at or below the negative float c (round toward -inf),
+// for |c| in the normal binary16 range: the truncated magnitude, plus one unit when bits were cut.
+constexpr uint16_t half_round_down_neg(float c)
+{
+    const uint32_t b = __builtin_bit_cast(uint32_t, c);
+    const uint32_t e16 = ((b >> 23) & 0xffu) - 127u + 15u, m = b & 0x7fffffu;
+    uint32_t h = 0x8000u | (e16 << 10) | (m >> 13);
+    if (m & 0x1fffu) h += 1u;
+    return (uint16_t)h;
+}
+
 template <>
 struct Store<float, _Float16> {
-    __device__ static float dec(_Float16 s, float w) { return (float)s + w; }
+    // (float)s + w: one mixed-precision FMA, s * 1 + w with the exact half-to-float conversion of
+    // s and one rounding -- the conversion and the add.
+    __device__ static float dec(_Float16 s, float w)
+    {
+        float r;
+        const uint32_t bits = __builtin_bit_cast(uint16_t, s);
+        asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(bits), "v"(w));
+        return r;
+    }
     __device__ static _Float16 enc(float f, float w) { return (_Float16)(f - w); }
     // dec(s, w) > 0 without the add: a rounded float sum has the sign of the exact sum (a nonzero
-    // exact sum of these two is a multiple of 2^-133, far from underflow), and that sign is the
-    // exact comparison below.
-    __device__ static bool positive(_Float16 s, float w) { return (float)s > -w; }
+    // exact sum of these two is a multiple of 2^-133, far from underflow), so the test is
+    // (float)s > -w, and with t the largest half at or below -w that is s > t, one half compare
+    // (no half lies strictly between t and -w).
+    __device__ static bool positive(_Float16 s, float w)
+    {
+        return s > __builtin_bit_cast(_Float16, half_round_down_neg(-w));
+    }
     static constexpr bool quantized = true;
 };
 

@@ -142,6 +142,7 @@ struct Ctx {
     StepGraph graphs[8];
     unsigned mesh_epoch = 1;
     bool use_graphs = true;
+    int graph_steps = 4;    // coarse steps per recorded graph at most (agal_advance_loop)
     bool mrt_mfma = false;  // MRT moment transform on matrix cores (FP32 compute, D3Q19 / D3Q27)
 
     // Scheduler used while a step is being recorded.  Resources are (level, buffer parity) pairs,
@@ -209,6 +210,9 @@ int launch_average_interpolate(Ctx *c, int L);
 // `label` (may be null): per block ID, 1 when the criterion wants the block to have children, 0
 // when not, -1 for inactive IDs (the learned predictor's training target).
 int launch_ref_criteria(Ctx *c, int *label = nullptr);
+int marks_refine_only(Ctx *c);
+int count_refine_marks(Ctx *c, int *n);
+int marks_dilate(Ctx *c);
 int refine_coarsen(Ctx *c, int out_counts[4]);
 int rebuild_level_lists(Ctx *c, long long range_hint);
 int launch_fill(Ctx *c, int *p, long long n, int v);

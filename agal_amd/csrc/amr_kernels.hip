@@ -646,6 +646,67 @@ static int criterion_t(Ctx *c, int *label)
     return AGAL_OK;
 }
 
+// Controls of the learned-refinement study (no reference counterpart).  Refine-only marks: a
+// coarsening mark of the criterion reverts to the REFINED state it replaced.
+__global__ void k_unmark_coarsen(int *ref, int n)
+{
+    const int i = blockIdx.x * AWG + threadIdx.x;
+    if (i < n && ref[i] == REF_MARK_COARSEN) ref[i] = REF_REFINED;
+}
+
+// One-block dilation of the refinement marks: an unrefined block below the interior maximum level,
+// with no missing neighbour (the criterion's eligibility), next to a block marked in `snap` (the
+// marks before the pass) is marked too.  Same-level neighbours only.
+__global__ void k_dilate_refine(const int *snap, int *ref, const int *level, const int *nbr, long long nmax, int n,
+                                int mli)
+{
+    const int b = blockIdx.x * AWG + threadIdx.x;
+    if (b >= n || snap[b] != REF_UNREFINED || level[b] >= mli - 1) return;
+    bool near = false;
+    for (int p = 0; p < 27; p++) {
+        const int nb = nbr[b + p * nmax];
+        if (nb == N_SKIPID) return;
+        if (nb >= 0 && snap[nb] == REF_MARK_REFINE) near = true;
+    }
+    if (near) ref[b] = REF_MARK_REFINE;
+}
+
+int marks_refine_only(Ctx *c)
+{
+    const int idm = c->id_max[c->MAX_LEVELS];
+    if (idm <= 0) return AGAL_OK;
+    hipLaunchKernelGGL(k_unmark_coarsen, dim3(grid1(idm)), dim3(AWG), 0, c->stream, c->ref, idm);
+    AGAL_HIP(c, hipGetLastError());
+    return AGAL_OK;
+}
+
+static int read_scalars(Ctx *c, int n);
+
+int count_refine_marks(Ctx *c, int *n)
+{
+    *n = 0;
+    const int idm = c->id_max[c->MAX_LEVELS];
+    if (idm <= 0) return AGAL_OK;
+    int st;
+    if ((st = launch_fill(c, c->dscal, 2, 0))) return st;
+    hipLaunchKernelGGL(k_count_marks, dim3(grid1(idm)), dim3(AWG), 0, c->stream, c->ref, idm, c->dscal);
+    AGAL_HIP(c, hipGetLastError());
+    if ((st = read_scalars(c, 1))) return st;
+    *n = c->hscal[0];
+    return AGAL_OK;
+}
+
+int marks_dilate(Ctx *c)
+{
+    const int idm = c->id_max[c->MAX_LEVELS];
+    if (idm <= 0 || c->DIM != 3) return AGAL_OK;
+    AGAL_HIP(c, hipMemcpyAsync(c->scan_b, c->ref, sizeof(int) * idm, hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_dilate_refine, dim3(grid1(idm)), dim3(AWG), 0, c->stream, c->scan_b, c->ref, c->level, c->nbr,
+                       c->n_maxcblocks, idm, c->MAX_LEVELS_INTERIOR);
+    AGAL_HIP(c, hipGetLastError());
+    return AGAL_OK;
+}
+
 int launch_ref_criteria(Ctx *c, int *label)
 {
     if (c->precision == AGAL_FP64) return criterion_t<double>(c, label);

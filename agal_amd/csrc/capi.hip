@@ -491,6 +491,16 @@ int criterion_labels(Ctx *c)
     return st;
 }
 
+// Two timing events that live until the owning scope ends, on every return path.
+struct EventPair {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~EventPair()
+    {
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
 // M_AdvanceLoop with the learned predictor (no reference counterpart).  Every `infer_every` coarse
 // steps the features of all blocks are computed; during the first `train_steps` steps they are
 // kept as snapshots, and at each solution-based rebuild the plain criterion's labels train the
@@ -514,9 +524,10 @@ int advance_loop_learned_impl(Ctx *c, int i0, int n, const agal_learn_opts *o, a
     auto ms_since = [](std::chrono::steady_clock::time_point t0) {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
-    hipEvent_t e0, e1;
-    AGAL_HIP(c, hipEventCreate(&e0));
-    AGAL_HIP(c, hipEventCreate(&e1));
+    EventPair ev;
+    AGAL_HIP(c, hipEventCreate(&ev.e[0]));
+    AGAL_HIP(c, hipEventCreate(&ev.e[1]));
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     auto close_window = [&]() -> int {
         float ms = 0;
         AGAL_HIP(c, hipEventRecord(e1, c->stream));
@@ -539,9 +550,15 @@ int advance_loop_learned_impl(Ctx *c, int i0, int n, const agal_learn_opts *o, a
         if (!rebuild) {
             if (training) {
                 if (n_snap < max_slots && (st = predictor_snapshot(c, n_snap++, max_slots))) return st;
-            } else if (o->use_predictions) {
+            } else if (o->use_predictions == 1 || o->use_predictions == 2) {
                 int marked = 0;
-                if ((st = predictor_predict_blocks(c)) || (st = predictor_mark(c, thr, &marked))) return st;
+                if (o->use_predictions == 1) {
+                    if ((st = predictor_predict_blocks(c)) || (st = predictor_mark(c, thr, &marked))) return st;
+                } else {
+                    // Control: the plain criterion's refinement marks at the inference cadence.
+                    if ((st = launch_ref_criteria(c)) || (st = marks_refine_only(c)) || (st = count_refine_marks(c, &marked)))
+                        return st;
+                }
                 if (marked > 0) {
                     st = refine_coarsen(c, nullptr);
                     if (st && st != AGAL_ERR_POOL_EXHAUSTED) return st;
@@ -574,7 +591,8 @@ int advance_loop_learned_impl(Ctx *c, int i0, int n, const agal_learn_opts *o, a
             S->predictor_ms += ms_since(t0);
             auto t1 = std::chrono::steady_clock::now();
             if ((st = launch_ref_criteria(c))) return st;
-            if (!training && o->use_predictions) {
+            if (!training && o->use_predictions == 3 && (st = marks_dilate(c))) return st;
+            if (!training && o->use_predictions == 1) {
                 int marked = 0;
                 if ((st = predictor_predict_blocks(c)) || (st = predictor_keep(c, thr)) ||
                     (st = predictor_mark(c, thr, &marked)))
@@ -590,8 +608,6 @@ int advance_loop_learned_impl(Ctx *c, int i0, int n, const agal_learn_opts *o, a
         AGAL_HIP(c, hipEventRecord(e0, c->stream));
     }
     if ((st = close_window())) return st;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return check_exceptions(c);
 }
 
@@ -1311,15 +1327,7 @@ int agal_advance_loop(agal_ctx *ctx, int i0, int n, double *out_step_ms, double 
     if (!c->ic_done) return AGAL_ERR_STATE;
     if (n < 0) return AGAL_ERR_ARG;
     if (int st = usable(c)) return st;
-    // The two timing events live until the function returns, on every path.
-    struct Events {
-        hipEvent_t e[2] = {nullptr, nullptr};
-        ~Events()
-        {
-            for (hipEvent_t x : e)
-                if (x) (void)hipEventDestroy(x);
-        }
-    } ev;
+    EventPair ev;
     AGAL_HIP(c, hipEventCreate(&ev.e[0]));
     AGAL_HIP(c, hipEventCreate(&ev.e[1]));
     hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
@@ -1332,9 +1340,9 @@ int agal_advance_loop(agal_ctx *ctx, int i0, int n, double *out_step_ms, double 
     auto rebuild_after = [&](int i) { return adaptive && (i + 1) % c->P.P_REFINE == 0 && i > 0; };
     for (int i = i0; i < i0 + n; i++) {
         if (iter_mult == 1) {
-            // Up to four coarse steps with no rebuild between them run as one recorded graph.
+            // Up to graph_steps coarse steps with no rebuild between them run as one recorded graph.
             int k = 1;
-            while (k < 4 && i + k < i0 + n && !rebuild_after(i + k - 1)) k++;
+            while (k < c->graph_steps && i + k < i0 + n && !rebuild_after(i + k - 1)) k++;
             if ((st = steps_once(c, c->N_LEVEL_START, k))) return st;
             i += k - 1;
         } else {
@@ -1669,6 +1677,11 @@ int agal_set_option(agal_ctx *ctx, int option, long long value)
     if (option == AGAL_OPT_MRT_MFMA) {
         c->mrt_mfma = value != 0;
         c->mesh_epoch++;  // recorded step graphs hold the other kernel
+        return AGAL_OK;
+    }
+    if (option == AGAL_OPT_GRAPH_STEPS) {
+        if (value < 1 || value > 64) return AGAL_ERR_ARG;
+        c->graph_steps = (int)value;
         return AGAL_OK;
     }
     if (option == AGAL_OPT_EXC_CAP) {

@@ -220,10 +220,20 @@ __device__ __forceinline__ void collide_cell(const StepArgs<T, S> &a, const T *f
     }
     const T udotu = u * u + v * v + w * w;
     if (!a.mrt) {
+        // Pairs (p, pb(p)) share |c.u| and the weight: c_pb.u = -(c_p.u) bit for bit (sums of
+        // negated terms), so 3 c.u only changes sign and 4.5 (c.u)^2 is the same product; each
+        // direction still evaluates the reference's expression in its order.
+        const T a15 = (T)(1.5) * udotu;
+        fs[0] = f[0] * a.omegp + ((T)VS::w(0) * rho * (((T)(1.0) + (T)(3.0) * (T)0) + (T)(4.5) * (T)0 * (T)0 - a15)) * a.omeg;
 #pragma unroll
-        for (int p = 0; p < Q; p++) {
-            T cdotu = cdot<T, Q>(p, u, v, w);
-            fs[p] = f[p] * a.omegp + ((T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu)) * a.omeg;
+        for (int p = 1; p < Q; p++) {
+            if (!VS::half(p)) continue;
+            const int q = VS::pb(p);
+            const T cu = cdot<T, Q>(p, u, v, w);
+            const T t3 = (T)(3.0) * cu, t45 = (T)(4.5) * cu * cu;
+            const T wr = (T)VS::w(p) * rho;
+            fs[p] = f[p] * a.omegp + (wr * ((((T)(1.0) + t3) + t45) - a15)) * a.omeg;
+            fs[q] = f[q] * a.omegp + (wr * ((((T)(1.0) - t3) + t45) - a15)) * a.omeg;
         }
         return;
     }

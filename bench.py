@@ -61,7 +61,7 @@ def parse():
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 five-level cavity leg")
     ap.add_argument("--large-develop", type=int, default=300)
     ap.add_argument("--large-steps", type=int, default=10)
-    ap.add_argument("--large-capacity", type=int, default=12000000)
+    ap.add_argument("--large-capacity", type=int, default=18000000)
     ap.add_argument("--no-storage-study", action="store_true",
                     help="skip the FP16/BF16/FP32/FP64 comparison on one mesh sequence")
     ap.add_argument("--no-sphere-study", action="store_true",
@@ -74,6 +74,8 @@ def parse():
                     help="launch every kernel directly instead of replaying recorded step graphs (same kernels "
                          "and results; for runs under a kernel tracer)")
     ap.add_argument("--only-headline", action="store_true", help="headline leg only (no CPU, large leg or studies)")
+    ap.add_argument("--cavity-precision", choices=("fp32", "fp16", "bf16"), default="fp32",
+                    help="population storage of the cavity leg (fp32 is the headline; the others are for profiles)")
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (torch) to calibrate HBM byte counters")
@@ -222,7 +224,8 @@ def headline(args):
     from agal_amd import capi
 
     N, levels, Re = 64, 3, 1000.0
-    leg = cavity_leg(args, N, levels, Re, 100, args.capacity, args.develop, args.steps, "cavity 64^3")
+    prec = {"fp32": capi.FP32, "fp16": capi.FP16, "bf16": capi.BF16}[args.cavity_precision]
+    leg = cavity_leg(args, N, levels, Re, 100, args.capacity, args.develop, args.steps, "cavity 64^3", precision=prec)
     m, p = leg["m"], leg["p"]
     # Rebuild cost on this mesh (one solution-based rebuild after the window).
     t1 = time.time()
@@ -234,7 +237,7 @@ def headline(args):
     mlups_amortized = leg["cell_updates_per_step"] / ((step_ms + rebuild_one_ms / p.P_REFINE) * 1e-3) / 1e6
     fu = leg["fused"]
     traffic, prof_mesh = profiled_traffic("k_collide_stream_d3q19_f32_ldc64_3lev_traffic.json",
-                                          leg["blocks_per_level"])
+                                          leg["blocks_per_level"]) if args.cavity_precision == "fp32" else (None, None)
     roof = {
         "bound": "hbm",
         "achieved": round(fu["achieved"], 1),
@@ -242,7 +245,8 @@ def headline(args):
         "unit": "GB/s",
         "frac": round(fu["achieved"] / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "kernel": "k_collide_stream<float, float, 19, false>",
+        "kernel": "k_collide_stream<float, %s, 19, false>" % {"fp32": "float", "fp16": "_Float16",
+                                                               "bf16": "bf16s"}[args.cavity_precision],
         "launches": fu["launches"],
         "avg_launch_ms": round(fu["ms"] / max(1, fu["launches"]), 5),
         "bytes_per_launch_algorithmic": round(fu["alg_bytes"] / max(1, fu["launches"])),
@@ -323,7 +327,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": {"fp32": "f32", "fp16": "f16 storage, f32 compute", "bf16": "bf16 storage, f32 compute"}[args.cavity_precision],
         "data": "synthetic: lid-driven cavity from rest, mesh produced by the solver's own refinement",
         "config": {
             "workload": "D3Q19 BGK lid-driven cavity, root 64^3 (4x4x4 blocks), 3 levels by the |grad u| "
@@ -459,9 +463,12 @@ def learned_study(p, args):
     from agal_amd import capi
     from agal_amd.mesh import LbmMesh
 
-    out = {"train_steps": 2000, "inference_steps": 2000, "infer_every": 25, "batch": 4096}
+    out = {"train_steps": 2000, "inference_steps": 2000, "infer_every": 25, "batch": 4096,
+           "controls": "plain_every25: the plain criterion's refinement marks (no coarsening) at the same 25-step "
+                       "cadence as the network; plain_dilated: the plain criterion with its refinement marks grown "
+                       "by one same-level block at the rebuilds"}
     res = {}
-    for use in (0, 1):
+    for use in (0, 1, 2, 3):
         m = LbmMesh(p, args.capacity)
         m.S_SetIC(0)
         m.M_FreezeRefinedCells(0)
@@ -473,11 +480,12 @@ def learned_study(p, args):
         res[use]["blocks_per_level"] = blocks(m)
         m.close()
         log(f"learned study: use_predictions={use} done in {res[use]['wall_s']} s")
-    for key, r in (("plain", res[0]), ("learned", res[1])):
+    for key, r in (("plain", res[0]), ("learned", res[1]), ("plain_every25", res[2]), ("plain_dilated", res[3])):
         out[key] = {
             "flagged": r["flagged"], "missed": r["missed"],
             "missed_rate": round(r["missed"] / r["flagged"], 5) if r["flagged"] else None,
             "predicted_refinements": r["predicted_refinements"], "early_rebuilds": r["early_rebuilds"],
+            "finest_blocks_end": r["blocks_per_level"][-1],
             "step_ms": round(r["step_ms"], 2), "rebuild_ms": round(r["rebuild_ms"], 2),
             "predictor_ms": round(r["predictor_ms"], 2), "blocks_per_level_end": r["blocks_per_level"],
         }

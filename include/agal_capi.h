@@ -228,7 +228,12 @@ typedef struct agal_learn_opts {
     double threshold;        /* a block is predicted to need children when p > threshold (0.5)    */
     double pos_weight;       /* loss weight of positive labels (4)                                */
     unsigned long long seed; /* weight initialisation and batch sampling                          */
-    int    use_predictions;  /* 0: plain criterion only (training and audit counts still run)     */
+    int    use_predictions;  /* 0: plain criterion only (training and audit counts still run);
+                                1: the network's predictions refine ahead at every inference step;
+                                controls without the network -- 2: the plain criterion's refinement
+                                marks (no coarsening) at the same inference steps; 3: the plain
+                                criterion with its refinement marks grown by one block (same level)
+                                at the rebuilds                                                     */
 } agal_learn_opts;
 
 typedef struct agal_learn_stats {
@@ -388,6 +393,10 @@ int  agal_import_pool(agal_ctx *ctx, const agal_pool *pool);
  * agal_sync / agal_advance_loop reports AGAL_ERR_OVERFLOW and the context refuses further steps
  * (AGAL_ERR_STATE). */
 #define AGAL_OPT_EXC_CAP 3
+/* AGAL_OPT_GRAPH_STEPS: coarse steps of agal_advance_loop recorded into one graph at most (1..64,
+ * default 4).  Longer graphs amortize the gap between graph launches; each distinct length is a
+ * separate recording per mesh epoch. */
+#define AGAL_OPT_GRAPH_STEPS 4
 int  agal_set_option(agal_ctx *ctx, int option, long long value);
 
 /* ---- measurement ----------------------------------------------------------------------------- */

@@ -46,9 +46,9 @@ def test_five_level_cavity_matches_restatement():
 
 @pytest.mark.parametrize("prec", [capi.FP32, capi.FP16])
 def test_large_cavity_five_levels(prec):
-    """Sized for the GPU's memory: a pool of 12 M blocks (768 M cells; FP32 populations double-
-    buffered 117 GB) that the mesh grows into over four rebuilds."""
-    N, cap = 256, 12_000_000
+    """Sized for the GPU's memory: a pool of 18 M blocks (1.15 G cells; FP32 populations double-
+    buffered 175 GB) that the mesh grows into over four rebuilds (about 15 M blocks at the peak)."""
+    N, cap = 256, 18_000_000
     p = cavity5(N, 50, prec)
     print(f"large cavity 256^3, 5 levels, precision {prec}", flush=True)
     m = LbmMesh(p, cap)

@@ -7,6 +7,8 @@
 * Training: gradients of successive Adam steps against the restatement's, and the loss going down.
 * The learned advance loop on the cavity: the predictor refines ahead of the plain criterion, which
   misses fewer of the blocks it flags than the plain method does on its own.
+* The study's two controls without the network (criterion marks at the inference cadence, and
+  criterion marks dilated by one block): valid meshes and fewer missed blocks than the plain method.
 """
 from __future__ import annotations
 
@@ -15,6 +17,7 @@ import pytest
 
 from agal_amd import capi
 from agal_amd.mesh import LbmMesh
+from amr_checks import check_state_fast
 from oracle_lib import MlpOracle, OracleMesh
 from parity import compare_states, ldc_params
 
@@ -115,3 +118,32 @@ def test_learned_loop_refines_ahead_of_the_criterion():
     assert learned["predicted_refinements"] > 0
     rate = lambda s: s["missed"] / s["flagged"]
     assert rate(learned) < rate(plain)
+
+
+def test_learned_loop_controls_without_the_network():
+    """The two controls of the learned study on the same cavity: the plain criterion's refinement
+    marks applied at the inference cadence (mode 2), and its marks grown by one block at the
+    rebuilds (mode 3).  Both keep every invariant of the block pool and miss fewer flagged blocks
+    than the plain method; the dilated run holds more refined blocks."""
+    p = ldc_params(vs=capi.D3Q19, N=32, Re=200.0, levels=3, P_REFINE=20, S_CRITERION=2, N_REFINE_START=-2.0)
+    res, sizes = {}, {}
+    for use in (0, 2, 3):
+        m = LbmMesh(p, 60000)
+        m.S_SetIC(0)
+        m.M_FreezeRefinedCells(0)
+        opts = capi.learn_opts(train_steps=0, infer_every=5, batch=2048, adam_steps=0, lr=1e-3, threshold=0.5,
+                               pos_weight=4.0, seed=3, use_predictions=use)
+        res[use] = m.advance_loop_learned(0, 400, opts)
+        st = m.state(with_f=False, with_cells=False)
+        bad = check_state_fast(st, (8, 8, 8), n_max=60000)
+        assert not bad, f"mode {use}: " + "\n".join(bad[:10])
+        sizes[use] = [int(m.sizes.n_ids[L]) for L in range(3)]
+        m.close()
+    print(res, sizes)
+    rate = lambda s: s["missed"] / s["flagged"]
+    assert res[0]["flagged"] > 0 and res[2]["flagged"] > 0 and res[3]["flagged"] > 0
+    assert res[2]["predicted_refinements"] > 0 and res[2]["early_rebuilds"] > 0
+    assert res[0]["predicted_refinements"] == res[3]["predicted_refinements"] == 0
+    assert rate(res[2]) < rate(res[0])
+    assert rate(res[3]) <= rate(res[0])
+    assert sum(sizes[3][1:]) > sum(sizes[0][1:])
