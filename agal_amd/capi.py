@@ -41,7 +41,7 @@ N_SKIPID = -999
 CELL_INTERIOR, CELL_INTERFACE, CELL_GHOST, CELL_SOLID, CELL_BOUNDARY = 0, 1, 2, -1, -6
 BLOCKMASK_REGULAR, BLOCKMASK_INTERFACE, BLOCKMASK_SOLIDB, BLOCKMASK_SOLIDA, BLOCKMASK_SOLID = 0, 1, -1, -2, -3
 BLK_NBR, BLK_NBR_CHILD, BLK_REF, BLK_LEVEL, BLK_ONB, BLK_MASK = 0, 1, 2, 3, 4, 5
-OPT_GRAPHS, OPT_MRT_MFMA, OPT_EXC_CAP, OPT_GRAPH_STEPS = 1, 2, 3, 4
+OPT_GRAPHS, OPT_MRT_MFMA, OPT_EXC_CAP, OPT_GRAPH_STEPS, OPT_LINK_MASKS = 1, 2, 3, 4, 5
 
 
 class Params(C.Structure):

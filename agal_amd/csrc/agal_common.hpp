@@ -343,6 +343,11 @@ struct Store {  // S == T: the pool holds f
     static constexpr bool quantized = false;
 };
 
+// Fused multiply-add a * b + c in the operands' own type, one rounding (v_fma_f32 / v_fma_f64);
+// __builtin_fma alone would promote float operands to double and round twice.
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Binary16 bits of the largest half value at or below the negative float c (round toward -inf),
 // for |c| in the normal binary16 range: the truncated magnitude, plus one unit when bits were cut.
 constexpr uint16_t half_round_down_neg(float c)

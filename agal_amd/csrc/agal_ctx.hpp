@@ -91,6 +91,7 @@ struct Ctx {
     int *id_set = nullptr, *gap_set = nullptr, *iface = nullptr;
     // Per level, the same blocks as id_set in octree traversal order (the fused step's list).
     int *order = nullptr;
+    std::vector<int> root_zorder;  // level 0's IDs in Z order (host cache; cleared when the pool is replaced)
     // Rebuild scratch.
     unsigned *map_stamp = nullptr;
     int *first_child = nullptr, *scan_a = nullptr, *scan_b = nullptr, *partials = nullptr;
@@ -117,6 +118,7 @@ struct Ctx {
     // each level (lk_ids + lk_off[L], lk_n[L] of them) and the per-block momentum-exchange sums
     // (3 x n_maxcblocks values of the compute type).
     bool links_on = false;
+    bool link_masks = false;  // links made to agree with the cell masks (AGAL_OPT_LINK_MASKS)
     int *lk_bcount = nullptr, *lk_onb_solid = nullptr, *lk_ids = nullptr, *lk_counter = nullptr;
     float *lk_X = nullptr;
     void *lk_blkF = nullptr;

@@ -1595,6 +1595,7 @@ int agal_import_pool(agal_ctx *ctx, const agal_pool *pl)
         return AGAL_ERR_ARG;
     const long long nb = pl->n_blocks;
     if (nb < c->n_coarsecblocks || nb > nmax || pl->n_gaps < 0 || pl->n_gaps > nmax) return AGAL_ERR_ARG;
+    c->root_zorder.clear();  // the imported root level may place its IDs differently
     long long total = 0;
     for (int L = 0; L < ML; L++) {
         if (pl->n_ids[L] < 0 || pl->n_ids[L] > nb) return AGAL_ERR_ARG;
@@ -1682,6 +1683,10 @@ int agal_set_option(agal_ctx *ctx, int option, long long value)
     if (option == AGAL_OPT_GRAPH_STEPS) {
         if (value < 1 || value > 64) return AGAL_ERR_ARG;
         c->graph_steps = (int)value;
+        return AGAL_OK;
+    }
+    if (option == AGAL_OPT_LINK_MASKS) {
+        c->link_masks = value != 0;
         return AGAL_OK;
     }
     if (option == AGAL_OPT_EXC_CAP) {

@@ -177,18 +177,22 @@ __global__ void k_mark_finalize(int idm, int *ref, int *tmp)
 // Cu_MarkBlocks_CheckMasks_S1/S2 (mesh_voxelizer_masks.cu:21-226, the neighbour-table variant):
 // in leaf blocks of level L whose block mask is SOLIDB or SOLIDA, every non-solid cell with a solid
 // cell among its 26 neighbours becomes a BOUNDARY cell, and the block is flagged as on a boundary.
-// Only SOLID masks are read and only non-solid cells are written, so one pass is race-free.
+// With all_leaf (links made to agree with the cell masks) every leaf block is examined, so that
+// fluid cells beside a solid cell are BOUNDARY cells also in REGULAR blocks (no solid cell of their
+// own) and in blocks classified SOLID that still hold fluid cells (the reference skips both).  Only SOLID masks are read
+// and only non-solid cells are written, so one pass is race-free.
 __global__ __launch_bounds__(GWG) void k_check_masks(const int *__restrict__ ids, int n, const int *__restrict__ nbr_slot,
                                                      const int *__restrict__ nbr_child, const int *__restrict__ blk_mask,
                                                      int *__restrict__ cells_mask, int *__restrict__ onb,
-                                                     int *__restrict__ bcount)
+                                                     int *__restrict__ bcount, int all_leaf)
 {
     const int t = threadIdx.x & 63;
     const int bi = blockIdx.x * (GWG / 64) + (threadIdx.x >> 6);
     if (bi >= n) return;  // whole wavefronts
     const int b = ids[bi];
     const int bm = blk_mask[b];
-    if (nbr_child[b] >= 0 || !(bm < 0 && bm != BLK_SOLID)) return;  // wave-uniform
+    if (nbr_child[b] >= 0) return;  // wave-uniform
+    if (!all_leaf && !(bm < 0 && bm != BLK_SOLID)) return;
     const int my_nb = t < 27 ? nbr_slot[(long long)b * 27 + t] : 0;
     const long long cell = (long long)b * 64 + t;
     const int I = t & 3, J = (t >> 2) & 3, K = t >> 4;
@@ -352,7 +356,8 @@ int check_masks_level(Ctx *c, int L)
     const int n = c->n_ids[L];
     if (n <= 0) return AGAL_OK;
     hipLaunchKernelGGL(k_check_masks, dim3((unsigned)((n + 3) / 4)), dim3(GWG), 0, c->stream, c->id_set_L(L), n,
-                       c->nbr_slot, c->nbr_child, c->blk_mask, c->cells_mask, c->onb, c->lk_bcount);
+                       c->nbr_slot, c->nbr_child, c->blk_mask, c->cells_mask, c->onb, c->lk_bcount,
+                       c->link_masks ? 1 : 0);
     AGAL_HIP(c, hipGetLastError());
     return AGAL_OK;
 }

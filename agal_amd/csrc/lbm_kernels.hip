@@ -53,9 +53,11 @@ __device__ __forceinline__ void moments(const T *f, T &rho, T &u, T &v, T &w)
         if (VS::DIM == 3 && VS::cz(p) > 0) w += f[p];
         if (VS::DIM == 3 && VS::cz(p) < 0) w -= f[p];
     }
-    u /= rho;
-    v /= rho;
-    if (VS::DIM == 3) w /= rho;
+    // One correctly rounded division, then products (the CPU restatement does the same).
+    const T r = (T)(1.0) / rho;
+    u = u * r;
+    v = v * r;
+    if (VS::DIM == 3) w = w * r;
 }
 
 // c_p . u in the reference's left-to-right order (cx u + cy v) + cz w, without the terms of zero
@@ -220,20 +222,23 @@ __device__ __forceinline__ void collide_cell(const StepArgs<T, S> &a, const T *f
     }
     const T udotu = u * u + v * v + w * w;
     if (!a.mrt) {
-        // Pairs (p, pb(p)) share |c.u| and the weight: c_pb.u = -(c_p.u) bit for bit (sums of
-        // negated terms), so 3 c.u only changes sign and 4.5 (c.u)^2 is the same product; each
-        // direction still evaluates the reference's expression in its order.
-        const T a15 = (T)(1.5) * udotu;
-        fs[0] = f[0] * a.omegp + ((T)VS::w(0) * rho * (((T)(1.0) + (T)(3.0) * (T)0) + (T)(4.5) * (T)0 * (T)0 - a15)) * a.omeg;
+        // BGK in fused multiply-adds, in the same order as the CPU restatement:
+        //   b_p = (4.5 c.u) c.u + (1 - 1.5 u.u),  feq_p / (w_p rho) = 3 c.u + b_p,
+        //   f*_p = (w_p rho omega) (feq_p / (w_p rho)) + (1 - omega) f_p.
+        // A pair (p, pb(p)) shares |c.u|, b and the weight: c_pb.u = -(c_p.u) bit for bit (sums of
+        // negated terms), so the pb direction takes -3 c.u.
+        const T c1 = (T)(1.0) - (T)(1.5) * udotu;
+        const T ro = rho * a.omeg;
+        fs[0] = fmadd((T)VS::w(0) * ro, c1, f[0] * a.omegp);
 #pragma unroll
         for (int p = 1; p < Q; p++) {
             if (!VS::half(p)) continue;
             const int q = VS::pb(p);
             const T cu = cdot<T, Q>(p, u, v, w);
-            const T t3 = (T)(3.0) * cu, t45 = (T)(4.5) * cu * cu;
-            const T wr = (T)VS::w(p) * rho;
-            fs[p] = f[p] * a.omegp + (wr * ((((T)(1.0) + t3) + t45) - a15)) * a.omeg;
-            fs[q] = f[q] * a.omegp + (wr * ((((T)(1.0) - t3) + t45) - a15)) * a.omeg;
+            const T b = fmadd((T)(4.5) * cu, cu, c1);
+            const T wro = (T)VS::w(p) * ro;
+            fs[p] = fmadd(wro, fmadd((T)(3.0), cu, b), f[p] * a.omegp);
+            fs[q] = fmadd(wro, fmadd((T)(-3.0), cu, b), f[q] * a.omegp);
         }
         return;
     }

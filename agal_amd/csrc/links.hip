@@ -353,6 +353,41 @@ __global__ __launch_bounds__(KWG) void k_link_validation(const int *__restrict__
     if (ncorr) atomicAdd(corrections, ncorr);
 }
 
+// Links made to agree with the cell masks (AGAL_OPT_LINK_MASKS; no counterpart in the reference):
+// for each BOUNDARY cell of a linked block and each of the 26 directions, the fraction stays only
+// when the partner cell -- the same-level neighbour, its mask read at its block ID -- is SOLID, and a
+// SOLID partner without a cut face gets 1/2.  One wavefront per block; a cell writes only its own
+// slots.
+__global__ __launch_bounds__(KWG) void k_link_mask_fix(const int *__restrict__ ids, int n,
+                                                       const int *__restrict__ nbr_slot,
+                                                       const int *__restrict__ onb_solid,
+                                                       const int *__restrict__ cells_mask, float *__restrict__ X_b,
+                                                       long long ncb)
+{
+    using V = VelSet<27>;
+    const int t = threadIdx.x & 63;
+    const int bi = blockIdx.x * (KWG / 64) + (threadIdx.x >> 6);
+    if (bi >= n) return;
+    const int b = ids[bi];
+    const int rank = onb_solid[b];
+    if (rank < 0) return;
+    const int my_nb = nbr_slot[(long long)b * 27 + (t < 27 ? t : 26)];
+    const int I = t & 3, J = (t >> 2) & 3, K = t >> 4;
+    const long long a = (long long)rank * 64 + t;
+    const bool bnd = cells_mask[(long long)b * 64 + t] == CELL_BOUNDARY;
+    for (int p = 1; p < 27; p++) {
+        const int Ip = I + V::cx(p), Jp = J + V::cy(p), Kp = K + V::cz(p);
+        const int slot = (Ip < 0 ? 0 : (Ip >= 4 ? 2 : 1)) + 3 * (Jp < 0 ? 0 : (Jp >= 4 ? 2 : 1)) +
+                         9 * (Kp < 0 ? 0 : (Kp >= 4 ? 2 : 1));
+        const int nb = __shfl(my_nb, slot, 64);
+        if (!bnd) continue;
+        const int nc = (Ip & 3) + 4 * (Jp & 3) + 16 * (Kp & 3);
+        const bool solid = nb > -1 && cells_mask[(long long)nb * 64 + nc] == CELL_SOLID;
+        const float x = X_b[a + (long long)p * ncb];
+        X_b[a + (long long)p * ncb] = solid ? (x > 0.0f ? x : 0.5f) : -1.0f;
+    }
+}
+
 __global__ void k_fill_float(float *a, long long n, float v)
 {
     const long long i = (long long)blockIdx.x * KWG + threadIdx.x;
@@ -599,6 +634,9 @@ int compute_links(Ctx *c)
                            c->lb_start, c->lb_faces, c->lk_X, c->lk_ncb);
         hipLaunchKernelGGL(k_link_validation, dim3((unsigned)((n + 3) / 4)), dim3(KWG), 0, c->stream, c->id_set_L(L), n,
                            c->nbr_slot, c->lk_onb_solid, c->cells_mask, c->lk_X, c->lk_ncb, c->lk_counter);
+        if (c->link_masks)
+            hipLaunchKernelGGL(k_link_mask_fix, dim3((unsigned)((n + 3) / 4)), dim3(KWG), 0, c->stream, c->id_set_L(L), n,
+                               c->nbr_slot, c->lk_onb_solid, c->cells_mask, c->lk_X, c->lk_ncb);
         AGAL_HIP(c, hipGetLastError());
     }
     if (!c->lk_ids) AGAL_HIP(c, hipMalloc(&c->lk_ids, sizeof(int) * nmax));

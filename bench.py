@@ -66,7 +66,7 @@ def parse():
                     help="skip the FP16/BF16/FP32/FP64 comparison on one mesh sequence")
     ap.add_argument("--no-sphere-study", action="store_true",
                     help="skip the flow past the noisy icosphere (D3Q27 MRT, FP64/FP32/FP16)")
-    ap.add_argument("--sphere-steps", type=int, default=1500, help="coarse steps of the sphere flow")
+    ap.add_argument("--sphere-steps", type=int, default=1000, help="coarse steps of the sphere flow")
     ap.add_argument("--no-learned-study", action="store_true",
                     help="skip the learned refinement predictor on the cavity workload")
     ap.add_argument("--roofline-steps", type=int, default=20)
@@ -498,10 +498,19 @@ def learned_study(p, args):
 def sphere_study(args):
     """Flow past the seeded noisy icosphere: 5,120 triangles, radius 0.08 L with vertex radii
     perturbed by +-5% (uniform, seeded), centred at 0.3 L in the cubic domain; D3Q27 MRT, root 64^3,
-    4 levels refined near the surface, inflow 0.04, Re 200 on the diameter.  FP64 (vector MRT), FP32
-    (matrix-core MRT and vector MRT) and FP16 storage of f - w (matrix-core MRT) run the same static
-    mesh from the same uniform start in lockstep; leaf velocities are compared with FP64 every 250
-    coarse steps and the drag (momentum exchange, FP64 reduction) is averaged over the last 500."""
+    4 levels refined near the surface, inflow 0.04, Re 200 on the diameter.  Plain bounce-back on the
+    body and the first-order momentum exchange on face-cell links that agree with the cell masks
+    (AGAL_OPT_LINK_MASKS); C_D = 8 F_x / (U^2 pi D^2) with F the momentum-exchange force reported as
+    the reference reports it (block sums x dx^3, total / dx of the finest level).
+
+    First the force on the body in a fluid at rest (one step), with the reference's link rules and
+    with links agreeing with the masks.  Then FP64 (vector MRT), FP32 (vector MRT and matrix-core
+    MRT) and FP16 storage of f - w (vector MRT) run the same static mesh from the same uniform start
+    in lockstep; leaf velocities are compared with FP64 every 250 coarse steps and C_D is averaged
+    over the second half of the run (sampled every 10 coarse steps).  The run ends at 1000 coarse
+    steps (6.25 diameters of travel): the wake reaches the outflow face about 1100 coarse steps
+    after the start, and from there the outflow's anti-bounce-back rule feeds disturbances back
+    into the level-0 region."""
     import math
 
     from agal_amd import capi
@@ -510,20 +519,41 @@ def sphere_study(args):
 
     N, U, Re, D, levels = 64, 0.04, 200.0, 0.16, 4
     tri = capi.icosphere((0.3, 0.5, 0.5), D / 2, 4, 0.05, 20240)
+    rates = dict(s6=1.2)
     out = {"workload": "D3Q27 MRT flow past the noisy icosphere (5120 triangles, r 0.08 L +-5%, seed 20240), "
                        "root 64^3, 4 wall levels, inflow 0.04, Re 200, MRT rates s1..s5 parser defaults, s6 1.2 "
-                       "(1.98 diverges from the impulsive start)",
+                       "(1.98 diverges from the uniform start), plain bounce-back, first-order momentum exchange "
+                       "on links agreeing with the cell masks",
+           "cd_formula": "8 F_x / (U^2 pi D^2), F = agal_forces (S_ReportForces scaling)",
            "coarse_steps": args.sphere_steps, "mlups": {}, "cd": {}, "fused_kernel": {},
-           "mrt_transform": {"fp64": "vector", "fp32": "matrix cores", "fp32_vector": "vector",
-                             "fp16": "matrix cores"}}
+           "mrt_transform": {"fp64": "vector", "fp32": "vector", "fp32_mfma": "matrix cores", "fp16": "vector"}}
+
+    def params(prec, u):
+        return capi.default_params(velocity_set=capi.D3Q27, precision=prec, collision=capi.MRT, Nx=N,
+                                   MAX_LEVELS_WALL=levels, MAX_LEVELS_INTERIOR=1, v0=U * D / Re,
+                                   bc_preset=capi.BC_FPSC3D, bc_velocity=u, ic_u=u, P_REFINE=1 << 30,
+                                   G_BIN_MAKE_LINKS=1, G_BIN_DENSITY=16, G_BIN_LEVELS=10, G_BIN_SPEC=7,
+                                   S_BC_TYPE=0, S_FORCE_TYPE=0, S_FORCE_ORDER=1, **rates)
+
+    cd_scale = 8.0 / (U * U * math.pi * D * D)
+    rest = {}
+    for key, fix in (("reference_link_rules", 0), ("links_agree_with_masks", 1)):
+        m = LbmMesh(params(capi.FP64, 0.0), args.capacity)
+        m.set_option(capi.OPT_LINK_MASKS, fix)
+        m.set_geometry(tri)
+        m.M_Advance_RefineNearWall()
+        m.S_Advance(0)
+        F = m.S_ComputeForces()
+        rest[key] = [float(f"{cd_scale * x:.3g}") for x in F]
+        m.close()
+    out["rest_force_in_cd_units"] = rest
+    log(f"sphere: force at rest in C_D units {rest}")
     ctx = {}
-    for k, prec in (("fp64", capi.FP64), ("fp32", capi.FP32), ("fp32_vector", capi.FP32), ("fp16", capi.FP16)):
-        p = capi.default_params(velocity_set=capi.D3Q27, precision=prec, collision=capi.MRT, Nx=N,
-                                MAX_LEVELS_WALL=levels, MAX_LEVELS_INTERIOR=1, v0=U * D / Re,
-                                bc_preset=capi.BC_FPSC3D, bc_velocity=U, ic_u=U, P_REFINE=1 << 30, s6=1.2)
-        m = LbmMesh(p, args.capacity)
-        if k in ("fp32", "fp16"):
+    for k, prec in (("fp64", capi.FP64), ("fp32", capi.FP32), ("fp32_mfma", capi.FP32), ("fp16", capi.FP16)):
+        m = LbmMesh(params(prec, U), args.capacity)
+        if k == "fp32_mfma":
             m.set_option(capi.OPT_MRT_MFMA, 1)
+        m.set_option(capi.OPT_LINK_MASKS, 1)
         m.set_geometry(tri)
         t0 = time.time()
         m.M_Advance_RefineNearWall()
@@ -536,40 +566,39 @@ def sphere_study(args):
     out["blocks_per_level"] = n_ids
     out["cell_updates_per_step"] = upd
     K = args.sphere_steps
-    avg_from = max(0, K - 500)
-    # All runs advance in lockstep chunks; after each chunk the leaf velocities are compared with
-    # FP64 (the error series shows how fast the runs separate in this unsteady flow).
+    avg_from = K // 2
     ms = {k: 0.0 for k in ctx}
     cds = {k: [] for k in ctx}
-    series = []
+    series, cd_series = [], []
     done = 0
     while done < K:
-        step = min(10 if done >= avg_from else 250, K - done, (avg_from - done) if done < avg_from else K)
+        step = min(10, K - done)
         for k, m in ctx.items():
             _, t, _ = m.advance_loop(done, step)
             ms[k] += t
+            F = m.S_ComputeForces()
+            cds[k].append(cd_scale * F[0])
         done += step
-        if done > avg_from:
-            for k, m in ctx.items():
-                F = m.S_ComputeForces()
-                cds[k].append(8.0 * F[0] / (U * U * math.pi * D * D))
         if done % 250 == 0 or done == K:
             row = {"step": done}
             for k in ctx:
                 if k != "fp64":
                     row[k] = float(f"{velocity_error(ctx['fp64'], ctx[k])['rel_l2_velocity']:.3g}")
             series.append(row)
-            log(f"sphere: {done} coarse steps, velocity error vs fp64 {row}")
+            cd_series.append({"step": done, **{k: round(float(sum(cds[k][-25:]) / len(cds[k][-25:])), 4) for k in ctx}})
+            log(f"sphere: {done} coarse steps, velocity error vs fp64 {row}, C_D {cd_series[-1]}")
+    nwin = (K - avg_from) // 10
     for k, m in ctx.items():
         out["mlups"][k] = round(upd * K / (ms[k] * 1e-3) / 1e6, 1)
-        out["cd"][k] = round(sum(cds[k]) / len(cds[k]), 5)
+        out["cd"][k] = round(sum(cds[k][-nwin:]) / nwin, 5)
         fms, nl, fc = roofline_pass(m, 5)
-        b = {"fp64": 8, "fp32": 4, "fp32_vector": 4, "fp16": 2}[k]
+        b = {"fp64": 8, "fp32": 4, "fp32_mfma": 4, "fp16": 2}[k]
         out["fused_kernel"][k] = {"avg_launch_ms": round(fms / max(1, nl), 5),
                                   "achieved_gbs": round(2 * 27 * b * fc / (fms * 1e-3) / 1e9, 1)
                                   if fms > 0 else None}
-    out["cd_window"] = f"mean over the last {K - avg_from} coarse steps, sampled every 10"
+    out["cd_window"] = f"mean over coarse steps {avg_from}-{K}, sampled every 10"
     out["vel_err_vs_fp64_series"] = series
+    out["cd_series_250_step_means"] = cd_series
     out["cd_rel_err_vs_fp64"] = {k: float(f"{abs(out['cd'][k] / out['cd']['fp64'] - 1):.3g}") for k in ctx if k != "fp64"}
     out["cd_schiller_naumann_unbounded"] = round(24.0 / Re * (1 + 0.15 * Re ** 0.687), 4)
     for m in ctx.values():

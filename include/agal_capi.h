@@ -397,6 +397,17 @@ int  agal_import_pool(agal_ctx *ctx, const agal_pool *pool);
  * default 4).  Longer graphs amortize the gap between graph launches; each distinct length is a
  * separate recording per mesh epoch. */
 #define AGAL_OPT_GRAPH_STEPS 4
+/* AGAL_OPT_LINK_MASKS: 1 makes the boundary cells and face-cell links of the next
+ * agal_refine_near_wall agree with the cell masks (no counterpart in the reference): every non-solid
+ * cell with a SOLID neighbour becomes a BOUNDARY cell in every leaf block (the reference examines
+ * SOLIDA / SOLIDB blocks only, not REGULAR ones nor SOLID ones that still hold fluid cells); after the reference's validation a BOUNDARY
+ * cell's fraction towards direction p is kept only when the partner cell is SOLID, and a SOLID
+ * partner without a cut face gets 1/2.  The links are then exactly the pairs that plain
+ * bounce-back reflects, and the momentum exchange of a fluid at rest sums to zero.  0 (default)
+ * keeps the reference's rules, under which fluid cells beside a solid cell of a neighbouring block
+ * are not linked, a face cut inside a link to a fluid cell stays a link, and the validation reads
+ * the partner's mask at its rank. */
+#define AGAL_OPT_LINK_MASKS 5
 int  agal_set_option(agal_ctx *ctx, int option, long long value);
 
 /* ---- measurement ----------------------------------------------------------------------------- */

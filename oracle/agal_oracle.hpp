@@ -705,9 +705,11 @@ struct Oracle {
             v += V.c[1][p] * f;
             if (DIM == 3) w += V.c[2][p] * f;
         }
-        u /= rho;
-        v /= rho;
-        if (DIM == 3) w /= rho;
+        // One correctly rounded division, then products (as the GPU kernels).
+        const T rinv = (T)(1.0) / rho;
+        u = u * rinv;
+        v = v * rinv;
+        if (DIM == 3) w = w * rinv;
     }
 
     // Cu_RefreshVariables (solver_lbm_kernels.cu:110-159).
@@ -762,9 +764,11 @@ struct Oracle {
                     if (DIM == 3) w += V.c[2][p] * f_pi;
                     f_p[p] = f_pi;
                 }
-                u /= rho;
-                v /= rho;
-                if (DIM == 3) w /= rho;
+                // One correctly rounded division, then products.
+                const T rinv = (T)(1.0) / rho;
+                u = u * rinv;
+                v = v * rinv;
+                if (DIM == 3) w = w * rinv;
                 T udotu = u * u + v * v + w * w;
                 Fp(Q + 0)[cell] = rho;
                 Fp(Q + 1)[cell] = u;
@@ -779,10 +783,15 @@ struct Oracle {
                         if (valid_mask != CELL_SOLID) Fp(p)[cell] = fs[p];
                     continue;
                 }
+                // The equilibrium polynomial in fused multiply-adds (std::fma, correctly rounded as
+                // the GPU's): b = (4.5 c.u) c.u + (1 - 1.5 u.u), feq / (w rho) = 3 c.u + b,
+                // f* = (w rho omega) (feq / (w rho)) + (1 - omega) f.
+                const T c1 = (T)(1.0) - (T)(1.5) * udotu;
+                const T ro = rho * omeg;
                 for (int p = 0; p < Q; p++) {
                     T cdotu = (T)V.c[0][p] * u + (T)V.c[1][p] * v + (T)V.c[2][p] * w;
-                    T f_pi = f_p[p] * omegp +
-                             ((T)V.w[p] * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu)) * omeg;
+                    T b = std::fma((T)(4.5) * cdotu, cdotu, c1);
+                    T f_pi = std::fma((T)V.w[p] * ro, std::fma((T)(3.0), cdotu, b), f_p[p] * omegp);
                     if (valid_mask != CELL_SOLID) Fp(p)[cell] = f_pi;
                 }
             }
@@ -2037,7 +2046,9 @@ struct Oracle {
 
     // Cu_MarkBlocks_CheckMasks_S1/S2 (mesh_voxelizer_masks.cu:21-226): non-solid cells of SOLIDB /
     // SOLIDA leaf blocks of level L with a solid neighbour cell become BOUNDARY cells; their
-    // blocks are flagged as on a boundary.
+    // blocks are flagged as on a boundary.  With links made to agree with the cell masks
+    // (link_masks), every leaf block is examined (REGULAR ones and SOLID ones holding fluid cells
+    // too).
     void check_masks(int L)
     {
         if ((long)bcount.size() != n_maxcblocks) bcount.assign(n_maxcblocks, 0);
@@ -2045,7 +2056,8 @@ struct Oracle {
         for (int bi = 0; bi < n_ids[L]; bi++) {
             const int b = ids[bi];
             const int bm = blk_mask[b];
-            if (nbr_child[b] >= 0 || !(bm < 0 && bm != BLK_SOLID)) continue;
+            if (nbr_child[b] >= 0) continue;
+            if (!link_masks && !(bm < 0 && bm != BLK_SOLID)) continue;
             int s_nbr[27];
             load_block_nbrs(b, s_nbr);
             bool any = false;
@@ -2084,6 +2096,7 @@ struct Oracle {
     LinkBins lbins;
     bool links = false;
     long n_link_corrections = 0;
+    bool link_masks = false;      // links made to agree with the cell masks (AGAL_OPT_LINK_MASKS)
     std::vector<T> blk_F;         // cblock_f_Ff: per block, the x / y / z momentum-exchange sums
 
     void rank_boundary_blocks()
@@ -2206,6 +2219,34 @@ struct Oracle {
         }
     }
 
+    // Links made to agree with the cell masks (AGAL_OPT_LINK_MASKS; no reference counterpart): a
+    // BOUNDARY cell keeps its fraction towards p only when the partner cell (same-level neighbour,
+    // mask at its block ID) is SOLID; a SOLID partner without a cut face gets 1/2.
+    void link_mask_fix(int L)
+    {
+        const int *ids = idset(L);
+        for (int bi = 0; bi < n_ids[L]; bi++) {
+            const int b = ids[bi];
+            const int rank = onb_solid[b];
+            if (rank < 0) continue;
+            int s_nbr[27];
+            load_block_nbrs(b, s_nbr);
+            for (int t = 0; t < M_CBLOCK; t++) {
+                if (cells_mask[(long)b * M_CBLOCK + t] != CELL_BOUNDARY) continue;
+                int I, J, K;
+                cell_ijk(t, I, J, K);
+                for (int p = 1; p < 27; p++) {
+                    const int Ip = I + V.c[0][p], Jp = J + V.c[1][p], Kp = K + V.c[2][p];
+                    const int nb = s_nbr[nbr_map(3, Ip, Jp, Kp)];
+                    const int nc = nbr_cell_id(3, Ip, Jp, Kp);
+                    const bool solid = nb > -1 && cells_mask[(long)nb * M_CBLOCK + nc] == CELL_SOLID;
+                    float &x = X_b[(size_t)rank * M_CBLOCK + t + (size_t)p * n_maxcells_b];
+                    x = solid ? (x > 0.0f ? x : 0.5f) : -1.0f;
+                }
+            }
+        }
+    }
+
     // Mesh::M_LinkLengthComputation (mesh_voxelizer.cu:262-296) on the wall levels, after the bins
     // (Geometry::Bins, geometry_bins.h:197-203).
     void compute_links()
@@ -2217,6 +2258,7 @@ struct Oracle {
             if (n_ids[L] <= 0) continue;
             link_lengths(L);
             link_validation(L);
+            if (link_masks) link_mask_fix(L);
         }
         links = true;
     }

@@ -92,6 +92,17 @@ int oracle_set_threads(int n)
 #endif
 }
 
+// Options of the GPU library that change results; the others (graphs, graph length, exception-list
+// cap) have no meaning here.
+int oracle_set_option(void *h, int option, long long value)
+{
+    if (option != AGAL_OPT_LINK_MASKS) return AGAL_ERR_ARG;
+    return dispatch(h, [&](auto &O) {
+        O.link_masks = value != 0;
+        return AGAL_OK;
+    });
+}
+
 int oracle_sync(void *h) { return h ? AGAL_OK : AGAL_ERR_ARG; }
 
 int oracle_get_sizes(void *h, agal_sizes *s)

@@ -28,6 +28,7 @@ def oracle_library():
         _lib = C.CDLL(ORACLE_PATH)
         for extra, (res, args) in {
             "set_threads": (C.c_int, [C.c_int]),
+            "set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_longlong]),
             "plant": (C.c_int, [C.c_void_p, C.c_int]),
             "collide": (C.c_int, [C.c_void_p, C.c_int]),
             "impose_bc": (C.c_int, [C.c_void_p, C.c_int]),
@@ -64,6 +65,10 @@ class OracleMesh(LbmMesh):
         lib = oracle_library()
         fns = capi.bind(lib, "oracle_", _NAMES)
         super().__init__(params, capacity, fns=fns, lib=lib, prefix="oracle_")
+
+    def set_option(self, option, value):
+        """Only options that change results exist here (AGAL_OPT_LINK_MASKS)."""
+        self._call_raw("set_option", int(option), int(value))
 
     def plant(self, code: int):
         """Plant a known mistake in the restatement (see Oracle::plant); 0 removes it."""

@@ -113,3 +113,40 @@ def test_slab_couette_links(zw):
     g.sync()
     assert_same(g, o)
     np.testing.assert_allclose(g.S_ComputeForces(), o.S_ComputeForces(), rtol=1e-9, atol=1e-14)
+
+
+@pytest.mark.parametrize("U", [0.0, 0.04])
+def test_links_agreeing_with_masks(U):
+    """AGAL_OPT_LINK_MASKS on the workload's noisy icosphere (32^3 root, four wall levels, D3Q19 FP64):
+    BOUNDARY masks, ranks and fractions bit for bit; at rest the momentum exchange is zero to
+    rounding; in the flow, two rounds of two coarse steps with populations bit for bit and the
+    force to a relative 1e-9."""
+    print(f"links agreeing with masks: U {U}", flush=True)
+    tri = capi.icosphere((0.3, 0.5, 0.5), 0.08, 4, 0.05, 20240)
+    p = capi.default_params(velocity_set=capi.D3Q19, precision=capi.FP64, collision=capi.BGK, Nx=32,
+                            MAX_LEVELS_WALL=4, MAX_LEVELS_INTERIOR=1, v0=0.04 * 0.16 / 200.0,
+                            bc_preset=capi.BC_FPSC3D, bc_velocity=U, ic_u=U, P_REFINE=1 << 30,
+                            G_BIN_MAKE_LINKS=1, G_BIN_DENSITY=16, G_BIN_LEVELS=10, G_BIN_SPEC=7,
+                            S_BC_TYPE=0, S_FORCE_TYPE=0, S_FORCE_ORDER=1)
+    g, o = LbmMesh(p, 60000), OracleMesh(p, 60000)
+    for m in (g, o):
+        m.set_option(capi.OPT_LINK_MASKS, 1)
+        m.set_geometry(tri)
+    assert g.M_Advance_RefineNearWall() == o.M_Advance_RefineNearWall() == capi.AGAL_OK
+    g.sync()
+    assert_same(g, o)
+    assert_links_same(g, o)
+    for k in range(2):
+        for m in (g, o):
+            for _ in range(2):
+                m.S_Advance(0)
+        g.sync()
+        assert_same(g, o)
+        fg, fo = g.S_ComputeForces(), o.S_ComputeForces()
+        np.testing.assert_allclose(fg, fo, rtol=1e-9, atol=1e-15)
+        if U == 0.0:
+            assert np.abs(fg).max() < 1e-15
+    if U > 0:
+        assert np.all(np.isfinite(fg)) and np.abs(fg).max() > 0
+    g.close()
+    o.close()

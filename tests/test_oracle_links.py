@@ -283,3 +283,70 @@ def test_force_orders_agree_at_rest():
     _, F2, _, _ = _slab_couette(0.19, bc=1, order=2, U=0.0, steps=10)
     np.testing.assert_allclose(F1, F2, rtol=1e-6, atol=1e-12)
     assert abs(F1[2] / (-area / 3) - 1) < 1e-6 and abs(F1[0]) < 1e-12 and abs(F1[1]) < 1e-12
+
+
+def _pairs_vs_links(m):
+    """Compares the links (fraction > 0) of every BOUNDARY cell of the finest level with the (fluid,
+    SOLID) cell pairs of that level over the 26 directions; returns (spurious links, missing links,
+    fluid cells beside a SOLID cell that are not BOUNDARY)."""
+    st, L = m.state(), m.links()
+    c, _, _, _ = velset(27)
+    cidx = {tuple(c[:, q]): q for q in range(27)}
+    nbr, cm, onb, Xb = st["nbr"], st["cell_mask"], L["onb_solid"], L["X_b"]
+    finest = np.nonzero((st["ref"] != capi.REF_INACTIVE) & (st["level"] == m.sizes.max_levels - 1))[0]
+    t = np.arange(64)
+    I, J, K = t % 4, (t // 4) % 4, t // 16
+    spurious = missing = unlinked = 0
+    for b in finest:
+        mk = cm[b * 64:(b + 1) * 64]
+        fluid = mk != capi.CELL_SOLID
+        if not fluid.any():
+            continue
+        for q in range(1, 27):
+            Ip, Jp, Kp = I + c[0, q], J + c[1, q], K + c[2, q]
+            ox, oy, oz = Ip // 4, Jp // 4, Kp // 4
+            nb = np.array([b if (a, bb, cc) == (0, 0, 0) else nbr[cidx[(a, bb, cc)], b] for a, bb, cc in zip(ox, oy, oz)])
+            ok = nb >= 0
+            solid = np.zeros(64, bool)
+            solid[ok] = cm[nb[ok] * 64 + (Ip[ok] % 4) + 4 * (Jp[ok] % 4) + 16 * (Kp[ok] % 4)] == capi.CELL_SOLID
+            bnd = mk == capi.CELL_BOUNDARY
+            link = np.zeros(64, bool)
+            if onb[b] >= 0:
+                link = Xb[q, onb[b] * 64 + t] > 0
+            spurious += int((bnd & link & ~solid).sum())
+            missing += int((bnd & ~link & solid).sum())
+            unlinked += int((fluid & ~bnd & solid).sum())
+    return spurious, missing, unlinked
+
+
+def test_links_agree_with_masks_and_the_rest_force_vanishes():
+    """The workload's noisy icosphere (5,120 triangles) on a 32^3 root with four wall levels, at rest.
+    With the reference's rules the link set differs from the (fluid, SOLID) cell pairs that plain
+    bounce-back reflects -- cut faces towards fluid partners stay links (and, on other meshes, fluid
+    cells of blocks classified SOLID are never BOUNDARY cells) -- so the static part of the momentum exchange,
+    sum 2 w_p c_p over the links, does not cancel and a fluid at rest feels a force.  With
+    AGAL_OPT_LINK_MASKS the links are exactly those pairs and the force at rest is zero to rounding
+    (the pairs cancel two by two, c and -c)."""
+    set_threads(8)
+    tri = capi.icosphere((0.3, 0.5, 0.5), 0.08, 4, 0.05, 20240)
+    forces = {}
+    for fix in (0, 1):
+        p = capi.default_params(velocity_set=capi.D3Q19, precision=capi.FP64, collision=capi.BGK, Nx=32,
+                                MAX_LEVELS_WALL=4, MAX_LEVELS_INTERIOR=1, v0=0.04 * 0.16 / 200.0,
+                                bc_preset=capi.BC_FPSC3D, bc_velocity=0.0, ic_u=0.0, P_REFINE=1 << 30,
+                                G_BIN_MAKE_LINKS=1, G_BIN_DENSITY=16, G_BIN_LEVELS=10, G_BIN_SPEC=7,
+                                S_BC_TYPE=0, S_FORCE_TYPE=0, S_FORCE_ORDER=1)
+        m = OracleMesh(p, 60000)
+        m.set_option(capi.OPT_LINK_MASKS, fix)
+        m.set_geometry(tri)
+        assert m.M_Advance_RefineNearWall() == capi.AGAL_OK
+        spurious, missing, unlinked = _pairs_vs_links(m)
+        if fix:
+            assert spurious == missing == unlinked == 0
+        else:
+            assert spurious > 0 and missing == 0
+        m.S_Advance(0)
+        forces[fix] = m.S_ComputeForces()
+        m.close()
+    assert np.abs(forces[0]).max() > 1e-6          # C_D-scale offset of several units at U = 0.04
+    assert np.abs(forces[1]).max() < 1e-15

