@@ -254,14 +254,25 @@ __host__ __device__ constexpr uint16_t bf16_of_int(int v)  // exact for |v| < 25
     return (uint16_t)(sign | ((uint32_t)(127 + e) << 7) | mant);
 }
 
+__host__ __device__ constexpr uint16_t f16_of_int(int v)  // exact for |v| <= 2048
+{
+    if (v == 0) return 0;
+    const uint32_t sign = v < 0 ? 0x8000u : 0u;
+    const uint32_t a = (uint32_t)(v < 0 ? -v : v);
+    int e = 0;
+    while ((a >> e) > 1u) e++;
+    const uint32_t mant = (a << (10 - e)) & 0x3FFu;
+    return (uint16_t)(sign | ((uint32_t)(15 + e) << 10) | mant);
+}
+
 __host__ __device__ constexpr int mfma_moment_of(int q, int s) { return s < 4 ? 4 * q + s : 16 + 4 * q + (s - 4); }
 
 template <int Q>
 struct alignas(16) MfmaTab {
     uint16_t fw[64][2][8];  // forward A fragments (bf16 bits) per lane and moment half
     uint16_t bk[64][2][8];  // back A fragments per lane and direction half
-    float invD[64][2][4];   // 1 / sum_p M_kp^2 of the C row in register r
-    uint8_t code[64][2][4]; // rate code of that row: 0 conserved, 1 omega, 2..7 s1..s6
+    uint16_t fw16[64][2][8];  // the same fragments as binary16 bits
+    uint16_t bk16[64][2][8];
 };
 
 template <int Q>
@@ -269,11 +280,9 @@ constexpr MfmaTab<Q> make_mfma_tab()
 {
     MfmaTab<Q> t{};
     int M[32][32] = {};
-    int D[32] = {};
     for (int k = 0; k < Q; k++)
         for (int p = 0; p < Q; p++) {
             M[k][p] = Mrt<Q>::m(k, p);
-            D[k] += M[k][p] * M[k][p];
         }
     uint16_t Mb[32][32] = {};
     for (int k = 0; k < 32; k++)
@@ -285,11 +294,8 @@ constexpr MfmaTab<Q> make_mfma_tab()
                 const int i = 16 * h + j;
                 t.fw[l][h][s] = Mb[i][8 * q + s];
                 t.bk[l][h][s] = Mb[mfma_moment_of(q, s)][i];
-            }
-            for (int r = 0; r < 4; r++) {
-                const int i = 16 * h + 4 * q + r;
-                t.code[l][h][r] = (uint8_t)(i < Q ? Mrt<Q>::rate(i) : 0);
-                t.invD[l][h][r] = i < Q ? (float)(1.0 / (double)D[i]) : 0.0f;
+                t.fw16[l][h][s] = f16_of_int(M[i][8 * q + s]);
+                t.bk16[l][h][s] = f16_of_int(M[mfma_moment_of(q, s)][i]);
             }
         }
     }

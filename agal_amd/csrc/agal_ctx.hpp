@@ -91,7 +91,6 @@ struct Ctx {
     int *id_set = nullptr, *gap_set = nullptr, *iface = nullptr;
     // Per level, the same blocks as id_set in octree traversal order (the fused step's list).
     int *order = nullptr;
-    std::vector<int> root_zorder;  // level 0's IDs in Z order (host cache; cleared when the pool is replaced)
     // Rebuild scratch.
     unsigned *map_stamp = nullptr;
     int *first_child = nullptr, *scan_a = nullptr, *scan_b = nullptr, *partials = nullptr;

@@ -1595,7 +1595,6 @@ int agal_import_pool(agal_ctx *ctx, const agal_pool *pl)
         return AGAL_ERR_ARG;
     const long long nb = pl->n_blocks;
     if (nb < c->n_coarsecblocks || nb > nmax || pl->n_gaps < 0 || pl->n_gaps > nmax) return AGAL_ERR_ARG;
-    c->root_zorder.clear();  // the imported root level may place its IDs differently
     long long total = 0;
     for (int L = 0; L < ML; L++) {
         if (pl->n_ids[L] < 0 || pl->n_ids[L] > nb) return AGAL_ERR_ARG;

@@ -149,7 +149,6 @@ struct StepArgs {
     T omeg, omegp, U;
     int mrt;      // collision operator: 0 BGK, 1 MRT
     T ck[27];     // MRT: rate_k / D_k per moment row (0 for conserved rows)
-    T rates[8];   // MRT rates by code: 0, omega, s1..s6 (the matrix-core path)
     int bc_preset;
     int *exc_idx;
     T *exc_val;
@@ -283,12 +282,18 @@ __device__ __forceinline__ void collide_cell(const StepArgs<T, S> &a, const T *f
 }
 
 // MRT transform on matrix cores for one 3D block per wavefront (Q = 19, 27; FP32 compute):
-// g = M^T (c . M d) for the non-equilibrium parts d of the 64 cells.  d and c . M d are split into
-// two bf16 terms each (hi + lo carry 16 significant bits; the products with the integer matrix are
-// exact and the sums FP32), so the result agrees with the vector path to about 2^-17 of d, below the
-// FP32 rounding of the populations themselves.  `lds` is this wavefront's 64 x 36-float region;
-// only the wavefront itself touches it.
+// g = M^T (c . M d) for the non-equilibrium parts d of the 64 cells.  FP32 pools: d and c . M d are
+// split into two bf16 terms each (hi + lo carry 16 significant bits; the products with the integer
+// matrix are exact and the sums FP32), so the result agrees with the vector path to about 2^-17 of
+// d, below the FP32 rounding of the populations themselves.  16-bit pools (F16): one binary16 term
+// of 2^10 d and of 2^10 c . M d each (11 significant bits, the scale keeps small parts out of the
+// subnormal range), products exact and sums FP32, on v_mfma_f32_16x16x32_f16: an error of about
+// 2^-11 of the non-equilibrium part, below the binary16 rounding of the stored deviation f - w,
+// which is larger than that part, and the correction goes back through LDS in binary16.  `lds` is
+// this wavefront's region of 64 rows (36 floats; 80 bytes with 16-bit pools); only the wavefront
+// itself touches it.
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
+typedef __attribute__((ext_vector_type(8))) _Float16 f16x8_t;
 typedef __attribute__((ext_vector_type(4))) float f32x4_t;
 typedef __attribute__((ext_vector_type(4))) unsigned u32x4_t;
 
@@ -312,19 +317,34 @@ __device__ __forceinline__ __bf16 split_term(float &rem)
     return h;
 }
 
-template <int Q>
-__device__ __forceinline__ void mrt_mfma(const float *d, float *g, const float *rates, float *lds)
+template <int Q, bool F16>
+__device__ __forceinline__ void mrt_mfma(const float *d, float *g, const float *ckrow, float *lds)
 {
+    using VT = typename std::conditional<F16, f16x8_t, bf16x8_t>::type;
+    constexpr int SPLITS = F16 ? 1 : MFMA_SPLITS;
     const MfmaTab<Q> &tab = *(Q == 19 ? reinterpret_cast<const MfmaTab<Q> *>(&kMfma19)
                                       : reinterpret_cast<const MfmaTab<Q> *>(&kMfma27));
     const int l = threadIdx.x & 63, q = l >> 4, j = l & 15;
-    const u32x4_t *fw = reinterpret_cast<const u32x4_t *>(&tab.fw[l][0][0]);
-    const u32x4_t *bk = reinterpret_cast<const u32x4_t *>(&tab.bk[l][0][0]);
-    const bf16x8_t afw0 = __builtin_bit_cast(bf16x8_t, fw[0]), afw1 = __builtin_bit_cast(bf16x8_t, fw[1]);
-    const bf16x8_t abk0 = __builtin_bit_cast(bf16x8_t, bk[0]), abk1 = __builtin_bit_cast(bf16x8_t, bk[1]);
-    bf16x8_t *ldsv = reinterpret_cast<bf16x8_t *>(lds);  // rows of 5 vectors (80 bytes)
+    const u32x4_t *fw = reinterpret_cast<const u32x4_t *>(F16 ? &tab.fw16[l][0][0] : &tab.fw[l][0][0]);
+    const u32x4_t *bk = reinterpret_cast<const u32x4_t *>(F16 ? &tab.bk16[l][0][0] : &tab.bk[l][0][0]);
+    const VT afw0 = __builtin_bit_cast(VT, fw[0]), afw1 = __builtin_bit_cast(VT, fw[1]);
+    const VT abk0 = __builtin_bit_cast(VT, bk[0]), abk1 = __builtin_bit_cast(VT, bk[1]);
+    VT *ldsv = reinterpret_cast<VT *>(lds);  // rows of 5 vectors (80 bytes)
+    auto mfma = [](const VT &a, const VT &b, const f32x4_t &c) {
+        if constexpr (F16)
+            return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+        else
+            return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    };
+    auto term = [](float &rem) {
+        if constexpr (F16) {
+            return (_Float16)rem;
+        } else {
+            return split_term(rem);
+        }
+    };
 
-    // Forward: one pass per bf16 term of d through a 64-row transpose region.
+    // Forward: one pass per term of d through a 64-row transpose region.
     f32x4_t C0[4], C1[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) C0[t] = C1[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
@@ -332,36 +352,36 @@ __device__ __forceinline__ void mrt_mfma(const float *d, float *g, const float *
 #pragma unroll
     for (int k = 0; k < 32; k++) rem[k] = k < Q ? d[k] : 0.f;
 #pragma unroll
-    for (int sp = 0; sp < MFMA_SPLITS; sp++) {
+    for (int sp = 0; sp < SPLITS; sp++) {
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            bf16x8_t v;
+            VT v;
 #pragma unroll
-            for (int e = 0; e < 8; e++) v[e] = split_term(rem[8 * c + e]);
+            for (int e = 0; e < 8; e++) v[e] = term(rem[8 * c + e]);
             ldsv[5 * l + c] = v;
         }
         wave_lds_sync();
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            const bf16x8_t b = ldsv[5 * (16 * t + j) + q];
-            C0[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afw0, b, C0[t], 0, 0, 0);
-            C1[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afw1, b, C1[t], 0, 0, 0);
+            const VT b = ldsv[5 * (16 * t + j) + q];
+            C0[t] = mfma(afw0, b, C0[t]);
+            C1[t] = mfma(afw1, b, C1[t]);
         }
         wave_lds_sync();
     }
-    // Relaxation factors of the rows this lane holds.
+    // Relaxation factors rate / D of the rows this lane holds (rows 4q + r and 16 + 4q + r, zero
+    // beyond the velocity set), picked from the level's row table by the lane's quarter q.
     float ck0[4], ck1[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        const int c0 = tab.code[l][0][r], c1 = tab.code[l][1][r];
-        float r0 = 0.f, r1 = 0.f;
+        float a0[4], a1[4];
 #pragma unroll
-        for (int cc = 1; cc < 8; cc++) {
-            r0 = c0 == cc ? rates[cc] : r0;
-            r1 = c1 == cc ? rates[cc] : r1;
+        for (int qq = 0; qq < 4; qq++) {
+            a0[qq] = 4 * qq + r < Q ? ckrow[4 * qq + r] : 0.f;
+            a1[qq] = 16 + 4 * qq + r < Q ? ckrow[16 + 4 * qq + r] : 0.f;
         }
-        ck0[r] = r0 * tab.invD[l][0][r];
-        ck1[r] = r1 * tab.invD[l][1][r];
+        ck0[r] = q == 0 ? a0[0] : (q == 1 ? a0[1] : (q == 2 ? a0[2] : a0[3]));
+        ck1[r] = q == 0 ? a1[0] : (q == 1 ? a1[1] : (q == 2 ? a1[2] : a1[3]));
     }
     // Back: B operand straight from the forward results of the same lane.
     f32x4_t G0[4], G1[4];
@@ -375,24 +395,52 @@ __device__ __forceinline__ void mrt_mfma(const float *d, float *g, const float *
             v[4 + r] = C1[t][r] * ck1[r];
         }
 #pragma unroll
-        for (int sp = 0; sp < MFMA_SPLITS; sp++) {
-            bf16x8_t b;
+        for (int sp = 0; sp < SPLITS; sp++) {
+            VT b;
 #pragma unroll
-            for (int e = 0; e < 8; e++) b[e] = split_term(v[e]);
-            G0[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abk0, b, G0[t], 0, 0, 0);
-            G1[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abk1, b, G1[t], 0, 0, 0);
+            for (int e = 0; e < 8; e++) b[e] = term(v[e]);
+            G0[t] = mfma(abk0, b, G0[t]);
+            G1[t] = mfma(abk1, b, G1[t]);
         }
     }
-    // Transpose back: G row p of cell 16t + j -> LDS row of that cell (36 floats), then own row.
+    if constexpr (F16) {
+        // Transpose back in binary16 (the scaled correction, 11 significant bits, far below the
+        // storage step of the deviation it corrects): rows of 40 halves (80 bytes) per cell.
+        typedef __attribute__((ext_vector_type(4))) _Float16 f16x4_t;
+        _Float16 *ldsh = reinterpret_cast<_Float16 *>(lds);
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-        *reinterpret_cast<f32x4_t *>(lds + 36 * (16 * t + j) + 4 * q) = G0[t];
-        *reinterpret_cast<f32x4_t *>(lds + 36 * (16 * t + j) + 16 + 4 * q) = G1[t];
+        for (int t = 0; t < 4; t++) {
+            f16x4_t a, b;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                a[r] = (_Float16)G0[t][r];
+                b[r] = (_Float16)G1[t][r];
+            }
+            *reinterpret_cast<f16x4_t *>(ldsh + 40 * (16 * t + j) + 4 * q) = a;
+            *reinterpret_cast<f16x4_t *>(ldsh + 40 * (16 * t + j) + 16 + 4 * q) = b;
+        }
+        wave_lds_sync();
+        const f16x8_t *row = reinterpret_cast<const f16x8_t *>(ldsh + 40 * l);
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const f16x8_t h = row[c];
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                if (8 * c + e < Q) g[8 * c + e] = (float)h[e];
+        }
+        wave_lds_sync();
+    } else {
+        // Transpose back: G row p of cell 16t + j -> LDS row of that cell (36 floats), then own row.
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            *reinterpret_cast<f32x4_t *>(lds + 36 * (16 * t + j) + 4 * q) = G0[t];
+            *reinterpret_cast<f32x4_t *>(lds + 36 * (16 * t + j) + 16 + 4 * q) = G1[t];
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int p = 0; p < Q; p++) g[p] = lds[36 * l + p];
+        wave_lds_sync();
     }
-    wave_lds_sync();
-#pragma unroll
-    for (int p = 0; p < Q; p++) g[p] = lds[36 * l + p];
-    wave_lds_sync();
 }
 
 // Population load of the fused step.  FP32/FP64 rows are read with non-temporal loads: they are
@@ -677,17 +725,27 @@ __device__ __forceinline__ void collide_stream_3d(const StepArgs<T, S> &a, int w
     moments<T, Q>(f, rho, u, v, w);
     if constexpr (MF) {
         // MRT with the moment transform on matrix cores (whole wavefront, inert lanes carry zeros).
-        const T udotu = u * u + v * v + w * w;
+        // The non-equilibrium parts go in scaled by SC (16-bit pools: 2^10, see mrt_mfma) and the
+        // correction comes back scaled by SC; the equilibrium in pairs (p, pb(p)) with fused
+        // multiply-adds, SC folded into the density.
+        constexpr float SC = ST::quantized ? 1024.0f : 1.0f;
+        const T c1 = (T)(1.0) - (T)(1.5) * (u * u + v * v + w * w);
+        const T rs = rho * (T)SC;
         float d[Q], g[Q];
+        d[0] = inert ? 0.f : fmadd(f[0], (T)SC, -((T)VS::w(0) * rs * c1));
 #pragma unroll
-        for (int p = 0; p < Q; p++) {
-            const T cdotu = cdot<T, Q>(p, u, v, w);
-            const T feq = (T)VS::w(p) * rho * ((T)(1.0) + (T)(3.0) * cdotu + (T)(4.5) * cdotu * cdotu - (T)(1.5) * udotu);
-            d[p] = inert ? 0.f : (float)(f[p] - feq);
+        for (int p = 1; p < Q; p++) {
+            if (!VS::half(p)) continue;
+            const int q = VS::pb(p);
+            const T cu = cdot<T, Q>(p, u, v, w);
+            const T b = fmadd((T)(4.5) * cu, cu, c1);
+            const T wr = (T)VS::w(p) * rs;
+            d[p] = inert ? 0.f : fmadd(f[p], (T)SC, -(wr * fmadd((T)(3.0), cu, b)));
+            d[q] = inert ? 0.f : fmadd(f[q], (T)SC, -(wr * fmadd((T)(-3.0), cu, b)));
         }
-        mrt_mfma<Q>(d, g, a.rates, lds);
+        mrt_mfma<Q, ST::quantized>(d, g, a.ck, lds);
 #pragma unroll
-        for (int p = 0; p < Q; p++) fs[p] = inert ? f[VS::pb(p)] : f[p] - (T)g[p];
+        for (int p = 0; p < Q; p++) fs[p] = inert ? f[VS::pb(p)] : fmadd((T)g[p], (T)(-1.0f / SC), f[p]);
     } else {
         collide_cell<T, S, Q>(a, f, fs, rho, u, v, w, inert);
     }
@@ -1375,12 +1433,19 @@ template <typename T, typename S, int Q, bool MF>
 __global__ __launch_bounds__(WG) void k_collide_stream(MultiStep<T, S> m, StepExtras<T, S> x)
 {
     using VS = VelSet<Q>;
+    using IL = InterpLds<T, Q>;
+    // One LDS region shared by the two workgroup kinds of the launch: the interpolation's parent
+    // rows and coefficients, or the matrix-core MRT's per-wavefront transpose rows.
+    constexpr int ROW = Store<T, S>::quantized ? 20 : 36;
+    constexpr int ALL16 = (IL::ALL * (int)sizeof(T) + 15) / 16 * 16 / (int)sizeof(T);  // coefficients 16-byte aligned
+    constexpr size_t INTERP_BYTES = sizeof(T) * (ALL16 + IL::COEF);
+    constexpr size_t MRT_BYTES = MF ? sizeof(float) * (WG / 64) * 64 * ROW : 0;
+    __shared__ __attribute__((aligned(16))) unsigned char s_raw[INTERP_BYTES > MRT_BYTES ? INTERP_BYTES : MRT_BYTES];
     if ((int)blockIdx.x < x.lead) {
         const int w2 = blockIdx.x;
         if (w2 < x.interp_wgs) {
-            using IL = InterpLds<T, Q>;
-            __shared__ T s_all[IL::ALL];
-            __shared__ T s_coef[IL::COEF];
+            T *s_all = reinterpret_cast<T *>(s_raw);
+            T *s_coef = s_all + ALL16;
             interp_parents<T, S, Q, false>(x.ia, w2, s_all, s_coef);
         } else if (w2 < x.interp_wgs + x.fix_wgs) {
             fixup_level<T, S, Q>(x.fa, w2 - x.interp_wgs);
@@ -1398,9 +1463,10 @@ __global__ __launch_bounds__(WG) void k_collide_stream(MultiStep<T, S> m, StepEx
     const StepArgs<T, S> &a = m.lv[k];
     const int wave = slot - m.start[k];
     if constexpr (MF) {
-        // One 64 x 36-float transpose region per wavefront for the matrix-core MRT.
-        __shared__ float s_mrt[(WG / 64) * 64 * 36];
-        collide_stream_3d<T, S, Q, true>(a, wave, s_mrt + (threadIdx.x >> 6) * 64 * 36);
+        // One transpose region per wavefront for the matrix-core MRT: 64 rows of 36 floats, or of
+        // 20 floats (80 bytes: the forward rows, and the back rows in binary16) with 16-bit pools.
+        float *s_mrt = reinterpret_cast<float *>(s_raw);
+        collide_stream_3d<T, S, Q, true>(a, wave, s_mrt + (threadIdx.x >> 6) * 64 * ROW);
     } else if constexpr (VS::DIM == 3) {
         collide_stream_3d<T, S, Q, false>(a, wave, nullptr);
     } else {
@@ -1779,9 +1845,6 @@ static StepArgs<T, S> step_args(Ctx *c, int L, int rd, int vmode = 0)
             const T rate = rc == 1 ? a.omeg : (T)s[rc - 2];
             a.ck[k] = (T)((double)rate / (double)mrt_norm<Q>(k));
         }
-        a.rates[0] = (T)0;
-        a.rates[1] = a.omeg;
-        for (int k = 0; k < 6; k++) a.rates[2 + k] = (T)s[k];
     }
     a.U = (T)c->P.bc_velocity;
     a.bc_preset = c->P.bc_preset;

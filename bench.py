@@ -504,13 +504,15 @@ def sphere_study(args):
     the reference reports it (block sums x dx^3, total / dx of the finest level).
 
     First the force on the body in a fluid at rest (one step), with the reference's link rules and
-    with links agreeing with the masks.  Then FP64 (vector MRT), FP32 (vector MRT and matrix-core
-    MRT) and FP16 storage of f - w (vector MRT) run the same static mesh from the same uniform start
-    in lockstep; leaf velocities are compared with FP64 every 250 coarse steps and C_D is averaged
-    over the second half of the run (sampled every 10 coarse steps).  The run ends at 1000 coarse
-    steps (6.25 diameters of travel): the wake reaches the outflow face about 1100 coarse steps
-    after the start, and from there the outflow's anti-bounce-back rule feeds disturbances back
-    into the level-0 region."""
+    with links agreeing with the masks.  Then FP64 (vector MRT), FP32 (vector and matrix-core MRT)
+    and FP16 storage of f - w (vector and matrix-core MRT) run the same static mesh from the same
+    uniform start in lockstep, with a BGK pair (FP64, FP16) as the control; leaf velocities are
+    compared with the FP64 run of the same collision every 250 coarse steps, and C_D is averaged
+    over coarse steps 250-500 and over the second half of the run (sampled every 10 coarse steps).
+    The run ends at 1000 coarse steps (6.25 diameters of travel): the wake reaches the outflow face
+    about 1100 coarse steps after the start.  With these MRT rates small differences between runs
+    grow by about 10^3 per 500 coarse steps from step 500 on (BGK: about 10 per 500), so the early
+    window is where the precisions are compared on equal flows."""
     import math
 
     from agal_amd import capi
@@ -526,10 +528,12 @@ def sphere_study(args):
                        "on links agreeing with the cell masks",
            "cd_formula": "8 F_x / (U^2 pi D^2), F = agal_forces (S_ReportForces scaling)",
            "coarse_steps": args.sphere_steps, "mlups": {}, "cd": {}, "fused_kernel": {},
-           "mrt_transform": {"fp64": "vector", "fp32": "vector", "fp32_mfma": "matrix cores", "fp16": "vector"}}
+           "mrt_transform": {"fp64": "vector", "fp32": "vector", "fp32_mfma": "matrix cores (bf16 two-term)",
+                             "fp16": "vector", "fp16_mfma": "matrix cores (binary16 one term)"},
+           "bgk_control": "bgk_fp64 / bgk_fp16: the same flow with BGK collision, compared with each other"}
 
-    def params(prec, u):
-        return capi.default_params(velocity_set=capi.D3Q27, precision=prec, collision=capi.MRT, Nx=N,
+    def params(prec, u, coll=capi.MRT):
+        return capi.default_params(velocity_set=capi.D3Q27, precision=prec, collision=coll, Nx=N,
                                    MAX_LEVELS_WALL=levels, MAX_LEVELS_INTERIOR=1, v0=U * D / Re,
                                    bc_preset=capi.BC_FPSC3D, bc_velocity=u, ic_u=u, P_REFINE=1 << 30,
                                    G_BIN_MAKE_LINKS=1, G_BIN_DENSITY=16, G_BIN_LEVELS=10, G_BIN_SPEC=7,
@@ -549,9 +553,12 @@ def sphere_study(args):
     out["rest_force_in_cd_units"] = rest
     log(f"sphere: force at rest in C_D units {rest}")
     ctx = {}
-    for k, prec in (("fp64", capi.FP64), ("fp32", capi.FP32), ("fp32_mfma", capi.FP32), ("fp16", capi.FP16)):
-        m = LbmMesh(params(prec, U), args.capacity)
-        if k == "fp32_mfma":
+    runs = (("fp64", capi.FP64, capi.MRT, 0), ("fp32", capi.FP32, capi.MRT, 0), ("fp32_mfma", capi.FP32, capi.MRT, 1),
+            ("fp16", capi.FP16, capi.MRT, 0), ("fp16_mfma", capi.FP16, capi.MRT, 1),
+            ("bgk_fp64", capi.FP64, capi.BGK, 0), ("bgk_fp16", capi.FP16, capi.BGK, 0))
+    for k, prec, coll, mf in runs:
+        m = LbmMesh(params(prec, U, coll), args.capacity)
+        if mf:
             m.set_option(capi.OPT_MRT_MFMA, 1)
         m.set_option(capi.OPT_LINK_MASKS, 1)
         m.set_geometry(tri)
@@ -582,24 +589,32 @@ def sphere_study(args):
         if done % 250 == 0 or done == K:
             row = {"step": done}
             for k in ctx:
-                if k != "fp64":
-                    row[k] = float(f"{velocity_error(ctx['fp64'], ctx[k])['rel_l2_velocity']:.3g}")
+                if k not in ("fp64", "bgk_fp64"):
+                    ref = "bgk_fp64" if k.startswith("bgk") else "fp64"
+                    row[k] = float(f"{velocity_error(ctx[ref], ctx[k])['rel_l2_velocity']:.3g}")
             series.append(row)
             cd_series.append({"step": done, **{k: round(float(sum(cds[k][-25:]) / len(cds[k][-25:])), 4) for k in ctx}})
             log(f"sphere: {done} coarse steps, velocity error vs fp64 {row}, C_D {cd_series[-1]}")
     nwin = (K - avg_from) // 10
+    out["cd_early"] = {}
     for k, m in ctx.items():
         out["mlups"][k] = round(upd * K / (ms[k] * 1e-3) / 1e6, 1)
         out["cd"][k] = round(sum(cds[k][-nwin:]) / nwin, 5)
+        out["cd_early"][k] = round(sum(cds[k][25:50]) / 25, 5) if len(cds[k]) >= 50 else None
         fms, nl, fc = roofline_pass(m, 5)
-        b = {"fp64": 8, "fp32": 4, "fp32_mfma": 4, "fp16": 2}[k]
+        b = 8 if "fp64" in k else (2 if "fp16" in k else 4)
         out["fused_kernel"][k] = {"avg_launch_ms": round(fms / max(1, nl), 5),
                                   "achieved_gbs": round(2 * 27 * b * fc / (fms * 1e-3) / 1e9, 1)
                                   if fms > 0 else None}
     out["cd_window"] = f"mean over coarse steps {avg_from}-{K}, sampled every 10"
+    out["cd_early_window"] = "mean over coarse steps 250-500, sampled every 10"
     out["vel_err_vs_fp64_series"] = series
     out["cd_series_250_step_means"] = cd_series
-    out["cd_rel_err_vs_fp64"] = {k: float(f"{abs(out['cd'][k] / out['cd']['fp64'] - 1):.3g}") for k in ctx if k != "fp64"}
+    ref_of = lambda k: "bgk_fp64" if k.startswith("bgk") else "fp64"
+    out["cd_rel_err_vs_fp64"] = {k: float(f"{abs(out['cd'][k] / out['cd'][ref_of(k)] - 1):.3g}") for k in ctx
+                                 if k not in ("fp64", "bgk_fp64")}
+    out["cd_early_rel_err_vs_fp64"] = {k: float(f"{abs(out['cd_early'][k] / out['cd_early'][ref_of(k)] - 1):.3g}")
+                                       for k in ctx if k not in ("fp64", "bgk_fp64") and out["cd_early"][k]}
     out["cd_schiller_naumann_unbounded"] = round(24.0 / Re * (1 + 0.15 * Re ** 0.687), 4)
     for m in ctx.values():
         m.close()

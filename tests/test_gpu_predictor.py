@@ -19,7 +19,7 @@ from agal_amd import capi
 from agal_amd.mesh import LbmMesh
 from amr_checks import check_state_fast
 from oracle_lib import MlpOracle, OracleMesh
-from parity import compare_states, ldc_params
+from parity import body, compare_states, ldc_params, sphere_params
 
 pytestmark = pytest.mark.gpu
 
@@ -48,6 +48,30 @@ def test_features_and_labels_match_the_restatement():
     assert (lg == 1).sum() > 0 and (lg == 0).sum() > 0
     # Labels leave the refinement marks untouched.
     assert np.array_equal(g.block_ints(capi.BLK_REF), o.block_ints(capi.BLK_REF))
+
+
+def test_features_on_the_sphere_hierarchy_use_the_geometry_distance():
+    """Features of the sphere flow's wall hierarchy (a body set): the signed distance feature (5) is
+    the distance to the nearest triangle centroid, negative inside the body -- identical inputs,
+    features to 1e-6 of the restatement's, both signs present."""
+    tri = body(center=(0.3, 0.5, 0.5), radius=0.08, subdivisions=3, noise=0.05, seed=1)
+    p = sphere_params(vs=capi.D3Q19, N=16, levels=3, collision=capi.BGK, D=0.16)
+    g, o = LbmMesh(p, 20000), OracleMesh(p, 20000)
+    for m in (g, o):
+        m.set_geometry(tri)
+        m.M_Advance_RefineNearWall()
+        for _ in range(3):
+            m.S_Advance(0)
+        for L in range(m.sizes.max_levels):
+            m.S_RefreshVariables(L)
+    g.sync()
+    assert not compare_states(g.state(), o.state())
+    g.predictor_init(7, 1024)
+    fg, fo = g.predictor_features(), o.predictor_features()
+    assert np.abs(fg - fo).max() <= 1e-6
+    act = g.block_ints(capi.BLK_REF) != capi.REF_INACTIVE
+    d = fg[act, 5]
+    assert (d < 0).any() and (d > 0).any()
 
 
 def _rand_params(rng):
