@@ -495,6 +495,17 @@ __device__ __forceinline__ void store_pop(S *row, unsigned idx, S v)
     }
 }
 
+// Store of a 16-bit value at a byte offset within a row (the same pointer on every lane).
+template <typename S>
+__device__ __forceinline__ void store_pop_bytes(S *row, unsigned byte_off, S v)
+{
+    static_assert(sizeof(S) == 2, "16-bit rows only");
+    const unsigned long long r = reinterpret_cast<unsigned long long>(row);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)r), hi = __builtin_amdgcn_readfirstlane((unsigned)(r >> 32));
+    auto *base = (__attribute__((address_space(1))) char *)(((unsigned long long)hi << 32) | lo);
+    *(__attribute__((address_space(1))) uint16_t *)(base + byte_off) = __builtin_bit_cast(uint16_t, v);
+}
+
 // One link of the fused step for the cell `cell`, direction q >= 1, whose neighbour block in that
 // direction is nbk (target cell tgt): boundary hook, swap guard, and the single store of f*_q
 // either into the neighbour (the link streams) or into the cell's own opposite slot (bounce-back).
@@ -553,7 +564,11 @@ __device__ __forceinline__ void push_interior(const StepArgs<T, S> &a, const T *
     const int xp = I == 3 ? 4 : 0, xm = I == 0 ? -4 : 0;
     const int yp = J == 3 ? 12 : 0, ym = J == 0 ? -12 : 0;
     const int zp = K == 3 ? 36 : 0, zm = K == 0 ? -36 : 0;
-    const unsigned tl = (unsigned)t & 0x15u, th = (unsigned)t & 0x2Au;
+    // 16-bit rows: the target as a byte offset within the row (block << 7 | cell << 1), the same
+    // field-wise sum on fields shifted up by one bit, so the store needs no further shift.
+    constexpr int SH = sizeof(S) == 2 ? 1 : 0;
+    const unsigned tl = ((unsigned)t & 0x15u) << SH, th = ((unsigned)t & 0x2Au) << SH;
+    const int nb_src = (int)((unsigned)my_nb << (6 + SH));
 #pragma unroll
     for (int q = 1; q < Q; q++) {
         const int cx = VS::cx(q), cy = VS::cy(q), cz = VS::cz(q);
@@ -564,20 +579,25 @@ __device__ __forceinline__ void push_interior(const StepArgs<T, S> &a, const T *
         if (cy < 0) sa += ym;
         if (cz > 0) sa += zp;
         if (cz < 0) sa += zm;
-        const unsigned nbk = (unsigned)__builtin_amdgcn_ds_bpermute(sa, my_nb);
+        const unsigned nbk = (unsigned)__builtin_amdgcn_ds_bpermute(sa, nb_src);
         // ((I+cx)&3) + 4((J+cy)&3) + 16((K+cz)&3): field-wise sum without carries between fields.
         const unsigned d = (unsigned)((cx & 3) | ((cy & 3) << 2) | ((cz & 3) << 4));
-        const unsigned tgt = (nbk << 6) + ((tl + (d & 0x15u)) ^ th ^ (d & 0x2Au));
+        const unsigned tgt_s = nbk + ((tl + ((d & 0x15u) << SH)) ^ th ^ ((d & 0x2Au) << SH));
         const S e = ST::enc(fs[q], (T)VS::w(q));
         bool push = true;
         if (!VS::half(q)) {
             // Partner of the link initiated by the cell x + c_q: the guard on the stored value.
             if (!ST::positive(e, (T)VS::w(q))) {
                 push = false;
-                record_exception(a, (long long)tgt, (long long)cell, q, ST::dec(e, (T)VS::w(q)));
+                record_exception(a, (long long)(tgt_s >> SH), (long long)cell, q, ST::dec(e, (T)VS::w(q)));
             }
         }
-        if (push) store_pop(a.B + (long long)q * a.ncm, tgt, e);
+        if (push) {
+            if constexpr (SH == 1)
+                store_pop_bytes(a.B + (long long)q * a.ncm, tgt_s, e);
+            else
+                store_pop(a.B + (long long)q * a.ncm, tgt_s, e);
+        }
     }
 }
 
