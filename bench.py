@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--develop", type=int, default=1000,
                     help="coarse steps from rest with rebuilds before the warm-up (builds the adaptive mesh)")
     ap.add_argument("--capacity", type=int, default=400000, help="block pool size of the 64^3 cavity")
+    ap.add_argument("--storage-develop", type=int, default=10000,
+                    help="coarse steps of the FP64/FP32/FP16/BF16 runs before the velocity comparison (configuration 2 runs 10 000)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample (both legs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 five-level cavity leg")
@@ -421,12 +423,12 @@ def storage_study(p, args):
         ctx[k].S_SetIC(0)
         ctx[k].M_FreezeRefinedCells(0)
     t0 = time.time()
-    advance_twins(ctx["fp64"], [ctx["fp32"], ctx["fp16"], ctx["bf16"]], 0, args.develop)
+    advance_twins(ctx["fp64"], [ctx["fp32"], ctx["fp16"], ctx["bf16"]], 0, args.storage_develop)
     warm_s = time.time() - t0
     errs = {k: velocity_error(ctx["fp64"], ctx[k]) for k in ("fp32", "fp16", "bf16")}
     n_ids = blocks(ctx["fp64"])
     upd = updates_per_step(n_ids)
-    out = {"develop_steps": args.develop, "develop_seconds": round(warm_s, 2), "blocks_per_level": n_ids,
+    out = {"develop_steps": args.storage_develop, "develop_seconds": round(warm_s, 2), "blocks_per_level": n_ids,
            "cell_updates_per_step": upd, "vel_err_vs_fp64": {}, "mlups": {}, "fused_kernel": {}}
     bpp = {"fp64": 8, "fp32": 4, "fp16": 2, "bf16": 2}
     K = min(max(args.steps, 20), p.P_REFINE - 1)  # no rebuild inside the window
@@ -434,7 +436,7 @@ def storage_study(p, args):
         m = ctx[k]
         log(f"storage study: timing {k}")
         m.sync()
-        _, step_ms, _ = m.advance_loop(args.develop, K)
+        _, step_ms, _ = m.advance_loop(args.storage_develop, K)
         m.sync()
         out["mlups"][k] = round(upd * K / (step_ms * 1e-3) / 1e6, 1)
         fms, nl, fc = roofline_pass(m, max(1, args.roofline_steps // 2))

@@ -4,7 +4,10 @@
   100 coarse steps: the HIP library and the CPU restatement run the same driver through two
   solution-based rebuilds; the full state is compared bit for bit and the block pool checked
   against brute force.
-* D2Q9 cavity 128^2, Re 100, FP32 and FP64.
+* D2Q9 cavity 128^2, Re 100, FP32 and FP64, its full 20 000 steps.
+* The sphere flow at its own size (D3Q27 MRT, root 64^3, four wall levels, the 5 120-triangle noisy
+  icosphere, links that agree with the cell masks): hierarchy, masks, links, two coarse steps and
+  the force.
 * Size-independent property at 256^3 (16.7 M cells): on a periodic grid a state that repeats every
   8 cells evolves as the 8^3 periodic state does, so every cell of the large GPU run must equal the
   matching cell of the small CPU run, bit for bit.
@@ -55,9 +58,40 @@ def test_d2q9_ldc128(prec):
     g, o = LbmMesh(p, 1100), OracleMesh(p, 1100)
     for m in (g, o):
         m.S_SetIC(0)
-        m.advance_loop(0, 3000)
+        m.advance_loop(0, 20000)
     bad = compare_states(g.state(), o.state())
     assert not bad, bad
+
+
+def test_sphere_flow_full_size():
+    set_threads(16)
+    print("sphere flow 64^3, 4 wall levels, D3Q27 MRT FP32", flush=True)
+    tri = capi.icosphere((0.3, 0.5, 0.5), 0.08, 4, 0.05, 20240)
+    U, D = 0.04, 0.16
+    p = capi.default_params(velocity_set=capi.D3Q27, precision=capi.FP32, collision=capi.MRT, Nx=64,
+                            MAX_LEVELS_WALL=4, MAX_LEVELS_INTERIOR=1, v0=U * D / 200.0, bc_preset=capi.BC_FPSC3D,
+                            bc_velocity=U, ic_u=U, P_REFINE=1 << 30, G_BIN_MAKE_LINKS=1, G_BIN_DENSITY=16,
+                            G_BIN_LEVELS=10, G_BIN_SPEC=7, S_BC_TYPE=0, S_FORCE_TYPE=0, S_FORCE_ORDER=1, s6=1.2)
+    g, o = LbmMesh(p, 60000), OracleMesh(p, 60000)
+    for m in (g, o):
+        m.set_option(capi.OPT_LINK_MASKS, 1)
+        m.set_geometry(tri)
+    assert g.M_Advance_RefineNearWall() == o.M_Advance_RefineNearWall() == capi.AGAL_OK
+    g.sync()
+    assert list(g.sizes.n_ids[:4]) == [4096, 3168, 6728, 21216]
+    bad = compare_states(g.state(), o.state())
+    assert not bad, bad
+    lg, lo = g.links(), o.links()
+    np.testing.assert_array_equal(lg["X_b"].view(np.uint32), lo["X_b"].view(np.uint32))
+    for k in range(2):
+        for m in (g, o):
+            m.S_Advance(0)
+        g.sync()
+        bad = compare_states(g.state(), o.state())
+        assert not bad, (k, bad)
+        fg, fo = g.S_ComputeForces(), o.S_ComputeForces()
+        np.testing.assert_allclose(fg, fo, rtol=1e-9, atol=1e-15)
+    assert np.abs(fg).max() > 0
 
 
 @pytest.mark.parametrize("prec", [capi.FP32, capi.FP64])
