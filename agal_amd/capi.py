@@ -86,6 +86,12 @@ class Params(C.Structure):
         ("S_BC_TYPE", C.c_int),
         ("S_FORCE_TYPE", C.c_int),
         ("S_FORCE_ORDER", C.c_int),
+        ("S_FORCEVOLUME_Xm", C.c_double),
+        ("S_FORCEVOLUME_XM", C.c_double),
+        ("S_FORCEVOLUME_Ym", C.c_double),
+        ("S_FORCEVOLUME_YM", C.c_double),
+        ("S_FORCEVOLUME_Zm", C.c_double),
+        ("S_FORCEVOLUME_ZM", C.c_double),
     ]
 
 

@@ -128,6 +128,12 @@ struct Ctx {
     int lb_G = 0, lb_level = 0;
     long long lb_n_bins = 0, lb_n_ent = 0, lb_n_kept = 0;
 
+    // Control-volume force (S_FORCE_TYPE 1): the box snapped to root-block edges, in the precision's
+    // type (widened), and the per-block sums (6 x n_maxcblocks values of the compute type: Fp, Fm
+    // per direction).
+    double cv_box[6] = {};
+    void *cv_F = nullptr;
+
     Predictor pred;
 
     LevelTiming timing;
@@ -231,6 +237,8 @@ int links_reset_counts(Ctx *c);
 int compute_links(Ctx *c);
 int links_rebuild_lists(Ctx *c);
 int launch_mea_post(Ctx *c);
+// Control-volume force sums of level L, post 0 (start of a coarse step) or 1 (end).
+int launch_cv_force(Ctx *c, int L, int post);
 long long icosphere(double cx, double cy, double cz, double r, int subdiv, double noise, unsigned long long seed,
                     float *out, long long cap);
 long long load_stl_ascii(const char *path, float *out, long long cap);

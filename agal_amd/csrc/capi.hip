@@ -163,6 +163,11 @@ void compute_level_params(Ctx *c)
         c->dxf_vec[L] = (double)dxf;
         c->tau_vec[L] = (double)(T)(c->P.v0 * 3.0 + 0.5 * (double)dxf);
     }
+    // The control volume snapped to root-block edges (solver_lbm_init.cu:36-41).
+    const double fr[6] = {c->P.S_FORCEVOLUME_Xm, c->P.S_FORCEVOLUME_XM, c->P.S_FORCEVOLUME_Ym,
+                          c->P.S_FORCEVOLUME_YM, c->P.S_FORCEVOLUME_Zm, c->P.S_FORCEVOLUME_ZM};
+    for (int k = 0; k < 6; k++)
+        c->cv_box[k] = (double)(T)(4.0 * (double)(int)(fr[k] * (double)(c->Nxi[k / 2] / 4)) * c->dxf_vec[0]);
 }
 
 void free_ctx(Ctx *c)
@@ -171,7 +176,7 @@ void free_ctx(Ctx *c)
     void *ptrs[] = {c->F[0], c->F[1], c->M, c->cells_mask, c->X, c->nbr, c->nbr_child, c->nbr_slot, c->ref, c->level,
                     c->onb, c->blk_mask, c->part, c->simple, c->id_set, c->order, c->gap_set, c->iface, c->map_stamp,
                     c->first_child, c->scan_a, c->scan_b, c->partials, c->dscal, c->exc_idx, c->exc_val,
-                    c->exc_count};
+                    c->exc_count, c->cv_F};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_geometry(c);
@@ -343,11 +348,33 @@ int advance_rec(Ctx *c, int L)
 // level's launches (the held-back launches are issued before returning).
 int advance_step(Ctx *c, int L0)
 {
+    int st;
+    const bool cv = c->cv_F != nullptr;
+    const int ML = c->MAX_LEVELS;
+    const bool multi = ML > 1 && ML != L0 + 1;
+    // Control-volume force (solver_lbm_advance.cu:45-49, :139-140): the first half on every level
+    // (the single level only, without refinement) before the step.
+    const int K_end = multi ? ML : L0 + 1;
+    if (cv)
+        for (int K = L0; K < K_end; K++)
+            if ((st = launch_cv_force(c, K, 0))) return st;
     c->defer_coarse = true;
-    int st = advance_rec(c, L0);
+    st = advance_rec(c, L0);
     c->defer_coarse = false;
     const int st2 = launch_flush_deferred(c);
     if (st || st2) return st ? st : st2;
+    if (cv) {
+        // The second half (:60-68, :148-149): with refinement, after a full fine-to-coarse average
+        // and the interface interpolations of every level.
+        if (multi) {
+            for (int K = ML - 2; K >= L0; K--)
+                if ((st = launch_average(c, K, AGAL_AVERAGE_GRID))) return st;
+            for (int K = L0; K < ML - 1; K++)
+                if ((st = launch_interpolate(c, K, AGAL_INTERP_INTERFACE))) return st;
+        }
+        for (int K = L0; K < K_end; K++)
+            if ((st = launch_cv_force(c, K, 1))) return st;
+    }
     // The after-stream half of the momentum exchange, on the finished finest level.
     return launch_mea_post(c);
 }
@@ -781,6 +808,9 @@ void agal_params_default(agal_params *p)
     p->S_BC_TYPE = 0;
     p->S_FORCE_TYPE = 0;
     p->S_FORCE_ORDER = 0;
+    p->S_FORCEVOLUME_Xm = p->S_FORCEVOLUME_XM = 0.0;
+    p->S_FORCEVOLUME_Ym = p->S_FORCEVOLUME_YM = 0.0;
+    p->S_FORCEVOLUME_Zm = p->S_FORCEVOLUME_ZM = 0.0;
 }
 
 const char *agal_status_string(int s)
@@ -917,6 +947,7 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     ALLOC(c->exc_idx, sizeof(int) * 3 * c->exc_cap * (c->MAX_LEVELS + 1));
     ALLOC(c->exc_val, cs * c->exc_cap * (c->MAX_LEVELS + 1));
     ALLOC(c->exc_count, sizeof(int) * 2 * (c->MAX_LEVELS + 1));
+    if (c->P.S_FORCE_TYPE == 1) ALLOC(c->cv_F, cs * 6 * nmax);
 #undef ALLOC
     e = hipHostMalloc((void **)&c->hscal, sizeof(int) * 64, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -931,7 +962,8 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
         (st = fill(c->nbr, c->QMAX * nmax, N_SKIPID)) || (st = fill(c->nbr_child, c->QMAX * nmax, N_SKIPID)) ||
         (st = fill(c->ref, nmax, REF_INACTIVE)) || (st = fill(c->level, nmax, 0)) || (st = fill(c->onb, nmax, 0)) ||
         (st = fill((int *)c->map_stamp, nmax, 0)) || (st = fill(c->exc_count, 2 * (c->MAX_LEVELS + 1), 0)) || (st = fill(c->part, nmax, 1)) ||
-        (st = fill(c->simple, nmax, 0)) || (st = fill((int *)c->X, c->DIM * nmax, 0))) {
+        (st = fill(c->simple, nmax, 0)) || (st = fill((int *)c->X, c->DIM * nmax, 0)) ||
+        (c->cv_F && (st = fill((int *)c->cv_F, (long long)(cs / 4) * 6 * nmax, 0)))) {
         free_ctx(c);
         return st;
     }

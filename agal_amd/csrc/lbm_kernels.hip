@@ -1793,6 +1793,121 @@ __global__ __launch_bounds__(WG) void k_mea_post(const int *__restrict__ ids, in
 }
 
 // ---------------------------------------------------------------------------------------------
+// Cu_ComputeForcesCV (solver_lbm_kernels_forces.cu:13-246): the control-volume force sums of one
+// level, one wavefront per block, lane t = cell t (lanes past the block's cells hold zeros).  A
+// cell takes part when its centre lies strictly inside the box and it is not solid.
+//   POST 0 (start of a coarse step): Fp = the momentum of taking-part leaf cells; on the root level
+//     Fm = sum over p >= 1 of f*_p c_p for the directions whose neighbour centre x + c_p dx lies
+//     outside the box, f*_p the BGK collision with the root rate, formed here only (the pool is
+//     not written).
+//   POST 1 (end of the coarse step): Fm = the momentum of taking-part leaf cells, plus on the root
+//     level f_p c_pb(p) for the directions whose upwind centre x + c_pb(p) dx lies outside.
+// Per block, the lane values go through the reference's shared-memory tree (s = cells/2 ... 1),
+// times dx^DIM; POST 0 assigns the entries [2d] (Fp) and [2d+1] (Fm), POST 1 adds to them.
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+struct CvArgs {
+    T dx, dv;        // spacing and cell volume of the level
+    T om, omp;       // root-level BGK rate dx/tau and 1 - rate
+    T box[6];        // xm, xM, ym, yM, zm, zM
+    int root;        // the level is N_LEVEL_START
+    T *blkF;         // 6 x nmax block sums
+    long long nmax;
+};
+
+template <typename T, int DIM>
+__device__ __forceinline__ bool cv_inside(const CvArgs<T> &cv, T x, T y, T z)
+{
+    bool in = x > cv.box[0] && x < cv.box[1] && y > cv.box[2] && y < cv.box[3];
+    if (DIM == 3) in = in && z > cv.box[4] && z < cv.box[5];
+    return in;
+}
+
+template <typename T, int CELLS>
+__device__ __forceinline__ T tree_sum_cells(T v)
+{
+#pragma unroll
+    for (int s = CELLS / 2; s > 0; s >>= 1) v = v + __shfl_down(v, s, 64);
+    return v;
+}
+
+template <typename T, typename S, int Q, int POST>
+__global__ __launch_bounds__(WG) void k_cv_force(const int *__restrict__ ids, int n, const S *__restrict__ F,
+                                                 long long ncm, const float *__restrict__ X,
+                                                 const int *__restrict__ cells_mask,
+                                                 const int *__restrict__ nbr_child, CvArgs<T> cv)
+{
+    using VS = VelSet<Q>;
+    using ST = Store<T, S>;
+    constexpr int CELLS = VS::CELLS, DIM = VS::DIM;
+    const int lane = threadIdx.x & 63;
+    const int bi = blockIdx.x * (WG / 64) + (int)(threadIdx.x >> 6);
+    if (bi >= n) return;
+    const int b = __builtin_amdgcn_readfirstlane(ids[bi]);
+    if (b < 0) return;
+    const bool act = lane < CELLS;
+    const int t = act ? lane : 0;
+    const long long cell = (long long)b * CELLS + t;
+    const bool leaf = nbr_child[b] < 0;
+    const int I = t & 3, J = (t >> 2) & 3, K = t >> 4;
+    const T x = (T)X[b] + cv.dx * ((T)0.5 + (T)I);
+    const T y = (T)X[b + cv.nmax] + cv.dx * ((T)0.5 + (T)J);
+    const T z = DIM == 3 ? (T)X[b + 2 * cv.nmax] + cv.dx * ((T)0.5 + (T)K) : (T)0;
+    const bool pV = act && cv_inside<T, DIM>(cv, x, y, z) && cells_mask[cell] != CELL_SOLID;
+    T f[Q];
+    T rho = (T)0, m[3] = {(T)0, (T)0, (T)0};
+#pragma unroll
+    for (int p = 0; p < Q; p++) {
+        f[p] = ST::dec(F[(long long)p * ncm + cell], (T)VS::w(p));
+        rho = rho + f[p];
+        m[0] = m[0] + f[p] * (T)VS::cx(p);
+        m[1] = m[1] + f[p] * (T)VS::cy(p);
+        if (DIM == 3) m[2] = m[2] + f[p] * (T)VS::cz(p);
+    }
+    T Fp[3] = {(T)0, (T)0, (T)0}, Fm[3] = {(T)0, (T)0, (T)0};
+    if (pV && leaf) {
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            if (POST == 0) Fp[d] = Fp[d] + m[d];
+            else Fm[d] = Fm[d] + m[d];
+        }
+    }
+    if (cv.root && pV) {
+        const T u = m[0] / rho, v = m[1] / rho, w = m[2] / rho;
+#pragma unroll
+        for (int p = 1; p < Q; p++) {
+            const int s = POST == 0 ? p : VS::pb(p);  // the direction whose far centre is tested
+            const int c[3] = {VS::cx(s), VS::cy(s), VS::cz(s)};
+            if (cv_inside<T, DIM>(cv, x + (T)c[0] * cv.dx, y + (T)c[1] * cv.dx, z + (T)c[2] * cv.dx)) continue;
+            T fv = f[p];
+            if (POST == 0) {
+                const T udotu = u * u + v * v + w * w;
+                const T cdotu = (T)c[0] * u + (T)c[1] * v + (T)c[2] * w;
+                fv = fv * cv.omp +
+                     ((T)VS::w(p) * rho * ((T)1 + (T)3 * cdotu + (T)4.5 * cdotu * cdotu - (T)1.5 * udotu)) * cv.om;
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; d++) Fm[d] = Fm[d] + fv * (T)c[d];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const T sp = tree_sum_cells<T, CELLS>(Fp[d]), sm = tree_sum_cells<T, CELLS>(Fm[d]);
+        if (lane == 0) {
+            T &ep = cv.blkF[(long long)(2 * d) * cv.nmax + b];
+            T &em = cv.blkF[(long long)(2 * d + 1) * cv.nmax + b];
+            if (POST == 0) {
+                ep = sp * cv.dv;
+                em = sm * cv.dv;
+            } else {
+                ep += sp * cv.dv;
+                em += sm * cv.dv;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------------------------
 template <typename T>
@@ -2260,6 +2375,37 @@ static int avg_interp_t(Ctx *c, int L)
     } while (0)
 
 template <typename T, typename S, int Q>
+static int cv_force_t(Ctx *c, int L, int post)
+{
+    const int n = c->n_ids[L];
+    if (n <= 0 || !c->cv_F) return AGAL_OK;
+    CvArgs<T> cv;
+    const T dx = (T)c->dxf_vec[L];
+    cv.dx = dx;
+    T dv = (T)1;
+    for (int d = 0; d < VelSet<Q>::DIM; d++) dv = dv * dx;  // dvf_vec (solver_lbm_init.cu:19-27)
+    cv.dv = dv;
+    const int L0 = c->N_LEVEL_START;
+    cv.om = (T)c->dxf_vec[L0] / (T)c->tau_vec[L0];
+    cv.omp = (T)1 - cv.om;
+    for (int k = 0; k < 6; k++) cv.box[k] = (T)c->cv_box[k];
+    cv.root = L == L0;
+    cv.blkF = static_cast<T *>(c->cv_F);
+    cv.nmax = c->n_maxcblocks;
+    const unsigned rm = res_bit(L, c->cur[L]);
+    hipStream_t s = sched_begin(c, 0, rm, 0);
+    const dim3 grid((unsigned)((n + WG / 64 - 1) / (WG / 64)));
+    if (post == 0)
+        hipLaunchKernelGGL((k_cv_force<T, S, Q, 0>), grid, dim3(WG), 0, s, c->id_set_L(L), n, Fbuf<S>(c, c->cur[L]),
+                           c->n_maxcells, c->X, c->cells_mask, c->nbr_child, cv);
+    else
+        hipLaunchKernelGGL((k_cv_force<T, S, Q, 1>), grid, dim3(WG), 0, s, c->id_set_L(L), n, Fbuf<S>(c, c->cur[L]),
+                           c->n_maxcells, c->X, c->cells_mask, c->nbr_child, cv);
+    AGAL_HIP(c, hipGetLastError());
+    return sched_done(c, s, 0, rm, 0);
+}
+
+template <typename T, typename S, int Q>
 static int mea_post_t(Ctx *c)
 {
     if constexpr (VelSet<Q>::DIM == 3) {
@@ -2277,6 +2423,7 @@ static int mea_post_t(Ctx *c)
 }
 
 int launch_mea_post(Ctx *c) { AGAL_DISPATCH(c, mea_post_t, c); }
+int launch_cv_force(Ctx *c, int L, int post) { AGAL_DISPATCH(c, cv_force_t, c, L, post); }
 int launch_set_ic(Ctx *c, int L) { AGAL_DISPATCH(c, set_ic_t, c, L); }
 int launch_refresh(Ctx *c, int L) { AGAL_DISPATCH(c, refresh_t, c, L); }
 int launch_collide_stream(Ctx *c, int L) { AGAL_DISPATCH(c, collide_stream_t, c, L); }

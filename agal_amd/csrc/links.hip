@@ -653,55 +653,81 @@ int compute_links(Ctx *c)
     return links_rebuild_lists(c);
 }
 
-// S_ReportForces (solver_lbm_compute.cu:165-218), force type 0: the per-block sums over IDs below
-// id_max in FP64, times 1/dx of the finest level; the sums are then reset.  One workgroup, lanes
-// striding over the IDs, then a tree: a fixed order, not the serial one.
-template <typename T>
+// S_ReportForces (solver_lbm_compute.cu:165-218): the per-block sums of each row over IDs below
+// id_max in FP64; the rows read are then reset.  One workgroup, lanes striding over the IDs, then a
+// tree: a fixed order, not the serial one.
+template <typename T, int R>
 __global__ void k_report_forces(T *blkF, long long nmax, int idm, double *out)
 {
-    __shared__ double sh[3][KWG];
-    double a[3] = {0.0, 0.0, 0.0};
+    __shared__ double sh[R][KWG];
+    double a[R];
+    for (int d = 0; d < R; d++) a[d] = 0.0;
     for (int k = threadIdx.x; k < idm; k += KWG)
-        for (int d = 0; d < 3; d++) {
+        for (int d = 0; d < R; d++) {
             a[d] += (double)blkF[(long long)d * nmax + k];
             blkF[(long long)d * nmax + k] = (T)0;
         }
-    for (int d = 0; d < 3; d++) sh[d][threadIdx.x] = a[d];
+    for (int d = 0; d < R; d++) sh[d][threadIdx.x] = a[d];
     __syncthreads();
     for (int w = KWG / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)
-            for (int d = 0; d < 3; d++) sh[d][threadIdx.x] += sh[d][threadIdx.x + w];
+            for (int d = 0; d < R; d++) sh[d][threadIdx.x] += sh[d][threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int d = 0; d < 3; d++) out[d] = sh[d][0];
+        for (int d = 0; d < R; d++) out[d] = sh[d][0];
 }
 
+// Force type 0: the momentum-exchange rows times 1/dx of the finest level.  Force type 1: the
+// control-volume rows, (sum Fp - sum Fm) per direction times 1/dx of the root level; the whole
+// array is cleared afterwards (Cu_ResetToValue over 6 x n_maxcblocks).
 int compute_forces(Ctx *c, double out[3])
 {
     out[0] = out[1] = out[2] = 0.0;
-    if (c->DIM != 3) return AGAL_ERR_NOT_BUILT;
-    if (!c->links_on || c->P.S_FORCE_TYPE != 0) {
-        c->last_error = "the momentum-exchange force needs face-cell links (G_BIN_MAKE_LINKS 1) and S_FORCE_TYPE 0";
+    const int type = c->P.S_FORCE_TYPE;
+    if (type == 0 && c->DIM != 3) return AGAL_ERR_NOT_BUILT;
+    if (type == 0 && !c->links_on) {
+        c->last_error = "the momentum-exchange force needs face-cell links (G_BIN_MAKE_LINKS 1)";
+        return AGAL_ERR_STATE;
+    }
+    if (type != 0 && !(type == 1 && c->cv_F)) {
+        c->last_error = "S_FORCE_TYPE must be 0 (momentum exchange) or 1 (control volume)";
         return AGAL_ERR_STATE;
     }
     double *d = nullptr;
-    AGAL_HIP(c, hipMalloc(&d, sizeof(double) * 3));
+    AGAL_HIP(c, hipMalloc(&d, sizeof(double) * 6));
     const int idm = c->id_max[c->MAX_LEVELS];
-    if (c->precision == AGAL_FP64)
-        hipLaunchKernelGGL(k_report_forces<double>, dim3(1), dim3(KWG), 0, c->stream, (double *)c->lk_blkF,
-                           c->n_maxcblocks, idm, d);
-    else
-        hipLaunchKernelGGL(k_report_forces<float>, dim3(1), dim3(KWG), 0, c->stream, (float *)c->lk_blkF, c->n_maxcblocks,
-                           idm, d);
-    double h[3] = {0.0, 0.0, 0.0};
+    const bool f64 = c->precision == AGAL_FP64;
+    if (type == 0) {
+        if (f64)
+            hipLaunchKernelGGL((k_report_forces<double, 3>), dim3(1), dim3(KWG), 0, c->stream, (double *)c->lk_blkF,
+                               c->n_maxcblocks, idm, d);
+        else
+            hipLaunchKernelGGL((k_report_forces<float, 3>), dim3(1), dim3(KWG), 0, c->stream, (float *)c->lk_blkF,
+                               c->n_maxcblocks, idm, d);
+    } else {
+        if (f64)
+            hipLaunchKernelGGL((k_report_forces<double, 6>), dim3(1), dim3(KWG), 0, c->stream, (double *)c->cv_F,
+                               c->n_maxcblocks, idm, d);
+        else
+            hipLaunchKernelGGL((k_report_forces<float, 6>), dim3(1), dim3(KWG), 0, c->stream, (float *)c->cv_F,
+                               c->n_maxcblocks, idm, d);
+    }
+    double h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && type == 1)
+        e = hipMemsetAsync(c->cv_F, 0, c->csize * 6 * (size_t)c->n_maxcblocks, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(c, e, "compute_forces");
-    const double factor = 1.0 / c->dxf_vec[c->MAX_LEVELS - 1];
-    for (int k = 0; k < 3; k++) out[k] = factor * h[k];
+    if (type == 0) {
+        const double factor = 1.0 / c->dxf_vec[c->MAX_LEVELS - 1];
+        for (int k = 0; k < 3; k++) out[k] = factor * h[k];
+    } else {
+        const double factor = 1.0 / c->dxf_vec[c->N_LEVEL_START];
+        for (int k = 0; k < c->DIM; k++) out[k] = factor * (h[2 * k] - h[2 * k + 1]);
+    }
     return AGAL_OK;
 }
 

@@ -129,8 +129,9 @@ class LbmMesh:
         self._call("check_masks", int(L))
 
     def S_ComputeForces(self) -> np.ndarray:
-        """S_ReportForces: the momentum-exchange force (x, y, z) accumulated over the last coarse step;
-        the accumulators are reset (needs G_BIN_MAKE_LINKS 1, S_FORCE_TYPE 0)."""
+        """S_ReportForces: the force (x, y, z) accumulated since the last report -- S_FORCE_TYPE 0: the
+        momentum exchange of the last coarse step (needs G_BIN_MAKE_LINKS 1); 1: the control-volume
+        balance over the S_FORCEVOLUME_* box in the last coarse step.  The accumulators are reset."""
         out = (C.c_double * 3)()
         self._call("forces", out)
         return np.array(list(out))

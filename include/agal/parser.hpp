@@ -110,6 +110,12 @@ public:
         base.S_BC_TYPE = Int("S_BC_TYPE");
         base.S_FORCE_TYPE = Int("S_FORCE_TYPE");
         base.S_FORCE_ORDER = Int("S_FORCE_ORDER");
+        base.S_FORCEVOLUME_Xm = Dbl("S_FORCEVOLUME_Xm");
+        base.S_FORCEVOLUME_XM = Dbl("S_FORCEVOLUME_XM");
+        base.S_FORCEVOLUME_Ym = Dbl("S_FORCEVOLUME_Ym");
+        base.S_FORCEVOLUME_YM = Dbl("S_FORCEVOLUME_YM");
+        base.S_FORCEVOLUME_Zm = Dbl("S_FORCEVOLUME_Zm");
+        base.S_FORCEVOLUME_ZM = Dbl("S_FORCEVOLUME_ZM");
         const int coll = Int("S_COLLISION");
         if (coll == 0) base.collision = AGAL_BGK;
         else if (coll == 2) base.collision = AGAL_MRT;

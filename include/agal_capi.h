@@ -132,6 +132,9 @@ typedef struct agal_params {
      * momentum-exchange force on the linked cells (solver_lbm_kernels_forces.cu:322-540). */
     int    G_BIN_DENSITY, G_BIN_LEVELS, G_BIN_SPEC, G_BIN_MAKE_LINKS;
     int    S_BC_TYPE, S_FORCE_TYPE, S_FORCE_ORDER;
+    /* Control volume of the force type 1 (parser.h:132-137; snapped to root-block edges,
+     * solver_lbm_init.cu:36-41), as fractions of the domain: x in (Xm, XM), y in (Ym, YM), z in (Zm, ZM). */
+    double S_FORCEVOLUME_Xm, S_FORCEVOLUME_XM, S_FORCEVOLUME_Ym, S_FORCEVOLUME_YM, S_FORCEVOLUME_Zm, S_FORCEVOLUME_ZM;
 } agal_params;
 
 typedef struct agal_ctx agal_ctx;
@@ -287,13 +290,18 @@ int  agal_voxelize(agal_ctx *ctx, int L);
 /* Mesh::M_Geometry_Voxelize_S2 (mesh_voxelizer.cu:139-174) on level L: BOUNDARY cell masks (non-
  * solid cells with a solid neighbour) in leaf SOLIDB / SOLIDA blocks. */
 int  agal_check_masks(agal_ctx *ctx, int L);
-/* Solver_LBM::S_ReportForces (solver_lbm_compute.cu:165-218), force type 0: the momentum-exchange
+/* Solver_LBM::S_ReportForces (solver_lbm_compute.cu:165-218).  Force type 0: the momentum-exchange
  * sums of Cu_ComputeForcesMEA (solver_lbm_kernels_forces.cu:322-540), which every coarse step forms
  * around the finest level's last sub-step (after the collision and after the stream) on the cut
  * links of boundary cells -- order 1: f c_p; order 2 (S_FORCE_ORDER 2): ((1/2 + q) f + (1/2 - q)
- * f_m) c_p -- per block times dx^3, summed over blocks in FP64 and times 1/dx of the finest level
- * into out (x, y, z); the block sums are then reset.  Needs G_BIN_MAKE_LINKS 1 and S_FORCE_TYPE 0
- * (else AGAL_ERR_STATE).  Drag / lift coefficients are left to the caller. */
+ * f_m) c_p -- per block times dx^3, summed over blocks in FP64 and times 1/dx of the finest level;
+ * needs G_BIN_MAKE_LINKS 1.  Force type 1: the control-volume balance of Cu_ComputeForcesCV
+ * (:13-246) over the box S_FORCEVOLUME_*, formed at the start of each coarse step (momentum of the
+ * leaf cells inside; root-level populations leaving through the box faces after a BGK collision
+ * with the root rate) and at its end (after a full fine-to-coarse averaging and the interface
+ * interpolations: the momentum again, and the root-level populations that entered), times 1/dx of
+ * the root level.  out = (x, y, z) of sum(first part) - sum(second part); the block sums are then
+ * reset.  Other force types: AGAL_ERR_STATE.  Drag / lift coefficients are left to the caller. */
 int  agal_forces(agal_ctx *ctx, double out[3]);
 /* Face-cell links after agal_refine_near_wall with G_BIN_MAKE_LINKS 1 (Mesh::M_Geometry_Voxelize_S3
  * and M_LinkLengthComputation, mesh_voxelizer.cu:177-296): onb_solid[k] = rank of block k among the

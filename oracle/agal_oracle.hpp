@@ -468,7 +468,9 @@ struct Oracle {
     // interpolation at child offsets +1/4 instead of -1/4; 4: averaging that drops one child cell
     // of the 2^d group; 5: the advance skips the mid-cycle interpolation; 6: the advance skips
     // the average after a level's second sub-cycle; 7: the root level averages before its finer
-    // levels advance; 8: the stream swaps without the f_pb > 0 guard.
+    // levels advance; 8: the stream swaps without the f_pb > 0 guard; 9: the control-volume inflow
+    // tested and weighted along c_p instead of c_pb(p); 10: the control-volume outflow without the
+    // collision.
     int plant = 0;
 
     T quant(T f, int dir) const
@@ -525,11 +527,16 @@ struct Oracle {
         tau_vec.assign(MAX_LEVELS, 0);
         for (int L = 0; L < MAX_LEVELS; L++)
             tau_vec[L] = (T)(P.v0 * 3.0 + 0.5 * (double)dxf_vec[L]);
+        // solver_lbm_init.cu:36-41: the control volume snapped to root-block edges.
+        const double fr[6] = {P.S_FORCEVOLUME_Xm, P.S_FORCEVOLUME_XM, P.S_FORCEVOLUME_Ym,
+                              P.S_FORCEVOLUME_YM, P.S_FORCEVOLUME_Zm, P.S_FORCEVOLUME_ZM};
+        for (int k = 0; k < 6; k++) cv_box[k] = (T)(4.0 * (double)(int)(fr[k] * (double)(Nxi[k / 2] / 4)) * (double)dxf_vec[0]);
 
         F.assign((size_t)N_U * n_maxcells, T(0));
         cells_mask.assign(n_maxcells, CELL_INTERIOR);
         X.assign((size_t)DIM * n_maxcblocks, -1.0f);
         blk_F.assign((size_t)3 * n_maxcblocks, T(0));
+        cv_F.assign(P.S_FORCE_TYPE == 1 ? (size_t)6 * n_maxcblocks : 0, T(0));
         blk_mask.assign(n_maxcblocks, BLK_REGULAR);
         nbr.assign((size_t)Q_max * n_maxcblocks, N_SKIPID);
         nbr_child.assign((size_t)Q_max * n_maxcblocks, N_SKIPID);
@@ -1296,17 +1303,27 @@ struct Oracle {
 
     // Solver_LBM::S_Advance (solver_lbm_advance.cu:30-174).  The momentum-exchange force (force
     // type 0) brackets the finest level's last sub-step of a coarse step, and every step on a
-    // single level (:124-126, :144-150).
+    // single level (:124-126, :144-150); the control-volume force (type 1) brackets the coarse
+    // step (:45-49, :60-68, :139-149).
     void advance(int L)
     {
         if (MAX_LEVELS > 1 && (MAX_LEVELS != N_LEVEL_START + 1)) {
             if (L == N_LEVEL_START) {
                 global_Liter = 0;
+                if (cv_on())
+                    for (int K = N_LEVEL_START; K < MAX_LEVELS; K++) forces_cv(K, 0);
                 interpolate(L, INTERP_INTERFACE);
                 collide_stream(L);
                 if (plant == 7) average(L, average_advance());
                 advance(L + 1);
                 if (plant != 7) average(L, average_advance());
+                if (cv_on()) {
+                    // solver_lbm_advance.cu:60-68: a full fine-to-coarse average and the interface
+                    // interpolations of every level before the second half.
+                    for (int K = MAX_LEVELS - 2; K >= N_LEVEL_START; K--) average(K, AVERAGE_GRID);
+                    for (int K = N_LEVEL_START; K < MAX_LEVELS - 1; K++) interpolate(K, INTERP_INTERFACE);
+                    for (int K = N_LEVEL_START; K < MAX_LEVELS; K++) forces_cv(K, 1);
+                }
             } else {
                 if (L < MAX_LEVELS - 1) interpolate(L, INTERP_INTERFACE);
                 collide_stream(L);
@@ -1324,7 +1341,9 @@ struct Oracle {
                 }
             }
         } else {
+            if (cv_on()) forces_cv(L, 0);
             collide_stream(L, mea_on());
+            if (cv_on()) forces_cv(L, 1);
         }
     }
 
@@ -2098,6 +2117,10 @@ struct Oracle {
     long n_link_corrections = 0;
     bool link_masks = false;      // links made to agree with the cell masks (AGAL_OPT_LINK_MASKS)
     std::vector<T> blk_F;         // cblock_f_Ff: per block, the x / y / z momentum-exchange sums
+    // cblock_f_Ff of the control-volume force (S_FORCE_TYPE 1): per block, rows 2d / 2d+1 hold the
+    // Fp / Fm sums of direction d; the box (xm, xM, ym, yM, zm, zM) snapped to root-block edges.
+    std::vector<T> cv_F;
+    T cv_box[6] = {0, 0, 0, 0, 0, 0};
 
     void rank_boundary_blocks()
     {
@@ -2363,18 +2386,106 @@ struct Oracle {
 
     bool mea_on() const { return links && P.S_FORCE_TYPE == 0 && DIM == 3; }
 
-    // S_ReportForces (solver_lbm_compute.cu:165-218), force type 0: the block sums over IDs below
-    // id_max, summed in FP64 in ID order, times 1/dx of the finest level; the sums are then reset.
+    bool cv_on() const { return P.S_FORCE_TYPE == 1; }
+
+    bool cv_inside(T x, T y, T z) const
+    {
+        // CheckPointInRegion2D / 3D (inc/util_tribin_old.h:96-110): strict inequalities.
+        bool in = x > cv_box[0] && x < cv_box[1] && y > cv_box[2] && y < cv_box[3];
+        if (DIM == 3) in = in && z > cv_box[4] && z < cv_box[5];
+        return in;
+    }
+
+    // Cu_ComputeForcesCV (solver_lbm_kernels_forces.cu:13-246) over the blocks of level L.  A cell
+    // takes part when its centre x = X_b + dx (1/2 + I) lies strictly inside the box and it is not
+    // solid.  Per cell, in ufloat_t: the momentum sum_p f_p c_p (p ascending) goes to Fp (post 0)
+    // or Fm (post 1) for leaf blocks.  On the root level, post 0 adds to Fm, for p >= 1 whose
+    // neighbour centre x + c_p dx is outside the box, f*_p c_p with f*_p = f_p (1 - om) +
+    // (w_p rho (1 + 3 c.u + 4.5 (c.u) c.u - 1.5 u.u)) om, u = momentum / rho, om = dx_0 / tau_0;
+    // post 1 adds f_p c_pb(p) for p >= 1 whose upwind centre x + c_pb(p) dx is outside.  The cell
+    // values are summed by the shared-memory tree (s = cells/2 ... 1) and times dx^DIM; post 0
+    // assigns the block's entries, post 1 adds to them.
+    void forces_cv(int L, int post_step)
+    {
+        const int *ids = idset(L);
+        const T dxL = dxf_vec[L];
+        T dv = (T)1;
+        for (int d = 0; d < DIM; d++) dv *= dxL;
+        const T om = dxf_vec[N_LEVEL_START] / tau_vec[N_LEVEL_START], omp = (T)1 - om;
+        const bool root = L == N_LEVEL_START;
+#pragma omp parallel for schedule(static)
+        for (int bi = 0; bi < n_ids[L]; bi++) {
+            const int b = ids[bi];
+            if (b < 0) continue;
+            const bool leaf = nbr_child[b] < 0;
+            T sFp[3][64], sFm[3][64];
+            for (int t = 0; t < M_CBLOCK; t++) {
+                for (int d = 0; d < 3; d++) sFp[d][t] = sFm[d][t] = (T)0;
+                const long cell = (long)b * M_CBLOCK + t;
+                const int I = t % 4, J = (t / 4) % 4, K = (t / 4) / 4;
+                const T x = (T)X[b + 0 * n_maxcblocks] + dxL * ((T)0.5 + (T)I);
+                const T y = (T)X[b + 1 * n_maxcblocks] + dxL * ((T)0.5 + (T)J);
+                const T z = DIM == 3 ? (T)X[b + 2 * n_maxcblocks] + dxL * ((T)0.5 + (T)K) : (T)0;
+                const bool pV = cv_inside(x, y, z) && cells_mask[cell] != CELL_SOLID;
+                T f[27], rho = (T)0, mom[3] = {(T)0, (T)0, (T)0};
+                for (int p = 0; p < Q; p++) {
+                    f[p] = Fp(V.pb[p])[cell];  // f_p sits in slot pb(p) between steps
+                    rho += f[p];
+                    for (int d = 0; d < DIM; d++) mom[d] += f[p] * (T)V.c[d][p];
+                }
+                if (pV && leaf)
+                    for (int d = 0; d < DIM; d++) (post_step == 0 ? sFp : sFm)[d][t] += mom[d];
+                if (!(root && pV)) continue;
+                const T u = mom[0] / rho, v = mom[1] / rho, w = mom[2] / rho;
+                for (int p = 1; p < Q; p++) {
+                    const int s_dir = (post_step == 0 || plant == 9) ? p : V.pb[p];
+                    const T cx = (T)V.c[0][s_dir], cy = (T)V.c[1][s_dir], cz = (T)V.c[2][s_dir];
+                    if (cv_inside(x + cx * dxL, y + cy * dxL, z + cz * dxL)) continue;
+                    T fv = f[p];
+                    if (post_step == 0 && plant != 10) {
+                        const T udotu = u * u + v * v + w * w;
+                        const T cdotu = cx * u + cy * v + cz * w;
+                        fv = fv * omp + ((T)V.w[p] * rho * ((T)1 + (T)3 * cdotu + (T)4.5 * cdotu * cdotu - (T)1.5 * udotu)) * om;
+                    }
+                    for (int d = 0; d < DIM; d++) sFm[d][t] += fv * (T)V.c[d][s_dir];
+                }
+            }
+            for (int st = M_CBLOCK / 2; st > 0; st >>= 1)
+                for (int t = 0; t < st; t++)
+                    for (int d = 0; d < DIM; d++) {
+                        sFp[d][t] = sFp[d][t] + sFp[d][t + st];
+                        sFm[d][t] = sFm[d][t] + sFm[d][t + st];
+                    }
+            for (int d = 0; d < DIM; d++) {
+                T &ep = cv_F[(size_t)(2 * d) * n_maxcblocks + b];
+                T &em = cv_F[(size_t)(2 * d + 1) * n_maxcblocks + b];
+                ep = post_step == 0 ? sFp[d][0] * dv : ep + sFp[d][0] * dv;
+                em = post_step == 0 ? sFm[d][0] * dv : em + sFm[d][0] * dv;
+            }
+        }
+    }
+
+    // S_ReportForces (solver_lbm_compute.cu:165-218): the block sums of each row over IDs below
+    // id_max, summed in FP64 in ID order.  Force type 0: the three momentum-exchange rows times
+    // 1/dx of the finest level; force type 1: (Fp - Fm) per direction times 1/dx of the root level.
+    // The sums are then reset.
     void report_forces(double out[3])
     {
         out[0] = out[1] = out[2] = 0.0;
-        const double factor = 1.0 / (double)dxf_vec[MAX_LEVELS - 1];
         const int idm = id_max[MAX_LEVELS];
-        for (int d = 0; d < 3; d++) {
+        auto row_sum = [&](const std::vector<T> &A, int r) {
             double acc = 0.0;
-            for (int k = 0; k < idm; k++) acc += (double)blk_F[(size_t)d * n_maxcblocks + k];
-            out[d] = factor * acc;
+            for (int k = 0; k < idm; k++) acc += (double)A[(size_t)r * n_maxcblocks + k];
+            return acc;
+        };
+        if (cv_on()) {
+            const double factor = 1.0 / (double)dxf_vec[N_LEVEL_START];
+            for (int d = 0; d < DIM; d++) out[d] = factor * (row_sum(cv_F, 2 * d) - row_sum(cv_F, 2 * d + 1));
+            std::fill(cv_F.begin(), cv_F.end(), (T)0);
+            return;
         }
+        const double factor = 1.0 / (double)dxf_vec[MAX_LEVELS - 1];
+        for (int d = 0; d < 3; d++) out[d] = factor * row_sum(blk_F, d);
         std::fill(blk_F.begin(), blk_F.end(), (T)0);
     }
 
