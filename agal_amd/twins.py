@@ -53,9 +53,10 @@ def advance_twins(lead: LbmMesh, followers: list, i0: int, n: int) -> list:
     return out
 
 
-def velocity_error(ref: LbmMesh, other: LbmMesh) -> dict:
+def velocity_error(ref: LbmMesh, other: LbmMesh, exclude_ghost: bool = False) -> dict:
     """Relative L2 velocity difference over the active, non-solid leaf cells of two contexts on one
-    mesh."""
+    mesh; with `exclude_ghost` the GHOST cells (coarse-to-fine interpolation targets, whose values
+    between steps depend on when they were last interpolated) are left out as well."""
     s = ref.refresh_sizes()
     ML = s.max_levels
     for m in (ref, other):
@@ -68,7 +69,10 @@ def velocity_error(ref: LbmMesh, other: LbmMesh) -> dict:
         raise RuntimeError("the contexts do not share one mesh")
     first_child = ref.block_ints(capi.BLK_NBR_CHILD, nb)[0]
     leaf = (ref_ids != capi.REF_INACTIVE) & (first_child < 0)
-    cells = np.repeat(leaf, cpb) & (ref.cell_masks(nb) != capi.CELL_SOLID)
+    cm = ref.cell_masks(nb)
+    cells = np.repeat(leaf, cpb) & (cm != capi.CELL_SOLID)
+    if exclude_ghost:
+        cells &= cm != capi.CELL_GHOST
     a = ref.macros(nb)[1:1 + s.n_dim][:, cells].astype(np.float64)
     b = other.macros(nb)[1:1 + s.n_dim][:, cells].astype(np.float64)
     err = float(np.sqrt(np.sum((a - b) ** 2) / np.sum(a ** 2)))

@@ -532,14 +532,23 @@ def sphere_study(args):
            "coarse_steps": args.sphere_steps, "mlups": {}, "cd": {}, "fused_kernel": {},
            "mrt_transform": {"fp64": "vector", "fp32": "vector", "fp32_mfma": "matrix cores (bf16 two-term)",
                              "fp16": "vector", "fp16_mfma": "matrix cores (binary16 one term)"},
-           "bgk_control": "bgk_fp64 / bgk_fp16: the same flow with BGK collision, compared with each other"}
+           "bgk_control": "bgk_fp64 / bgk_fp16: the same flow with BGK collision, compared with each other",
+           "control_volume": "cv_fp64: FP64 MRT with the control-volume force (S_FORCE_TYPE 1) over the box "
+                             "0.125-0.875 L; its schedule re-averages and re-interpolates the hierarchy after every "
+                             "coarse step, which rewrites the GHOST cells (cv_fp64 vs fp64 velocity errors are given "
+                             "over all leaf cells and without GHOST cells); its C_D against fp64's momentum exchange"}
 
-    def params(prec, u, coll=capi.MRT):
+    # The control volume of the force type 1 run: the box 0.125 L .. 0.875 L around the body (its edges
+    # are root-block edges, so the reference's snapping keeps them).
+    cv_box = dict(S_FORCEVOLUME_Xm=0.125, S_FORCEVOLUME_XM=0.875, S_FORCEVOLUME_Ym=0.125,
+                  S_FORCEVOLUME_YM=0.875, S_FORCEVOLUME_Zm=0.125, S_FORCEVOLUME_ZM=0.875)
+
+    def params(prec, u, coll=capi.MRT, ftype=0):
         return capi.default_params(velocity_set=capi.D3Q27, precision=prec, collision=coll, Nx=N,
                                    MAX_LEVELS_WALL=levels, MAX_LEVELS_INTERIOR=1, v0=U * D / Re,
                                    bc_preset=capi.BC_FPSC3D, bc_velocity=u, ic_u=u, P_REFINE=1 << 30,
                                    G_BIN_MAKE_LINKS=1, G_BIN_DENSITY=16, G_BIN_LEVELS=10, G_BIN_SPEC=7,
-                                   S_BC_TYPE=0, S_FORCE_TYPE=0, S_FORCE_ORDER=1, **rates)
+                                   S_BC_TYPE=0, S_FORCE_TYPE=ftype, S_FORCE_ORDER=1, **rates, **cv_box)
 
     cd_scale = 8.0 / (U * U * math.pi * D * D)
     rest = {}
@@ -555,11 +564,12 @@ def sphere_study(args):
     out["rest_force_in_cd_units"] = rest
     log(f"sphere: force at rest in C_D units {rest}")
     ctx = {}
-    runs = (("fp64", capi.FP64, capi.MRT, 0), ("fp32", capi.FP32, capi.MRT, 0), ("fp32_mfma", capi.FP32, capi.MRT, 1),
-            ("fp16", capi.FP16, capi.MRT, 0), ("fp16_mfma", capi.FP16, capi.MRT, 1),
-            ("bgk_fp64", capi.FP64, capi.BGK, 0), ("bgk_fp16", capi.FP16, capi.BGK, 0))
-    for k, prec, coll, mf in runs:
-        m = LbmMesh(params(prec, U, coll), args.capacity)
+    runs = (("fp64", capi.FP64, capi.MRT, 0, 0), ("fp32", capi.FP32, capi.MRT, 0, 0),
+            ("fp32_mfma", capi.FP32, capi.MRT, 1, 0), ("fp16", capi.FP16, capi.MRT, 0, 0),
+            ("fp16_mfma", capi.FP16, capi.MRT, 1, 0), ("bgk_fp64", capi.FP64, capi.BGK, 0, 0),
+            ("bgk_fp16", capi.FP16, capi.BGK, 0, 0), ("cv_fp64", capi.FP64, capi.MRT, 0, 1))
+    for k, prec, coll, mf, ftype in runs:
+        m = LbmMesh(params(prec, U, coll, ftype), args.capacity)
         if mf:
             m.set_option(capi.OPT_MRT_MFMA, 1)
         m.set_option(capi.OPT_LINK_MASKS, 1)
@@ -594,6 +604,9 @@ def sphere_study(args):
                 if k not in ("fp64", "bgk_fp64"):
                     ref = "bgk_fp64" if k.startswith("bgk") else "fp64"
                     row[k] = float(f"{velocity_error(ctx[ref], ctx[k])['rel_l2_velocity']:.3g}")
+                    if k == "cv_fp64":
+                        row["cv_fp64_non_ghost"] = float(
+                            f"{velocity_error(ctx[ref], ctx[k], exclude_ghost=True)['rel_l2_velocity']:.3g}")
             series.append(row)
             cd_series.append({"step": done, **{k: round(float(sum(cds[k][-25:]) / len(cds[k][-25:])), 4) for k in ctx}})
             log(f"sphere: {done} coarse steps, velocity error vs fp64 {row}, C_D {cd_series[-1]}")

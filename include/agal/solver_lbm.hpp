@@ -5,9 +5,9 @@
  * reference's Solver_LBM does (inc/solver_lbm.h:82-217): the step, interpolation, averaging,
  * refresh, criterion and initial condition run on the GPU through the C ABI; the output
  * properties (S_ComputeProperties / S_ComputeOutputProperties, solver_lbm_compute.cu:4-163) are
- * computed on the host from the mesh's retrieved arrays; S_ReportForces reduces the momentum-
- * exchange force on the GPU and prints / records the drag and lift coefficients
- * (solver_lbm_compute.cu:165-218).
+ * computed on the host from the mesh's retrieved arrays; S_ReportForces reduces the force of the
+ * configured S_FORCE_TYPE (0: momentum exchange, 1: control volume) on the GPU and prints / records
+ * the drag and lift coefficients (solver_lbm_compute.cu:165-218).
  */
 #ifndef AGAL_SOLVER_LBM_HPP
 #define AGAL_SOLVER_LBM_HPP

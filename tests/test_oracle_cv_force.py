@@ -119,3 +119,42 @@ def test_box_is_snapped_to_root_block_edges():
     np.testing.assert_array_equal(a, b)
     whole = sphere_forces(1, WIDE, steps=3)
     assert np.abs(a - whole).max() > 1e-3 * np.abs(whole).max()
+
+
+def test_schedule_leaves_the_flow_outside_ghost_cells_unchanged():
+    """On three levels the second half follows a full average (GRID mode) and the interface
+    interpolations of every level (solver_lbm_advance.cu:60-68).  Those rewrite GHOST cells and
+    cells of blocks with children; the next coarse step re-interpolates the GHOST cells before any
+    sub-step reads them, so on the leaf cells that are not GHOST the flow of a force-type-1 run is
+    bit for bit the flow of the same run with force type 0."""
+    set_threads(8)
+    runs = []
+    for ftype in (0, 1):
+        p = capi.default_params(velocity_set=capi.D3Q19, precision=capi.FP64, Nx=16, MAX_LEVELS_WALL=1,
+                                MAX_LEVELS_INTERIOR=3, v0=0.05 / 100.0, bc_preset=capi.BC_LDC3D, bc_velocity=0.05,
+                                P_REFINE=1 << 30, S_FORCE_TYPE=ftype, **BOX)
+        m = OracleMesh(p, 8192)
+        m.S_SetIC(0)
+        for _ in range(20):
+            m.S_Advance(0)
+        m.mark_blocks([5, 6, 9, 10, 21, 22, 25, 26, 42], capi.REF_MARK_REFINE)
+        m.M_RefineAndCoarsenBlocks()
+        for _ in range(20):
+            m.S_Advance(0)
+        runs.append(m)
+    a, b = runs
+    s = a.refresh_sizes()
+    nb = int(s.id_max[s.max_levels])
+    for m in runs:
+        for L in range(s.max_levels):
+            m.S_RefreshVariables(L)
+    leaf = (a.block_ints(capi.BLK_REF, nb) != capi.REF_INACTIVE) & (a.block_ints(capi.BLK_NBR_CHILD, nb)[0] < 0)
+    cm = a.cell_masks(nb)
+    cells = np.repeat(leaf, s.cells_per_block) & (cm != capi.CELL_SOLID)
+    ghost = cells & (cm == capi.CELL_GHOST)
+    ua, ub = a.macros(nb)[1:4], b.macros(nb)[1:4]
+    assert s.n_ids[1] > 0 and ghost.any()
+    np.testing.assert_array_equal(ua[:, cells & ~ghost], ub[:, cells & ~ghost])
+    assert np.abs(ua[:, ghost] - ub[:, ghost]).max() > 0  # the GHOST cells were rewritten
+    for m in runs:
+        m.close()
