@@ -103,6 +103,9 @@ struct Ctx {
     int *exc_idx = nullptr;   // 3 ints per entry
     void *exc_val = nullptr;  // one value per entry
     int *exc_count = nullptr; // per level (and the extra slot MAX_LEVELS): [2L] count, [2L+1] overflow flag
+    // The extra list (slot MAX_LEVELS) linked per block: chain heads per block ID (-1 when empty)
+    // and two links per entry (through its initiator's and its partner's block).
+    int *exc_head = nullptr, *exc_next = nullptr;
 
     // Triangle geometry: n_geo_tri triangles (9 floats each) binned on a G x G grid over the (y, z)
     // bounding box; bin (iy, iz) lists geo_bin_tri[geo_bin_start[iy + G iz] ...].

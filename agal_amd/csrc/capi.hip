@@ -176,7 +176,7 @@ void free_ctx(Ctx *c)
     void *ptrs[] = {c->F[0], c->F[1], c->M, c->cells_mask, c->X, c->nbr, c->nbr_child, c->nbr_slot, c->ref, c->level,
                     c->onb, c->blk_mask, c->part, c->simple, c->id_set, c->order, c->gap_set, c->iface, c->map_stamp,
                     c->first_child, c->scan_a, c->scan_b, c->partials, c->dscal, c->exc_idx, c->exc_val,
-                    c->exc_count, c->cv_F};
+                    c->exc_count, c->exc_head, c->exc_next, c->cv_F};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_geometry(c);
@@ -947,6 +947,8 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     ALLOC(c->exc_idx, sizeof(int) * 3 * c->exc_cap * (c->MAX_LEVELS + 1));
     ALLOC(c->exc_val, cs * c->exc_cap * (c->MAX_LEVELS + 1));
     ALLOC(c->exc_count, sizeof(int) * 2 * (c->MAX_LEVELS + 1));
+    ALLOC(c->exc_head, sizeof(int) * nmax);
+    ALLOC(c->exc_next, sizeof(int) * 2 * c->exc_cap_alloc);
     if (c->P.S_FORCE_TYPE == 1) ALLOC(c->cv_F, cs * 6 * nmax);
 #undef ALLOC
     e = hipHostMalloc((void **)&c->hscal, sizeof(int) * 64, hipHostMallocDefault);
@@ -961,7 +963,7 @@ int agal_create(const agal_params *p, long long block_capacity, int device, agal
     if ((st = fill(c->cells_mask, ncm, CELL_INTERIOR)) || (st = fill(c->blk_mask, nmax, BLK_REGULAR)) ||
         (st = fill(c->nbr, c->QMAX * nmax, N_SKIPID)) || (st = fill(c->nbr_child, c->QMAX * nmax, N_SKIPID)) ||
         (st = fill(c->ref, nmax, REF_INACTIVE)) || (st = fill(c->level, nmax, 0)) || (st = fill(c->onb, nmax, 0)) ||
-        (st = fill((int *)c->map_stamp, nmax, 0)) || (st = fill(c->exc_count, 2 * (c->MAX_LEVELS + 1), 0)) || (st = fill(c->part, nmax, 1)) ||
+        (st = fill((int *)c->map_stamp, nmax, 0)) || (st = fill(c->exc_count, 2 * (c->MAX_LEVELS + 1), 0)) || (st = fill(c->exc_head, nmax, -1)) || (st = fill(c->part, nmax, 1)) ||
         (st = fill(c->simple, nmax, 0)) || (st = fill((int *)c->X, c->DIM * nmax, 0)) ||
         (c->cv_F && (st = fill((int *)c->cv_F, (long long)(cs / 4) * 6 * nmax, 0)))) {
         free_ctx(c);

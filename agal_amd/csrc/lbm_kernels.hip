@@ -162,6 +162,11 @@ struct StepArgs {
     const int *va_idx;
     const T *va_val;
     const int *va_count;
+    // The extra list linked per block: x_head[block] is the last node 2k + s of the block's chain
+    // (entry k, s = 0 through its initiator cell, 1 through its partner cell), -1 when empty;
+    // x_next[node] the node before it.  Built in vmode 1, walked and emptied in vmode 2.
+    int *x_head;
+    int *x_next;
 };
 
 // Block flag bit: the block has entries in the finest level's extra exception list (bits 0-1 of
@@ -183,8 +188,13 @@ __device__ __forceinline__ void record_exception(const StepArgs<T, S> &a, long l
         a.exc_count[1] = 1;
     }
     if (a.vmode == 1) {
-        atomicOr(&a.flags[tgt >> 6], PEND_FLAG);
-        atomicOr(&a.flags[cell >> 6], PEND_FLAG);
+        const int bt = (int)(tgt >> 6), bc = (int)(cell >> 6);
+        atomicOr(&a.flags[bt], PEND_FLAG);
+        atomicOr(&a.flags[bc], PEND_FLAG);
+        if (k < a.exc_cap) {
+            a.x_next[2 * k] = atomicExch(&a.x_head[bt], 2 * k);
+            if (bc != bt) a.x_next[2 * k + 1] = atomicExch(&a.x_head[bc], 2 * k + 1);
+        }
     }
 }
 
@@ -656,50 +666,41 @@ __device__ __forceinline__ void push_simple(const StepArgs<T, S> &a, const T *fs
 // The previous sub-step's output is this sub-step's input, read by no one else in between, so
 // the same values are taken here as the loads' results instead.  No two entries share a slot (an
 // entry's q is the partner's, non-initiating direction and pb(q) an initiating one), so the order
-// of application is free.  The wavefront scans the list 64 entries at a time, one per lane, and
-// applies the hits on block b one by one; only flagged blocks scan.
+// of application is free.  The wavefront walks its block's chain (the entries with i or c in the
+// block, linked when they were recorded), so the work per block is its own entries however long
+// the list is; the chain is emptied for the next pair.
 template <typename T, typename S, int Q>
 __device__ __forceinline__ void apply_pending(const StepArgs<T, S> &a, T *f, int b, int t)
 {
     using VS = VelSet<Q>;
     using ST = Store<T, S>;
-    const int n = (int)min((long long)a.va_count[0], a.exc_cap);
     const unsigned my = ((unsigned)b << 6) | (unsigned)t;
-    for (int k0 = 0; k0 < n; k0 += 64) {
-        const int k = k0 + t;
-        unsigned i = ~0u, c = ~0u;
-        int q = 0;
-        if (k < n) {
-            i = (unsigned)a.va_idx[3 * (long long)k + 0];
-            c = (unsigned)a.va_idx[3 * (long long)k + 1];
-            q = a.va_idx[3 * (long long)k + 2];
-        }
-        unsigned long long hits = __ballot((i >> 6) == (unsigned)b || (c >> 6) == (unsigned)b);
-        while (hits) {
-            const int l = __builtin_ctzll(hits);
-            hits &= hits - 1;
-            const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)i, l);
-            const unsigned hc = (unsigned)__builtin_amdgcn_readlane((int)c, l);
-            const int hq = __builtin_amdgcn_readlane(q, l);
-            int qb = 0;
-            T wq = 0;
+    int e = __builtin_amdgcn_readfirstlane(a.x_head[b]);
+    while (e >= 0) {
+        const int k = e >> 1;
+        const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane(a.va_idx[3 * (long long)k + 0]);
+        const unsigned hc = (unsigned)__builtin_amdgcn_readfirstlane(a.va_idx[3 * (long long)k + 1]);
+        const int hq = __builtin_amdgcn_readfirstlane(a.va_idx[3 * (long long)k + 2]);
+        int qb = 0;
+        T wq = 0;
 #pragma unroll
-            for (int p = 0; p < Q; p++)
-                if (p == hq) {
-                    qb = VS::pb(p);
-                    wq = (T)VS::w(p);
-                }
-            // Through storage, as the fixup's store and this sub-step's load would take them.
-            const T vi = ST::dec(a.A[(long long)qb * a.ncm + hc], wq);
-            const T vc = ST::dec(ST::enc(a.va_val[k0 + l], wq), wq);
-            const bool at_i = hi == my, at_c = hc == my;
-#pragma unroll
-            for (int p = 1; p < Q; p++) {
-                if (at_i && p == hq) f[p] = vi;
-                if (at_c && p == qb) f[p] = vc;
+        for (int p = 0; p < Q; p++)
+            if (p == hq) {
+                qb = VS::pb(p);
+                wq = (T)VS::w(p);
             }
+        // Through storage, as the fixup's store and this sub-step's load would take them.
+        const T vi = ST::dec(a.A[(long long)qb * a.ncm + hc], wq);
+        const T vc = ST::dec(ST::enc(a.va_val[k], wq), wq);
+        const bool at_i = hi == my, at_c = hc == my;
+#pragma unroll
+        for (int p = 1; p < Q; p++) {
+            if (at_i && p == hq) f[p] = vi;
+            if (at_c && p == qb) f[p] = vc;
         }
+        e = __builtin_amdgcn_readfirstlane(a.x_next[e]);
     }
+    if (t == 0) a.x_head[b] = -1;
 }
 
 // 3D: one wavefront per block, so the block, its flags and its 27 neighbour IDs are wave-uniform
@@ -1994,6 +1995,8 @@ static StepArgs<T, S> step_args(Ctx *c, int L, int rd, int vmode = 0)
     a.va_idx = c->exc_idx + (size_t)3 * c->exc_cap * XL;
     a.va_val = static_cast<T *>(c->exc_val) + (size_t)c->exc_cap * XL;
     a.va_count = c->exc_count + 2 * XL;
+    a.x_head = c->exc_head;
+    a.x_next = c->exc_next;
     if (vmode == 1) {
         a.exc_idx = c->exc_idx + (size_t)3 * c->exc_cap * XL;
         a.exc_val = static_cast<T *>(c->exc_val) + (size_t)c->exc_cap * XL;
