@@ -13,7 +13,7 @@ Phases (the mesh is the one the flow produces, independent of --warmup):
   2. warm-up: `--warmup` coarse steps on the developed mesh, untimed;
   3. roofline pass: `--roofline-steps` coarse steps with HIP events around every launch of the
      fused collide + boundary + stream kernel (direct launches on the context stream);
-  4. timed window: `--steps` coarse steps of the driver (recorded HIP graphs), HIP events on the
+  4. timed window: `--steps` coarse steps of the advance loop (recorded HIP graphs), HIP events on the
      context stream; a rebuild that falls inside the window is timed separately and the cell
      updates are counted on the mesh of each stretch between rebuilds.
 
@@ -109,7 +109,7 @@ def leaf_updates_per_step(n_ids):
 
 
 def develop(m, n, what):
-    """n coarse steps of the driver from rest (rebuilds included); returns the next step index."""
+    """n coarse steps of the advance loop from rest (rebuilds included); returns the next step index."""
     t0 = time.time()
     done = 0
     while done < n:
@@ -122,7 +122,7 @@ def develop(m, n, what):
 
 
 def timed_window(m, i0, K):
-    """K coarse steps of the driver from step index i0, split at rebuilds; returns step and rebuild
+    """K coarse steps of the advance loop from step index i0, split at rebuilds; returns step and rebuild
     times and the cell updates counted per stretch on its own mesh."""
     P = m.params.P_REFINE
     i, left = i0, K

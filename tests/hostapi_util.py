@@ -1,4 +1,4 @@
-"""Build and run tests/cpp/hostapi_check.cpp, the driver of the header-library host API."""
+"""Build and run tests/cpp/hostapi_check.cpp, the test program of the header-library host API."""
 from __future__ import annotations
 
 import base64
