@@ -7,7 +7,7 @@ per level) and re-evaluated every 100 coarse steps, Re 1000, lid velocity 0.05, 
 structure-of-arrays layout.
 
 Phases (the mesh is the one the flow produces, independent of --warmup):
-  1. development: `--develop` coarse steps of the real driver (M_AdvanceLoop: steps plus the
+  1. development: `--develop` coarse steps of the advance loop (M_AdvanceLoop: steps plus the
      solution-based rebuilds every 100 steps) from rest; the run stops with an error if the finer
      levels stay empty;
   2. warm-up: `--warmup` coarse steps on the developed mesh, untimed;
@@ -19,6 +19,10 @@ Phases (the mesh is the one the flow produces, independent of --warmup):
 
 value: actual cell updates per second (every active block, 2^L sub-steps on level L) in MLUPS.
 roofline: the fused kernel, algorithmic bytes 2*Q*4 per cell update / its average launch time.
+--copy-calibration runs only a plain torch device copy (its own process: torch brings its own HIP
+runtime): three 1 GiB copies for the byte-counter calibration, then the copy rate (bytes read +
+written per second) for 256 MiB and 1 GiB sources, the practical HBM ceiling to set beside the
+fused kernel's rate.
 cpu_baseline: the CPU restatement (the oracle, OpenMP over blocks) stepping a copy of the
 developed mesh on all host cores (up to 16), plus a serial leg.
 
@@ -80,7 +84,8 @@ def parse():
                     help="population storage of the cavity leg (fp32 is the headline; the others are for profiles)")
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="run only a 1 GiB device copy (torch) to calibrate HBM byte counters")
+                    help="run only a plain device copy (torch): 1 GiB copies to calibrate HBM byte counters, then "
+                         "the copy rate of 256 MiB and 1 GiB sources")
     return ap.parse_args()
 
 
@@ -264,15 +269,33 @@ def headline(args):
 def main():
     args = parse()
     if args.copy_calibration:
+        # Three 1 GiB copies (the counter calibration: 1 GiB read + 1 GiB written each), then timed
+        # copies (events on torch's stream) for the copy rate.
         import torch
-        n = 1 << 28  # 2^28 float32 = 1 GiB read + 1 GiB written
+        n = 1 << 28  # 2^28 float32 = 1 GiB
         a = torch.ones(n, device="cuda:0")
         b = torch.empty_like(a)
         torch.cuda.synchronize()
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}))
+        del a, b
+        rates = {}
+        for label, nbytes in (("256MiB", 1 << 28), ("1GiB", 1 << 30)):
+            a = torch.ones(nbytes // 4, device="cuda:0")
+            b = torch.empty_like(a)
+            for _ in range(3):
+                b.copy_(a)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(20):
+                b.copy_(a)
+            e1.record()
+            torch.cuda.synchronize()
+            rates[label] = round(2 * nbytes / (e0.elapsed_time(e1) / 20 * 1e-3) / 1e9, 1)
+            del a, b
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n,
+                          "copy_gbs": rates}))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU partitioning is "
