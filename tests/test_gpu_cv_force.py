@@ -63,16 +63,19 @@ def test_cv_periodic_single_level(vs, prec):
     o.close()
 
 
-@pytest.mark.parametrize("vs,N,prec", [(capi.D3Q19, 16, capi.FP32), (capi.D3Q19, 16, capi.FP64),
-                                       (capi.D2Q9, 32, capi.FP32), (capi.D3Q27, 16, capi.FP16)])
-def test_cv_adaptive_three_levels(vs, N, prec):
+@pytest.mark.parametrize("vs,N,prec,graphs", [(capi.D3Q19, 16, capi.FP32, 1), (capi.D3Q19, 16, capi.FP64, 1),
+                                              (capi.D2Q9, 32, capi.FP32, 1), (capi.D3Q27, 16, capi.FP16, 1),
+                                              (capi.D3Q19, 16, capi.FP32, 0)])
+def test_cv_adaptive_three_levels(vs, N, prec, graphs):
     """Lid-driven cavity, three levels rebuilt from the solution every 15 coarse steps, the box
     holding the lid rows: the first half on every level, the full average, the interface
-    interpolations and the second half after each coarse step, through recorded step graphs."""
-    print(f"cv adaptive three levels: Q {vs} N {N} precision {prec}", flush=True)
+    interpolations and the second half after each coarse step, through recorded step graphs (and
+    once with direct launches)."""
+    print(f"cv adaptive three levels: Q {vs} N {N} precision {prec} graphs {graphs}", flush=True)
     box = dict(BOX, S_FORCEVOLUME_YM=1.0) if vs == capi.D2Q9 else dict(BOX, S_FORCEVOLUME_ZM=1.0)
     p = ldc_params(vs=vs, N=N, Re=40.0, levels=3, P_REFINE=15, prec=prec, S_FORCE_TYPE=1, **box)
     g, o = LbmMesh(p, 8192), OracleMesh(p, 8192)
+    g.set_option(capi.OPT_GRAPHS, graphs)
     for m in (g, o):
         m.S_SetIC(0)
         m.M_FreezeRefinedCells(0)
