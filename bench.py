@@ -21,8 +21,8 @@ value: actual cell updates per second (every active block, 2^L sub-steps on leve
 roofline: the fused kernel, algorithmic bytes 2*Q*4 per cell update / its average launch time.
 --copy-calibration runs only a plain torch device copy (its own process: torch brings its own HIP
 runtime): three 1 GiB copies for the byte-counter calibration, then the copy rate (bytes read +
-written per second) for 256 MiB and 1 GiB sources, the practical HBM ceiling to set beside the
-fused kernel's rate.
+written per second) for 256 MiB, 1 GiB and 4 GiB sources, the practical HBM ceiling to set beside
+the fused kernel's rate.
 cpu_baseline: the CPU restatement (the oracle, OpenMP over blocks) stepping a copy of the
 developed mesh on all host cores (up to 16), plus a serial leg.
 
@@ -85,7 +85,7 @@ def parse():
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy (torch): 1 GiB copies to calibrate HBM byte counters, then "
-                         "the copy rate of 256 MiB and 1 GiB sources")
+                         "the copy rate of 256 MiB, 1 GiB and 4 GiB sources")
     return ap.parse_args()
 
 
@@ -281,7 +281,7 @@ def main():
         torch.cuda.synchronize()
         del a, b
         rates = {}
-        for label, nbytes in (("256MiB", 1 << 28), ("1GiB", 1 << 30)):
+        for label, nbytes in (("256MiB", 1 << 28), ("1GiB", 1 << 30), ("4GiB", 1 << 32)):
             a = torch.ones(nbytes // 4, device="cuda:0")
             b = torch.empty_like(a)
             for _ in range(3):
